@@ -1,0 +1,152 @@
+"""ctypes binding of libehss_hip.so (the C-ABI declared in include/ehss_hip.h).
+
+The library is loaded from the package directory.  There is no fallback: if the
+shared object is missing or no HIP device is present, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libehss_hip.so"
+
+EHSS_OK = 0
+EHSS_ERR_INVALID = 1
+EHSS_ERR_HIP = 2
+EHSS_ERR_NOMEM = 3
+EHSS_ERR_ADMISSION = 4
+EHSS_ERR_BUSY = 5
+
+PRECISION_FP64 = 0
+PRECISION_FP32_FIXUP = 1
+PRECISIONS = {"fp64": PRECISION_FP64, "fp32-fixup": PRECISION_FP32_FIXUP}
+
+FLAG_PA = 1
+FLAG_KEEP_RAYS = 2
+
+RAY_ESCAPED = 0x1
+RAY_UNRESOLVED = 0x2
+RAY_PA_HIT = 0x4
+RAY_EHSS_HIT = 0x8
+RAY_GRAZING = 0x10
+
+_P = ct.c_void_p
+_I64 = ct.c_int64
+_INT = ct.c_int
+
+
+class EHSSError(RuntimeError):
+    """A libehss_hip call returned a nonzero status."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[status {status}] {message}")
+        self.status = status
+
+
+class AdmissionError(EHSSError):
+    """The request does not fit the configured device memory budget."""
+
+
+class Sobol(ct.Structure):
+    _fields_ = [("sv", (ct.c_uint32 * 32) * 4), ("shift", ct.c_uint32 * 4)]
+
+
+class SceneInfo(ct.Structure):
+    _fields_ = [("atoms", ct.c_int64), ("groups", ct.c_int64), ("nodes", ct.c_int64), ("pa_nodes", ct.c_int64),
+                ("leaf_size", ct.c_int32), ("max_depth", ct.c_int32), ("center", ct.c_double * 3),
+                ("radius", ct.c_double), ("scale", ct.c_double), ("device_bytes", ct.c_int64)]
+
+
+class Totals(ct.Structure):
+    _fields_ = [("omega", ct.c_double), ("omega_err", ct.c_double), ("pa", ct.c_double),
+                ("pa_from_ehss", ct.c_double), ("residual_mass", ct.c_double), ("tail_bound", ct.c_double),
+                ("weight", ct.c_double), ("rays", ct.c_int64), ("escaped", ct.c_int64),
+                ("unresolved", ct.c_int64), ("pa_hits", ct.c_int64), ("ehss_hits", ct.c_int64),
+                ("box_tests", ct.c_int64), ("sphere_tests", ct.c_int64), ("pa_box_tests", ct.c_int64),
+                ("pa_sphere_tests", ct.c_int64), ("fp64_retests", ct.c_int64), ("grazing_rays", ct.c_int64),
+                ("max_bounces", ct.c_int64), ("kernel_ms", ct.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+# Signatures: name -> (restype, argtypes)
+_SIGS = {
+    "ehss_abi_version": (_INT, []),
+    "ehss_last_error": (_INT, [ct.c_char_p, ct.c_size_t]),
+    "ehss_init": (_INT, [_INT]),
+    "ehss_device_info": (_INT, [ct.c_char_p, ct.c_size_t, _P, _P, _P, _P]),
+    "ehss_selftest_fp64": (_INT, [_I64, _P, _P, _P, _P]),
+    "ehss_trace": (None, [_I64, _I64, _INT, _INT, _INT] + [_P] * 17),
+    "ehss_trace_status": (_INT, [_I64, _I64, _INT, _INT, _INT] + [_P] * 16),
+    "ehss_scene_create": (_INT, [_I64, _P, _P, _INT, ct.POINTER(_P)]),
+    "ehss_scene_destroy": (_INT, [_P]),
+    "ehss_scene_get_info": (_INT, [_P, ct.POINTER(SceneInfo)]),
+    "ehss_scene_export_tree": (_INT, [_P] + [_P] * 7),
+    "ehss_scene_export_pa_tree": (_INT, [_P] + [_P] * 6),
+    "ehss_scene_trace": (_INT, [_P, _I64, _INT, _INT, _INT] + [_P] * 8),
+    "ehss_scene_pa_query": (_INT, [_P, _I64, _INT] + [_P] * 5),
+    "ehss_solve": (_INT, [_P, ct.POINTER(Sobol), _INT, _INT, _INT, ct.c_uint, ct.POINTER(Totals), _P, _P]),
+    "ehss_scene_last_rays": (_INT, [_P, _I64, _P, _P, _P]),
+    "ehss_source_rays": (_INT, [_P, ct.POINTER(Sobol), _INT, _P, _P, _P]),
+}
+
+_lib = None
+
+
+def lib() -> ct.CDLL:
+    """Load libehss_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m ehss_amd.build`")
+        handle = ct.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(handle, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def fn(name: str):
+    f = getattr(lib(), name, None)
+    if f is None:
+        raise RuntimeError(f"libehss_hip.so does not export {name}")
+    return f
+
+
+def last_error() -> str:
+    buf = ct.create_string_buffer(2048)
+    lib().ehss_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(status: int) -> None:
+    if status != EHSS_OK:
+        msg = last_error()
+        if status == EHSS_ERR_ADMISSION:
+            raise AdmissionError(status, msg)
+        raise EHSSError(status, msg)
+
+
+def call(name: str, *args) -> None:
+    check(fn(name)(*args))
+
+
+def ptr(a: np.ndarray | None) -> ct.c_void_p:
+    return ct.c_void_p(0 if a is None else a.ctypes.data)
+
+
+def device_info() -> dict:
+    name = ct.create_string_buffer(256)
+    cu, arch_major, arch_minor = ct.c_int(), ct.c_int(), ct.c_int()
+    mem = ct.c_int64()
+    call("ehss_device_info", name, len(name), ct.byref(cu), ct.byref(mem), ct.byref(arch_major),
+         ct.byref(arch_minor))
+    return dict(name=name.value.decode(), compute_units=cu.value, total_mem=mem.value,
+                arch=f"{arch_major.value}.{arch_minor.value}")

@@ -1,0 +1,166 @@
+// ehss_device.h -- device arithmetic shared by the gfx950 kernels.
+//
+// The FP64 functions restate the reference scalar kernel operation for
+// operation (src/native/ehss_cpu.cpp:7-40).  The whole library is compiled with
+// -ffp-contract=off, and IEEE division and square root are used, so these
+// return the same bits as the reference on the same inputs.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace ehss {
+
+constexpr double kEps64 = 2.220446049250313e-16;  // DBL_EPSILON
+constexpr float kU32 = 5.9604645e-08f;            // 2^-24, FP32 unit roundoff
+constexpr int kInfo = 0;
+
+// std::max / std::min semantics: the first argument wins unless strictly beaten.
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+
+struct V3 {
+    double x, y, z;
+};
+
+// Static EHSS tree node (FP64), reference layout (ehss_static.py:84-90):
+// padded box, preorder skip pointer end[node], leaf group (or -1).
+struct alignas(64) Node64 {
+    double lo[3];
+    double hi[3];
+    int32_t end;
+    int32_t leaf;
+    int32_t pad[2];
+};
+
+// FP32 filter node: outward-rounded box inflated by the scene's filter margin.
+struct alignas(32) Node32 {
+    float lo[3];
+    float hi[3];
+    int32_t end;
+    int32_t leaf;
+};
+
+// PA any-hit tree node (pa_spatial.py:17-28): unpadded box, skip pointer,
+// first id slot and count (count 0 for internal nodes).
+struct alignas(64) NodePA {
+    double lo[3];
+    double hi[3];
+    int32_t end;
+    int32_t first;
+    int32_t count;
+    int32_t pad;
+};
+
+// Per-query grazing bookkeeping: a decision is "grazing" when a relative
+// perturbation of order kGrazeTol could flip it.
+constexpr double kGrazeTol = 1e-9;
+
+// src/native/ehss_cpu.cpp:7-17.  Also returns, through *graze_t, the
+// along-ray distance of a near-threshold decision (or +inf when none):
+//   disc within kGrazeTol*r^2 of 0 (tangency), projection within
+//   kGrazeTol*|x| of 0, or near within kGrazeTol*r of the -64 eps r snap.
+__device__ __forceinline__ double entry64(const V3& o, const V3& d, double cx, double cy, double cz, double r,
+                                          double* graze_t) {
+    double x0 = o.x - cx, x1 = o.y - cy, x2 = o.z - cz;
+    double aa = (d.x * d.x + d.y * d.y) + d.z * d.z;
+    double projection = ((x0 * d.x + x1 * d.y) + x2 * d.z) / aa;
+    double p0 = x0 - projection * d.x, p1 = x1 - projection * d.y, p2 = x2 - projection * d.z;
+    double disc = r * r - ((p0 * p0 + p1 * p1) + p2 * p2);
+    double rr = r * r;
+    bool tangent = fabs(disc) <= kGrazeTol * rr;
+    if (disc < 0. || projection >= 0.) {
+        if (tangent && projection < 0.) *graze_t = fmin(*graze_t, -projection);
+        if (projection >= 0. && projection <= kGrazeTol * r && disc >= 0.) *graze_t = 0.;
+        return INFINITY;
+    }
+    double near = -projection - sqrt(smax(0., disc) / aa);
+    double snap = -64 * kEps64 * r;
+    if (tangent) *graze_t = fmin(*graze_t, -projection);
+    if (fabs(near - snap) <= kGrazeTol * r) *graze_t = 0.;
+    if (near < snap) return INFINITY;
+    return smax(0., near);
+}
+
+// src/native/ehss_cpu.cpp:19-29
+__device__ __forceinline__ bool box64(const V3& o, const V3& d, const double* lo, const double* hi,
+                                      double best) {
+    double near = 0., far = INFINITY;
+    const double oo[3] = {o.x, o.y, o.z};
+    const double dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (dd[c] == 0.) {
+            if (oo[c] < lo[c] || oo[c] > hi[c]) return false;
+        } else {
+            double a = (lo[c] - oo[c]) / dd[c], b = (hi[c] - oo[c]) / dd[c];
+            near = smax(near, smin(a, b));
+            far = smin(far, smax(a, b));
+        }
+    }
+    return far >= near && near <= best && far >= 0.;
+}
+
+// src/native/ehss_cpu.cpp:31-40 (in place on o, d).
+__device__ __forceinline__ void reflect64(V3& o, V3& d, double cx, double cy, double cz, double r,
+                                          double distance) {
+    double n0 = o.x + distance * d.x - cx, n1 = o.y + distance * d.y - cy, n2 = o.z + distance * d.z - cz;
+    double norm = sqrt((n0 * n0 + n1 * n1) + n2 * n2);
+    n0 /= norm;
+    n1 /= norm;
+    n2 /= norm;
+    o.x = cx + r * n0;
+    o.y = cy + r * n1;
+    o.z = cz + r * n2;
+    double dot = (d.x * n0 + d.y * n1) + d.z * n2;
+    d.x = d.x - 2 * dot * n0;
+    d.y = d.y - 2 * dot * n1;
+    d.z = d.z - 2 * dot * n2;
+    norm = sqrt((d.x * d.x + d.y * d.y) + d.z * d.z);
+    d.x /= norm;
+    d.y /= norm;
+    d.z /= norm;
+}
+
+// Nearest-atom update rule of src/native/ehss_cpu.cpp:67 (tie: lower id wins).
+__device__ __forceinline__ bool better(double distance, int32_t wanted, double best, int32_t atom) {
+    return distance < best || (distance == best && isfinite(distance) && (atom < 0 || wanted < atom));
+}
+
+// inside_out_pa.py:93-101
+__device__ __forceinline__ bool sphere_hit64(const V3& o, const V3& d, double cx, double cy, double cz, double r) {
+    double x = o.x - cx, y = o.y - cy, z = o.z - cz;
+    double b = (x * d.x + y * d.y) + z * d.z;
+    double cc = ((x * x + y * y) + z * z) - r * r;
+    double disc = b * b - cc;
+    return disc >= 0 && -b + sqrt(smax(0., disc)) > 0;
+}
+
+// pa_spatial.py:98-111
+__device__ __forceinline__ bool box_hit64(const V3& o, const V3& d, const double* lo, const double* hi) {
+    double near = 0., far = INFINITY;
+    const double oo[3] = {o.x, o.y, o.z};
+    const double dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+        if (dd[axis] == 0) {
+            if (oo[axis] < lo[axis] || oo[axis] > hi[axis]) return false;
+        } else {
+            double a = (lo[axis] - oo[axis]) / dd[axis], b = (hi[axis] - oo[axis]) / dd[axis];
+            near = smax(near, smin(a, b));
+            far = smin(far, smax(a, b));
+            if (far < near) return false;
+        }
+    }
+    return true;
+}
+
+// Wave-level helpers (wave64).
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace ehss

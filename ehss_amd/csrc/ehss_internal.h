@@ -1,0 +1,110 @@
+// ehss_internal.h -- host-side declarations shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "ehss_device.h"
+#include "../../include/ehss_hip.h"
+
+namespace ehss {
+
+// ---------------------------------------------------------------------------
+// Errors: every entry point returns a status; the message is thread-local.
+// ---------------------------------------------------------------------------
+int set_error(int code, const char* fmt, ...);
+int hip_error(hipError_t e, const char* what, const char* file, int line);
+
+#define EHSS_HIP(expr)                                                       \
+    do {                                                                     \
+        hipError_t e_ = (expr);                                              \
+        if (e_ != hipSuccess) return ::ehss::hip_error(e_, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+#define EHSS_CHECK(cond, ...)                                         \
+    do {                                                              \
+        if (!(cond)) return ::ehss::set_error(EHSS_ERR_INVALID, __VA_ARGS__); \
+    } while (0)
+
+#define EHSS_TRY(expr)            \
+    do {                          \
+        int s_ = (expr);          \
+        if (s_ != EHSS_OK) return s_; \
+    } while (0)
+
+int ensure_init();           // selects the device and creates the library stream
+hipStream_t lib_stream();
+
+// Owning device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int alloc(size_t n);     // exact size; releases the previous allocation
+    int reserve(size_t n);   // grow-only
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// ---------------------------------------------------------------------------
+// Device view of a scene (everything the kernels read).
+// ---------------------------------------------------------------------------
+struct SceneView {
+    int32_t n_atoms = 0, n_groups = 0, n_nodes = 0, pa_nodes = 0;
+    const Node64* nodes = nullptr;    // static tree, FP64 (reference-identical)
+    const Node32* nodes32 = nullptr;  // static tree, FP32 filter boxes
+    const int32_t* offsets = nullptr; // group -> first slot
+    const int32_t* ids = nullptr;     // slot -> atom
+    const double4* sph = nullptr;     // atom -> {x, y, z, r}, FP64
+    const float4* sph32 = nullptr;    // slot -> {x - Cx, y - Cy, z - Cz, r}, FP32
+    const NodePA* pa = nullptr;       // PA tree
+    const int32_t* pa_ids = nullptr;  // PA id slots
+    double cx = 0, cy = 0, cz = 0;    // filter origin (scene centre)
+    double scale = 1;                 // bound on |coordinates - centre| and radii
+    float m_box = 0, m_p = 0, m_disc = 0;  // FP32 filter margins
+};
+
+// Filter margins for a scale s (FP32 unit roundoff u = 2^-24):
+// box inflation and projection margin 64 u s, discriminant margin 1024 u s^2.
+void set_filter_margins(SceneView& v);
+
+// Builds the FP32 filter arrays (nodes32, sph32) from the FP64 ones, on the device.
+int launch_make_filter(const SceneView& v, Node32* nodes32, float4* sph32, hipStream_t s);
+
+// Per-ray trace of caller rays (AoS p, d as in ehss_trace).
+struct TraceIO {
+    int64_t nr = 0;
+    int32_t cap = 0;
+    int32_t record = 0;
+    double* p = nullptr;
+    double* d = nullptr;
+    const int64_t* lasts = nullptr;
+    int64_t* counts = nullptr;
+    uint8_t* escaped = nullptr;
+    uint8_t* unresolved = nullptr;
+    int32_t* history = nullptr;
+    unsigned long long* counters = nullptr;  // [boxes, spheres, fp64 retests]
+};
+int launch_trace(const SceneView& v, const TraceIO& io, int precision, hipStream_t s);
+
+struct PAIO {
+    int64_t nr = 0;
+    const double* o = nullptr;
+    const double* d = nullptr;
+    const int64_t* owners = nullptr;
+    uint8_t* hits = nullptr;
+    unsigned long long* counters = nullptr;  // [boxes, tests]
+};
+int launch_pa_query(const SceneView& v, const PAIO& io, hipStream_t s);
+
+}  // namespace ehss

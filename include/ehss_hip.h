@@ -1,0 +1,173 @@
+/*
+ * ehss_hip.h -- C-ABI of libehss_hip.so, the MI355X (gfx950) EHSS/PA engine.
+ *
+ * Plain C types only: pointers, sizes, status codes.  Every entry point except
+ * ehss_trace returns an int status (EHSS_OK == 0); the message of the last
+ * failure on the calling thread is available from ehss_last_error().
+ *
+ * Host arrays are owned by the caller; the library never retains a caller
+ * pointer after a call returns.  Device memory is owned by the library
+ * (scene handles, request handles, workspaces).
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   ehss_trace               src/native/ehss_cpu.cpp:42-46 (identical signature),
+ *                            bound by ehss_cpu_cpp.py:13-16.
+ *   ehss_scene_create        PreparedStaticCPUEHSS.__init__ (ehss_static.py:62-101)
+ *                            plus SphereIndex.__init__ (pa_spatial.py:157-160).
+ *   ehss_scene_trace         CPUEHSSTransport.trace (ehss_cpu.py:118-142).
+ *   ehss_scene_pa_query      SphereIndex.query / tree_blocked (pa_spatial.py:133-142,162-164).
+ *   ehss_solve               one replicate of solve_frame (ccs_cli.py:196-219):
+ *                            external_source (ehss_state.py:42-52) -> index.query ->
+ *                            engine.trace -> EHSSResult readout (ehss_state.py:84-103).
+ *   ehss_batch_*             no single reference counterpart: many solve_frame
+ *                            replicates over many structures in one launch.
+ */
+#ifndef EHSS_HIP_H
+#define EHSS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EHSS_ABI_VERSION 1
+
+/* Status codes. */
+#define EHSS_OK 0
+#define EHSS_ERR_INVALID 1   /* bad argument; nothing was launched */
+#define EHSS_ERR_HIP 2       /* HIP runtime or kernel failure */
+#define EHSS_ERR_NOMEM 3     /* device or host allocation failed */
+#define EHSS_ERR_ADMISSION 4 /* request does not fit the configured device budget */
+#define EHSS_ERR_BUSY 5      /* request still running (ehss_request_poll) */
+
+/* Precision / kernel selectors. */
+#define EHSS_PRECISION_FP64 0      /* plain FP64 kernel, reference operation order */
+#define EHSS_PRECISION_FP32_FIXUP 1 /* FP32 filter + FP64 re-test; identical results */
+
+/* ehss_solve flags. */
+#define EHSS_FLAG_PA 1u        /* also run the BVH any-hit PA on the same rays */
+#define EHSS_FLAG_KEEP_RAYS 2u /* keep per-ray records for ehss_request_rays */
+
+/* Library / device. */
+int ehss_abi_version(void);
+int ehss_last_error(char* buf, size_t size);
+int ehss_init(int device);
+int ehss_device_info(char* name, size_t name_size, int* cu_count, int64_t* total_mem, int* arch_major,
+                     int* arch_minor);
+/* Device FP64 arithmetic check (no reference counterpart): quotient[i] = a[i] / b[i]
+ * and root[i] = sqrt(a[i]) computed by a kernel, for comparison with host IEEE. */
+int ehss_selftest_fp64(int64_t n, const double* a, const double* b, double* quotient, double* root);
+
+/* ---------------------------------------------------------------------------
+ * Reference seam.  Exact signature and array contract of ehss_trace at
+ * src/native/ehss_cpu.cpp:42-46: all buffers are caller-owned host arrays;
+ * p, d, counts are updated in place; escaped/unresolved must be zeroed by the
+ * caller; history (nr x cap int32, -1 filled) is written only when record != 0;
+ * counters[0..1] receive box and sphere tests.  Runs the plain FP64 kernel on
+ * the current device.  `threads` is accepted for ABI compatibility and echoed in
+ * *observed_threads; on failure *observed_threads = -1 and counters = -1, and
+ * ehss_last_error() holds the reason.
+ * ------------------------------------------------------------------------- */
+void ehss_trace(int64_t nr, int64_t nn, int cap, int threads, int record, double* p, double* d,
+                const int64_t* lasts, int64_t* counts, uint8_t* escaped, uint8_t* unresolved, int32_t* history,
+                const double* lo, const double* hi, const int64_t* end, const int64_t* leaf,
+                const int64_t* offsets, const int64_t* ids, const double* world, const double* radii,
+                int64_t* counters, int* observed_threads);
+
+/* ehss_trace with a status return and a precision selector. */
+int ehss_trace_status(int64_t nr, int64_t nn, int cap, int precision, int record, double* p, double* d,
+                      const int64_t* lasts, int64_t* counts, uint8_t* escaped, uint8_t* unresolved,
+                      int32_t* history, const double* lo, const double* hi, const int64_t* end,
+                      const int64_t* leaf, const int64_t* offsets, const int64_t* ids, const double* world,
+                      const double* radii, int64_t* counters);
+
+/* ---------------------------------------------------------------------------
+ * Device-resident scenes: spheres + static EHSS tree + PA tree, built on the GPU.
+ * ------------------------------------------------------------------------- */
+typedef struct ehss_scene ehss_scene;
+
+typedef struct {
+    int64_t atoms, groups, nodes, pa_nodes;
+    int32_t leaf_size, max_depth;
+    double center[3]; /* centers.mean(axis=0) (ehss_state.py:46) */
+    double radius;    /* max(|c_i - center| + r_i) * (1 + 1e-9) (ehss_state.py:47) */
+    double scale;     /* FP32 filter scale: bound on |coordinate - center| */
+    int64_t device_bytes;
+} ehss_scene_info;
+
+/* centers: n x 3 (row-major), radii: n; leaf_size >= 1.  Builds on the device. */
+int ehss_scene_create(int64_t n, const double* centers, const double* radii, int leaf_size, ehss_scene** out);
+int ehss_scene_destroy(ehss_scene* scene);
+int ehss_scene_get_info(const ehss_scene* scene, ehss_scene_info* info);
+/* Copies the static tree back (arrays sized from ehss_scene_get_info):
+ * lo, hi: nodes x 3; end, leaf_group: nodes; offsets: groups + 1; ids, groups: atoms. */
+int ehss_scene_export_tree(const ehss_scene* scene, double* lo, double* hi, int64_t* end, int64_t* leaf_group,
+                           int64_t* offsets, int64_t* ids, int64_t* groups);
+/* PA tree (pa_spatial.py:17-28 layout): lo, hi: pa_nodes x 3; end, first, count: pa_nodes; ids: atoms. */
+int ehss_scene_export_pa_tree(const ehss_scene* scene, double* lo, double* hi, int64_t* end, int64_t* first,
+                              int64_t* count, int64_t* ids);
+
+/* Traces caller rays (host arrays, ehss_trace contract) against a scene. */
+int ehss_scene_trace(ehss_scene* scene, int64_t nr, int cap, int precision, int record, double* p, double* d,
+                     const int64_t* lasts, int64_t* counts, uint8_t* escaped, uint8_t* unresolved,
+                     int32_t* history, int64_t* counters);
+/* Any-hit PA on caller rays (SphereIndex.query contract): hits[nr] (0/1),
+ * counters[0..1] = box tests, sphere tests on the PA tree. */
+int ehss_scene_pa_query(ehss_scene* scene, int64_t nr, int precision, const double* origins,
+                        const double* directions, const int64_t* owners, uint8_t* hits, int64_t* counters);
+
+/* ---------------------------------------------------------------------------
+ * Fused solve: scrambled Sobol -> impact-disk source -> [PA any-hit] -> trace ->
+ * FP64 compensated (1 - cos chi) reduction, all on the device.
+ * ------------------------------------------------------------------------- */
+
+/* Scrambled direction numbers and digital shift of a 4-D, 30-bit Sobol net
+ * (SciPy qmc.Sobol LMS + shift); sv[dim][j] is the number for index bit j. */
+typedef struct {
+    uint32_t sv[4][32];
+    uint32_t shift[4];
+} ehss_sobol;
+
+typedef struct {
+    double omega;          /* sum w (1 - u.v) over escaped rays, A^2 */
+    double omega_err;      /* compensation residual of the FP64 sum (diagnostic) */
+    double pa;             /* BVH any-hit PA: w * #hits (EHSS_FLAG_PA) */
+    double pa_from_ehss;   /* w * #(bounces > 0 or unresolved) (ehss_state.py:101-103) */
+    double residual_mass;  /* w * #unresolved (ehss_state.py:93-95) */
+    double tail_bound;     /* 2 * residual_mass (ehss_state.py:97-99) */
+    double weight;         /* w = pi R^2 / n */
+    int64_t rays, escaped, unresolved, pa_hits, ehss_hits;
+    int64_t box_tests, sphere_tests, pa_box_tests, pa_sphere_tests;
+    int64_t fp64_retests;  /* FP32 path: exact FP64 re-evaluations */
+    int64_t grazing_rays;  /* rays flagged near a hit/miss or nearest-atom decision */
+    int64_t max_bounces;
+    double kernel_ms;      /* device time of the trace kernel(s) (HIP events) */
+} ehss_totals;
+
+/* Synchronous solve of 2^power rays.  histogram (cap+1 int64) and ledger
+ * (cap+1 double) may be NULL. */
+int ehss_solve(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, int precision, unsigned flags,
+               ehss_totals* totals, int64_t* histogram, double* ledger);
+
+/* Per-ray records of the last ehss_solve on this scene made with
+ * EHSS_FLAG_KEEP_RAYS, in ray-index order: info bits (see below), angle
+ * (1 - u.v clipped to [0,2], 0 unless escaped), bounces. */
+#define EHSS_RAY_ESCAPED 0x1u
+#define EHSS_RAY_UNRESOLVED 0x2u
+#define EHSS_RAY_PA_HIT 0x4u
+#define EHSS_RAY_EHSS_HIT 0x8u
+#define EHSS_RAY_GRAZING 0x10u
+int ehss_scene_last_rays(ehss_scene* scene, int64_t n, uint32_t* flags, double* angle, int32_t* bounces);
+
+/* Device-generated source rays of a Sobol net (for parity tests of the source
+ * mapping): origins/directions n x 3, u n x 4 (any may be NULL). */
+int ehss_source_rays(ehss_scene* scene, const ehss_sobol* sobol, int power, double* u, double* origins,
+                     double* directions);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EHSS_HIP_H */
