@@ -107,4 +107,45 @@ struct PAIO {
 };
 int launch_pa_query(const SceneView& v, const PAIO& io, hipStream_t s);
 
+// Uploads caller rays, traces them against `view`, downloads the outputs.
+int trace_host_rays(const SceneView& view, int64_t nr, int cap, int precision, int record, double* p, double* d,
+                    const int64_t* lasts, int64_t* counts, uint8_t* escaped, uint8_t* unresolved,
+                    int32_t* history, int64_t* counters_out, hipStream_t s);
+
+struct BuiltScene {
+    int32_t n_groups = 0, n_nodes = 0, pa_nodes = 0, max_depth = 0;
+};
+int build_scene_device(int32_t n, int32_t leaf_size, const double* d_centers, const double* d_radii, double4* d_sph,
+                       int32_t* d_groups, int32_t* d_offsets, int32_t* d_eids, Node64* d_nodes, NodePA* d_pa,
+                       int32_t* d_pa_ids, BuiltScene* out, hipStream_t s);
+
+// Impact disk of ehss_state.py:46-47 computed on the host in NumPy's order:
+// center = sequential sum / n, radius = max(sqrt((dx^2 + dy^2) + dz^2) + r) * (1 + 1e-9).
+void enclosing_disk(int64_t n, const double* centers, const double* radii, double center[3], double* radius);
+
+// Per-ray records of a solve (ray-index order).
+struct RayRecords {
+    uint32_t* info = nullptr;   // bounces (low 20 bits) | flags << 20
+    double* contrib = nullptr;  // w * angle for escaped rays, else 0
+};
+
+constexpr uint32_t kBounceMask = 0xFFFFFu;
+constexpr int kFlagShift = 20;
+
 }  // namespace ehss
+
+// Device-resident scene (opaque handle of the C-ABI).
+struct ehss_scene {
+    int32_t n = 0, ng = 0, nn = 0, pa_nn = 0, leaf_size = 4, max_depth = 0;
+    double center[3] = {0, 0, 0};
+    double radius = 0, scale = 1;
+    ehss::DevBuf centers, radii, sph, groups, offsets, eids, nodes, nodes32, sph32, pa, pa_ids;
+    ehss::SceneView view;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // solve workspace (grow-only)
+    ehss::DevBuf ray_info, ray_contrib, partials, hist, ledger, counters;
+    int64_t last_rays = 0;
+    double last_weight = 0;
+    int64_t device_bytes() const;
+};

@@ -11,124 +11,11 @@
 //     whose FP64 entry distance is provably infinite or larger than the running
 //     FP64 best, so the selected atom, the distance and every reflection are the
 //     FP64 ones: outputs are bit-identical to kPlain.
-#include "ehss_internal.h"
+#include "ehss_query.h"
 
 namespace ehss {
 
 namespace {
-
-struct Query {
-    int32_t atom;
-    double best;
-    double second;  // second-smallest finite entry distance seen (grazing test)
-    double graze;   // smallest along-ray distance of a near-threshold sphere test
-};
-
-__device__ __forceinline__ void consider(Query& q, double t, int32_t wanted) {
-    if (better(t, wanted, q.best, q.atom)) {
-        q.second = q.best;
-        q.best = t;
-        q.atom = wanted;
-    } else if (t < q.second) {
-        q.second = t;
-    }
-}
-
-// ehss_cpu.cpp:59-72 with FP64 box tests (reference traversal, reference counters).
-__device__ void nearest_plain(const SceneView& S, const V3& o, const V3& d, int32_t last, Query& q,
-                              uint64_t& boxes, uint64_t& spheres) {
-    q.atom = -1;
-    q.best = INFINITY;
-    q.second = INFINITY;
-    q.graze = INFINITY;
-    int32_t node = 0;
-    const int32_t nn = S.n_nodes;
-    while (node < nn) {
-        const Node64& n = S.nodes[node];
-        ++boxes;
-        if (!box64(o, d, n.lo, n.hi, q.best)) {
-            node = n.end;
-            continue;
-        }
-        const int32_t g = n.leaf;
-        if (g >= 0) {
-            const int32_t k1 = S.offsets[g + 1];
-            for (int32_t k = S.offsets[g]; k < k1; ++k) {
-                const int32_t wanted = S.ids[k];
-                if (wanted == last) continue;
-                ++spheres;
-                const double4 c = S.sph[wanted];
-                const double t = entry64(o, d, c.x, c.y, c.z, c.w, &q.graze);
-                consider(q, t, wanted);
-            }
-        }
-        ++node;
-    }
-}
-
-// Same query through the FP32 filter; FP64 entry for every undecided sphere.
-__device__ void nearest_fast(const SceneView& S, const V3& o, const V3& d, int32_t last, Query& q,
-                             uint64_t& boxes, uint64_t& spheres, uint64_t& retests) {
-    q.atom = -1;
-    q.best = INFINITY;
-    q.second = INFINITY;
-    q.graze = INFINITY;
-    const float ox = (float)(o.x - S.cx), oy = (float)(o.y - S.cy), oz = (float)(o.z - S.cz);
-    const float dx = (float)d.x, dy = (float)d.y, dz = (float)d.z;
-    const float sx = fabsf(dx) < 1e-20f ? copysignf(1e-20f, dx) : dx;
-    const float sy = fabsf(dy) < 1e-20f ? copysignf(1e-20f, dy) : dy;
-    const float sz = fabsf(dz) < 1e-20f ? copysignf(1e-20f, dz) : dz;
-    const float ix = 1.0f / sx, iy = 1.0f / sy, iz = 1.0f / sz;
-    const float oix = ox * ix, oiy = oy * iy, oiz = oz * iz;
-    const float m_p = S.m_p, m_disc = S.m_disc, m_box = S.m_box;
-    float bestf = INFINITY;  // FP32 upper bound of best used for box pruning
-    int32_t node = 0;
-    const int32_t nn = S.n_nodes;
-    while (node < nn) {
-        const Node32 n = S.nodes32[node];
-        ++boxes;
-        const float tx0 = __builtin_fmaf(n.lo[0], ix, -oix), tx1 = __builtin_fmaf(n.hi[0], ix, -oix);
-        const float ty0 = __builtin_fmaf(n.lo[1], iy, -oiy), ty1 = __builtin_fmaf(n.hi[1], iy, -oiy);
-        const float tz0 = __builtin_fmaf(n.lo[2], iz, -oiz), tz1 = __builtin_fmaf(n.hi[2], iz, -oiz);
-        const float near = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.f));
-        const float far = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
-        if (!(near <= far && near <= bestf)) {
-            node = n.end;
-            continue;
-        }
-        const int32_t g = n.leaf;
-        if (g >= 0) {
-            const int32_t k1 = S.offsets[g + 1];
-            for (int32_t k = S.offsets[g]; k < k1; ++k) {
-                const int32_t wanted = S.ids[k];
-                if (wanted == last) continue;
-                ++spheres;
-                const float4 s = S.sph32[k];
-                const float x0 = ox - s.x, x1 = oy - s.y, x2 = oz - s.z;
-                const float p = x0 * dx + x1 * dy + x2 * dz;
-                if (p >= m_p) continue;  // projection64 > 0: miss
-                const float p0 = x0 - p * dx, p1 = x1 - p * dy, p2 = x2 - p * dz;
-                const float disc = s.w * s.w - (p0 * p0 + p1 * p1 + p2 * p2);
-                if (disc < -m_disc) continue;  // disc64 < 0: miss
-                const float tlo = (-p - m_p) - sqrtf(fmaxf(disc, 0.f) + m_disc);
-                if ((double)tlo > q.best) continue;  // entry64 > best: cannot win or tie
-                ++retests;
-                const double4 c = S.sph[wanted];
-                const double t = entry64(o, d, c.x, c.y, c.z, c.w, &q.graze);
-                const int32_t before = q.atom;
-                consider(q, t, wanted);
-                if (q.atom != before) bestf = (float)q.best * (1.0f + 4.0f * kU32) + m_box;
-            }
-        }
-        ++node;
-    }
-}
-
-// Query end: a ray is grazing when a near-threshold sphere test lies at or
-// before the winning distance, or when the runner-up is within tolerance.
-__device__ __forceinline__ bool grazing(const Query& q, double tol) {
-    return q.graze <= q.best + tol || q.second - q.best <= tol;
-}
 
 template <int kFast>
 __global__ void __launch_bounds__(256) k_trace(SceneView S, TraceIO io) {
@@ -190,31 +77,9 @@ __global__ void __launch_bounds__(256) k_pa_query(SceneView S, PAIO io) {
     if (i < io.nr) {
         const V3 o = {io.o[3 * i], io.o[3 * i + 1], io.o[3 * i + 2]};
         const V3 d = {io.d[3 * i], io.d[3 * i + 1], io.d[3 * i + 2]};
-        const int64_t owner = io.owners ? io.owners[i] : -1;
-        int32_t node = 0;
-        const int32_t stop = S.pa[0].end;
-        bool hit = false;
-        while (node < stop && !hit) {
-            const NodePA& n = S.pa[node];
-            ++boxes;
-            if (!box_hit64(o, d, n.lo, n.hi)) {
-                node = n.end;
-                continue;
-            }
-            for (int32_t k = n.first; k < n.first + n.count; ++k) {
-                const int32_t j = S.pa_ids[k];
-                if (j != owner) {
-                    ++tests;
-                    const double4 c = S.sph[j];
-                    if (sphere_hit64(o, d, c.x, c.y, c.z, c.w)) {
-                        hit = true;
-                        break;
-                    }
-                }
-            }
-            ++node;
-        }
-        io.hits[i] = hit ? 1 : 0;
+        const int32_t owner = io.owners ? (int32_t)io.owners[i] : -1;
+        bool graze;
+        io.hits[i] = pa_plain(S, o, d, owner, boxes, tests, graze) ? 1 : 0;
     }
     const uint64_t b = (uint64_t)wave_sum_i64((int64_t)boxes);
     const uint64_t t = (uint64_t)wave_sum_i64((int64_t)tests);

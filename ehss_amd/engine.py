@@ -1,0 +1,223 @@
+"""Python host layer over libehss_hip.so: Sobol scrambles and device scenes.
+
+Everything numerical runs on the GPU through the C-ABI; this module only
+validates arguments, owns handles and converts between NumPy arrays and the
+C structures.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import time
+
+import numpy as np
+
+from . import _lib as L
+
+BITS = 30
+
+# Joe & Kuo (2008), new-joe-kuo-6.21201: (s, a, m_1..m_s) for Sobol dimensions
+# 2..5 (dimension 1 is van der Corput).  The same published table SciPy uses.
+_JOE_KUO = ((1, 0, (1,)), (2, 1, (1, 3)), (3, 1, (1, 3, 1)), (3, 2, (1, 1, 1)))
+
+
+def direction_numbers(dim: int = 4, bits: int = BITS) -> np.ndarray:
+    """v[d, j] = m_j << (bits - 1 - j), m_j from the Bratley-Fox recurrence."""
+    if not 1 <= dim <= len(_JOE_KUO) + 1:
+        raise ValueError(f"dimension must be 1..{len(_JOE_KUO) + 1}")
+    v = np.zeros((dim, bits), dtype=np.int64)
+    v[0] = [1 << (bits - 1 - j) for j in range(bits)]
+    for d in range(1, dim):
+        s, a, init = _JOE_KUO[d - 1]
+        m = list(init)
+        for j in range(s, bits):
+            x = m[j - s] ^ (m[j - s] << s)
+            for k in range(1, s):
+                if (a >> (s - 1 - k)) & 1:
+                    x ^= m[j - k] << k
+            m.append(x)
+        v[d] = [m[j] << (bits - 1 - j) for j in range(bits)]
+    return v
+
+
+def scramble_matrices(seed: int, dim: int = 4, bits: int = BITS):
+    """Scrambled direction numbers and digital shift of scipy.stats.qmc.Sobol(dim,
+    scramble=True, seed=seed): LMS + shift, _qmc.py:1808-1823.  The generator is
+    numpy.random.default_rng(seed) (PCG64); the shift bits are drawn first, then
+    the lower-triangular matrices (unit diagonal)."""
+    if not 0 <= int(seed) < 2 ** 64:
+        raise ValueError("seed must be a nonnegative integer")
+    rng = np.random.default_rng(int(seed))
+    shift_bits = rng.integers(0, 2, size=(dim, bits), dtype=np.uint32).astype(np.int64)
+    shift = shift_bits @ (np.int64(1) << np.arange(bits, dtype=np.int64))
+    ltm = np.tril(rng.integers(0, 2, size=(dim, bits, bits), dtype=np.uint32)).astype(np.int64)
+    v = direction_numbers(dim, bits)
+    msb = np.int64(1) << (bits - 1 - np.arange(bits, dtype=np.int64))  # weight of MSB-first bit p
+    sv = np.zeros_like(v)
+    for d in range(dim):
+        lmat = ltm[d].copy()
+        np.fill_diagonal(lmat, 1)
+        vb = (v[d][:, None] & msb[None, :]) != 0           # vb[j, k]: MSB-first bit k of v[d, j]
+        out = (vb.astype(np.int64) @ lmat.T) & 1            # out[j, p] = parity(L[p, :] . vb[j, :])
+        sv[d] = out @ msb
+    return sv.astype(np.uint32), shift.astype(np.uint32)
+
+
+def scramble(seed: int) -> L.Sobol:
+    """ehss_sobol structure for a 4-D scrambled net with the given seed."""
+    sv, shift = scramble_matrices(seed, 4)
+    s = L.Sobol()
+    for d in range(4):
+        for j in range(BITS):
+            s.sv[d][j] = int(sv[d, j])
+        s.shift[d] = int(shift[d])
+    return s
+
+
+def replicate_seed(seed: int, repeat: int) -> int:
+    """ccs_cli.py:190: (seed + 104729 * repeat) mod 2^32."""
+    return (int(seed) + 104729 * int(repeat)) % (2 ** 32)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def precision_code(precision) -> int:
+    if isinstance(precision, str):
+        if precision not in L.PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(L.PRECISIONS)}")
+        return L.PRECISIONS[precision]
+    if precision not in (0, 1):
+        raise ValueError("precision must be 0 (fp64) or 1 (fp32-fixup)")
+    return int(precision)
+
+
+class GPUScene:
+    """Device-resident spheres with the static EHSS tree and the PA tree built
+    on the GPU (ehss_scene_create)."""
+
+    def __init__(self, centers, radii, leaf_size: int = 4):
+        c, r = _f64(centers), _f64(radii)
+        if c.ndim != 2 or c.shape[1] != 3 or r.shape != (len(c),) or len(c) == 0:
+            raise ValueError("Expected nonempty centers (N,3) and radii (N,).")
+        if type(leaf_size) is not int or leaf_size < 1:
+            raise ValueError("Positive integer leaf size required")
+        self.centers, self.radii = c, r
+        handle = ct.c_void_p()
+        t0 = time.perf_counter()
+        L.call("ehss_scene_create", len(r), L.ptr(c), L.ptr(r), leaf_size, ct.byref(handle))
+        self.build_s = time.perf_counter() - t0
+        self._h = handle
+        info = L.SceneInfo()
+        L.call("ehss_scene_get_info", self._h, ct.byref(info))
+        self.info = {name: (list(getattr(info, name)) if name == "center" else getattr(info, name))
+                     for name, _ in L.SceneInfo._fields_}
+
+    # -- lifetime -------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            L.fn("ehss_scene_destroy")(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        if not self._h:
+            raise RuntimeError("scene is closed")
+        return self._h
+
+    # -- tree export -------------------------------------------------------------
+    def export_tree(self) -> dict:
+        i = self.info
+        nn, ng, n = i["nodes"], i["groups"], i["atoms"]
+        lo, hi = np.empty((nn, 3)), np.empty((nn, 3))
+        end, leaf = np.empty(nn, np.int64), np.empty(nn, np.int64)
+        offsets, ids, groups = np.empty(ng + 1, np.int64), np.empty(n, np.int64), np.empty(n, np.int64)
+        L.call("ehss_scene_export_tree", self.handle, L.ptr(lo), L.ptr(hi), L.ptr(end), L.ptr(leaf), L.ptr(offsets),
+               L.ptr(ids), L.ptr(groups))
+        return dict(lo=lo, hi=hi, end=end, leaf_group=leaf, offsets=offsets, ids=ids, groups=groups)
+
+    def export_pa_tree(self) -> dict:
+        i = self.info
+        nn, n = i["pa_nodes"], i["atoms"]
+        lo, hi = np.empty((nn, 3)), np.empty((nn, 3))
+        end, first, count = (np.empty(nn, np.int64) for _ in range(3))
+        ids = np.empty(n, np.int64)
+        L.call("ehss_scene_export_pa_tree", self.handle, L.ptr(lo), L.ptr(hi), L.ptr(end), L.ptr(first),
+               L.ptr(count), L.ptr(ids))
+        return dict(lo=lo, hi=hi, end=end, first=first, count=count, ids=ids)
+
+    # -- caller rays ----------------------------------------------------------------
+    def trace(self, origins, directions, last_atom=None, initial_bounces=None, cap=64, record=False,
+              precision="fp32-fixup") -> dict:
+        p, d = _f64(origins).copy(), _f64(directions).copy()
+        n = len(p)
+        if p.shape != (n, 3) or d.shape != (n, 3):
+            raise ValueError("origins and directions must be (N,3)")
+        lasts = _i64(np.full(n, -1) if last_atom is None else last_atom)
+        counts = _i64(np.zeros(n) if initial_bounces is None else initial_bounces).copy()
+        escaped, unresolved = np.zeros(n, np.uint8), np.zeros(n, np.uint8)
+        history = np.full((n, cap if record else 0), -1, np.int32)
+        counters = np.zeros(3, np.int64)
+        L.call("ehss_scene_trace", self.handle, n, int(cap), precision_code(precision), int(bool(record)),
+               L.ptr(p), L.ptr(d), L.ptr(lasts), L.ptr(counts), L.ptr(escaped), L.ptr(unresolved), L.ptr(history),
+               L.ptr(counters))
+        return dict(positions=p, outgoing=d, escaped=escaped.astype(bool), unresolved=unresolved.astype(bool),
+                    bounces=counts, collider_ids=history, box_tests=int(counters[0]), sphere_tests=int(counters[1]),
+                    fp64_retests=int(counters[2]))
+
+    def pa_query(self, origins, directions, owners=None, precision="fp64"):
+        o, d = _f64(origins), _f64(directions)
+        n = len(o)
+        own = None if owners is None else _i64(owners)
+        hits = np.zeros(n, np.uint8)
+        counters = np.zeros(2, np.int64)
+        L.call("ehss_scene_pa_query", self.handle, n, precision_code(precision), L.ptr(o), L.ptr(d), L.ptr(own),
+               L.ptr(hits), L.ptr(counters))
+        return hits.astype(bool), int(counters[0]), int(counters[1])
+
+    # -- fused device solve ---------------------------------------------------------
+    def solve(self, sobol: L.Sobol, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
+              keep_rays: bool = False, histogram: bool = False) -> dict:
+        totals = L.Totals()
+        hist = np.zeros(cap + 1, np.int64) if histogram else None
+        ledger = np.zeros(cap + 1, np.float64) if histogram else None
+        flags = (L.FLAG_PA if pa else 0) | (L.FLAG_KEEP_RAYS if keep_rays else 0)
+        L.call("ehss_solve", self.handle, ct.byref(sobol), int(power), int(cap), precision_code(precision),
+               flags, ct.byref(totals), L.ptr(hist), L.ptr(ledger))
+        out = totals.as_dict()
+        if histogram:
+            out["histogram"] = hist
+            out["ledger"] = ledger
+        return out
+
+    def last_rays(self, n: int) -> dict:
+        flags = np.empty(n, np.uint32)
+        contrib = np.empty(n, np.float64)
+        bounces = np.empty(n, np.int32)
+        L.call("ehss_scene_last_rays", self.handle, n, L.ptr(flags), L.ptr(contrib), L.ptr(bounces))
+        return dict(flags=flags, contributions=contrib, bounces=bounces.astype(np.int64),
+                    escaped=(flags & L.RAY_ESCAPED) != 0, unresolved=(flags & L.RAY_UNRESOLVED) != 0,
+                    pa_hit=(flags & L.RAY_PA_HIT) != 0, ehss_hit=(flags & L.RAY_EHSS_HIT) != 0,
+                    grazing=(flags & L.RAY_GRAZING) != 0)
+
+    def source_rays(self, sobol: L.Sobol, power: int) -> dict:
+        n = 1 << power
+        u, o, d = np.empty((n, 4)), np.empty((n, 3)), np.empty((n, 3))
+        L.call("ehss_source_rays", self.handle, ct.byref(sobol), int(power), L.ptr(u), L.ptr(o), L.ptr(d))
+        return dict(u=u, origins=o, directions=d)

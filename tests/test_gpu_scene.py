@@ -1,0 +1,155 @@
+"""GPU parity of the device scene (tree build), scene trace / PA query, the
+device Sobol source and the fused solve, against the oracle."""
+import numpy as np
+import pytest
+
+from ehss_amd.engine import GPUScene, scramble
+from tests import scenes
+from tests.gpu_helpers import say
+
+pytestmark = pytest.mark.gpu
+
+BUILD_CASES = ["f29", "f37", "random47", "ties47", "coincident47", "single", "translated47", "pose4", "cross6",
+               "f23", "c60"]
+
+
+@pytest.mark.parametrize("leaf", [1, 4, 9])
+def test_device_tree_build_equals_oracle(oracle, leaf):
+    cases = [(n, scenes.SCENES[n]()) for n in BUILD_CASES]
+    cases.append(("globule650", scenes.globule(650, 1)))
+    cases.append(("ellipsoid2000", scenes.globule(2000, 2, (1., 1., 1.4))))
+    for name, (c, r) in cases:
+        say("build", name, "leaf", leaf)
+        with GPUScene(c, r, leaf_size=leaf) as sc:
+            got = sc.export_tree()
+            exp = oracle.static_tree(c, r, leaf)
+            for k in ("lo", "hi", "end", "leaf_group", "offsets", "ids", "groups"):
+                np.testing.assert_array_equal(got[k], exp[k], err_msg=f"{name}/leaf{leaf}/{k}")
+            gp, ep = sc.export_pa_tree(), oracle.pa_tree(c, r, leaf)
+            for k in ("lo", "hi", "end", "first", "count", "ids"):
+                np.testing.assert_array_equal(gp[k], ep[k], err_msg=f"{name}/leaf{leaf}/pa/{k}")
+            center, radius = oracle.enclosing(c, r)
+            assert sc.info["center"] == list(center) and sc.info["radius"] == radius
+            assert sc.info["groups"] == int(exp["groups"].max()) + 1
+
+
+def test_scene_trace_and_pa_match_oracle(oracle):
+    for name in ("f29", "c60", "coincident47"):
+        c, r = scenes.SCENES[name]()
+        tree = oracle.static_tree(c, r, 4)
+        ptree = oracle.pa_tree(c, r, 4)
+        with GPUScene(c, r) as sc:
+            for kind, src in (("external", oracle.external_source(c, r, 11, 5)),
+                              ("reverse", oracle.reverse_pa_source(c, r, 10, 6))):
+                exp = oracle.trace(tree, c, r, src["origins"], src["directions"], src["last_atom"],
+                                   src["initial_bounces"], cap=64)
+                for prec in ("fp64", "fp32-fixup"):
+                    say("scene trace", name, kind, prec)
+                    got = sc.trace(src["origins"], src["directions"], src["last_atom"], src["initial_bounces"], cap=64,
+                                   record=True, precision=prec)
+                    for k in ("positions", "outgoing", "escaped", "unresolved", "bounces", "collider_ids"):
+                        np.testing.assert_array_equal(got[k], exp[k], err_msg=f"{name}/{kind}/{prec}/{k}")
+                    if prec == "fp64":
+                        assert (got["box_tests"], got["sphere_tests"]) == (exp["box_tests"], exp["sphere_tests"])
+                owners = src["last_atom"]
+                hits, nb, nt = sc.pa_query(src["origins"], src["directions"], owners)
+                eh, eb, et = oracle.pa_query(ptree, c, r, src["origins"], src["directions"], owners)
+                np.testing.assert_array_equal(hits, eh)
+                assert (nb, nt) == (eb, et)
+
+
+def test_device_source_matches_numpy_source(oracle):
+    c, r = scenes.globule(650, 1)
+    with GPUScene(c, r) as sc:
+        for seed, power in ((0, 12), (190003, 14), (4294967295, 10)):
+            say("source", seed, power)
+            dev = sc.source_rays(scramble(seed), power)
+            sv, shift = oracle.sobol_scramble(seed)
+            np.testing.assert_array_equal(dev["u"], oracle.sobol_points(sv, shift, power))
+            ref = oracle.external_source(c, r, power, seed)
+            scale = sc.info["radius"]
+            assert np.abs(dev["directions"] - ref["directions"]).max() <= 4e-16
+            assert np.abs(dev["origins"] - ref["origins"]).max() <= 8e-16 * scale
+            same = np.all(dev["origins"] == ref["origins"], axis=1).mean()
+            say(f"  bit-identical origins: {same:.4f}")
+
+
+def _solve_vs_oracle(oracle, c, r, seed, power, cap, prec):
+    with GPUScene(c, r) as sc:
+        res = sc.solve(scramble(seed), power, cap, precision=prec, pa=True, keep_rays=True, histogram=True)
+        rays = sc.last_rays(1 << power)
+    src = oracle.external_source(c, r, power, seed)
+    tr = oracle.trace(oracle.static_tree(c, r, 4), c, r, src["origins"], src["directions"], cap=cap, record=False,
+                      threads=8)
+    ro = oracle.readout(src, tr, cap)
+    hits, _, _ = oracle.pa_query(oracle.pa_tree(c, r, 4), c, r, src["origins"], src["directions"], threads=8)
+    return res, rays, tr, ro, hits, src
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32-fixup"])
+def test_fused_solve_matches_oracle_on_same_sobol_points(oracle, prec):
+    """Same Sobol points (device source vs NumPy source): per-ray bounce counts,
+    escape/unresolved and PA hit flags equal except on rays the kernel flags as
+    grazing; CCS, PA and tail within 1e-9 relative (1e-6 is the stated bar)."""
+    cases = [("c60", scenes.c60(), 0, 14, 64), ("f29", scenes.fixture_29(), 881, 12, 4),
+             ("globule650", scenes.globule(650, 1), 1, 14, 64), ("cross6", scenes.fixture_cross(), 3, 12, 1)]
+    for name, (c, r), seed, power, cap in cases:
+        say("solve", name, prec, seed, power, cap)
+        res, rays, tr, ro, hits, src = _solve_vs_oracle(oracle, c, r, seed, power, cap, prec)
+        ok = ~rays["grazing"]
+        say(f"  grazing rays: {int((~ok).sum())} of {len(ok)}")
+        assert (~ok).mean() < 1e-3
+        np.testing.assert_array_equal(rays["bounces"][ok], tr["bounces"][ok])
+        np.testing.assert_array_equal(rays["escaped"][ok], tr["escaped"][ok])
+        np.testing.assert_array_equal(rays["unresolved"][ok], tr["unresolved"][ok])
+        np.testing.assert_array_equal(rays["pa_hit"][ok], hits[ok])
+        assert res["omega"] == pytest.approx(ro["omega"], rel=1e-9)
+        assert res["pa"] == pytest.approx(float(src["weights"][hits].sum()), rel=1e-9, abs=1e-12)
+        assert res["pa_from_ehss"] == pytest.approx(ro["pa"], rel=1e-9, abs=1e-12)
+        assert res["tail_bound"] == pytest.approx(ro["tail_bound"], rel=1e-9, abs=1e-12)
+        assert res["histogram"].sum() == 1 << power
+        np.testing.assert_allclose(res["ledger"].sum(), res["omega"], rtol=1e-10)
+
+
+def test_plain_and_fp32_fixup_kernels_are_bit_identical(oracle):
+    c, r = scenes.globule(650, 1)
+    with GPUScene(c, r) as sc:
+        out = {}
+        for prec in ("fp64", "fp32-fixup"):
+            say("kernels", prec)
+            res = sc.solve(scramble(7), 16, 64, precision=prec, pa=True, keep_rays=True, histogram=True)
+            out[prec] = (res, sc.last_rays(1 << 16))
+        a, b = out["fp64"], out["fp32-fixup"]
+        assert a[0]["omega"] == b[0]["omega"] and a[0]["pa"] == b[0]["pa"]
+        np.testing.assert_array_equal(a[1]["contributions"], b[1]["contributions"])
+        np.testing.assert_array_equal(a[1]["flags"], b[1]["flags"])
+        np.testing.assert_array_equal(a[0]["histogram"], b[0]["histogram"])
+        # determinism: repeat
+        again = sc.solve(scramble(7), 16, 64, precision="fp32-fixup", pa=True)
+        assert again["omega"] == b[0]["omega"]
+
+
+def test_single_sphere_solve_is_pi_r2():
+    c, r = np.zeros((1, 3)), np.array([3.1])
+    with GPUScene(c, r) as sc:
+        for prec in ("fp64", "fp32-fixup"):
+            res = sc.solve(scramble(91), 12, 64, precision=prec)
+            assert res["omega"] == pytest.approx(np.pi * 3.1 ** 2, rel=1e-6)
+            assert res["pa"] == pytest.approx(np.pi * 3.1 ** 2, rel=2e-3)
+
+
+def test_full_size_globule_properties(oracle):
+    """2^20 rays on the 650-atom globule: both kernels agree bitwise, the
+    histogram covers every ray, and the CCS agrees with the oracle's C solve."""
+    c, r = scenes.globule(650, 1)
+    with GPUScene(c, r) as sc:
+        say("full size fp64")
+        a = sc.solve(scramble(1), 20, 64, precision="fp64", pa=True, histogram=True)
+        say("full size fp32-fixup")
+        b = sc.solve(scramble(1), 20, 64, precision="fp32-fixup", pa=True, histogram=True)
+    assert a["omega"] == b["omega"] and a["pa"] == b["pa"] and a["unresolved"] == b["unresolved"]
+    assert b["histogram"].sum() == 1 << 20
+    say("oracle C solve")
+    cpu = oracle.solve_cpu(c, r, 1, 20, 64)
+    assert b["omega"] == pytest.approx(cpu["omega"], rel=1e-9)
+    assert b["pa"] == pytest.approx(cpu["pa"], rel=1e-9)
