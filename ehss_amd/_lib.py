@@ -25,6 +25,7 @@ PRECISIONS = {"fp64": PRECISION_FP64, "fp32-fixup": PRECISION_FP32_FIXUP}
 
 FLAG_PA = 1
 FLAG_KEEP_RAYS = 2
+VARIANT_SHIFT = 8
 
 RAY_ESCAPED = 0x1
 RAY_UNRESOLVED = 0x2
@@ -82,6 +83,7 @@ _SIGS = {
     "ehss_trace": (None, [_I64, _I64, _INT, _INT, _INT] + [_P] * 17),
     "ehss_trace_status": (_INT, [_I64, _I64, _INT, _INT, _INT] + [_P] * 16),
     "ehss_scene_create": (_INT, [_I64, _P, _P, _INT, ct.POINTER(_P)]),
+    "ehss_scene_create_grouped": (_INT, [_I64, _P, _P, _P, _INT, ct.POINTER(_P)]),
     "ehss_scene_destroy": (_INT, [_P]),
     "ehss_scene_get_info": (_INT, [_P, ct.POINTER(SceneInfo)]),
     "ehss_scene_export_tree": (_INT, [_P] + [_P] * 7),

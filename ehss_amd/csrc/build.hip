@@ -397,11 +397,49 @@ __global__ void __launch_bounds__(1024) k_pa_tree(BuildArgs A, int levels) {
     }
 }
 
+// Root segment temporarily marked as splitting [0, n) for the custom-group rank.
+__global__ void k_reset_root(BuildArgs A) {
+    A.ssize[0] = A.sleaf[0];  // remember the root's leaf flag in ssize[0]
+    A.sleaf[0] = 0;
+    A.sdepth[0] = 0;
+}
+
+__global__ void k_restore_root(BuildArgs A) { A.sleaf[0] = A.ssize[0]; }
+
+// Caller-supplied groups: eids = argsort(groups, stable) as the rank of
+// (group, atom) over all atoms, reusing k_rank on a single segment.
+__global__ void k_custom_keys(BuildArgs A, const int32_t* cg) {
+    const int32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= A.n) return;
+    A.skey[pos] = (double)cg[pos];
+    A.rank[pos] = 0;
+    A.pseg[pos] = 0;
+    A.groups[pos] = cg[pos];
+}
+
+__global__ void k_custom_place(BuildArgs A, const int32_t* cg) {
+    const int32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= A.n) return;
+    A.eids[A.rank[pos]] = pos;
+}
+
+__global__ void k_custom_offsets(BuildArgs A, const int32_t* cg, int32_t ng) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= A.n) return;
+    const int32_t g = cg[A.eids[k]];
+    if (k == 0 || cg[A.eids[k - 1]] != g) A.offsets[g] = k;
+    if (k == 0) {
+        A.offsets[ng] = A.n;
+        A.ctl->n_groups = ng;
+    }
+}
+
 }  // namespace
 
 int build_scene_device(int32_t n, int32_t leaf_size, const double* d_centers, const double* d_radii, double4* d_sph,
                        int32_t* d_groups, int32_t* d_offsets, int32_t* d_eids, Node64* d_nodes, NodePA* d_pa,
-                       int32_t* d_pa_ids, BuiltScene* out, hipStream_t s) {
+                       int32_t* d_pa_ids, const int32_t* d_custom_groups, int32_t custom_ng, BuiltScene* out,
+                       hipStream_t s) {
     // Temporaries.
     DevBuf keys, ids2, pseg, pseg2, skey, rank, seg, ctl, flag, ndepth, nright, leaf_node, pabox;
     const int32_t max_segs = 2 * n + 2;
@@ -482,9 +520,23 @@ int build_scene_device(int32_t n, int32_t leaf_size, const double* d_centers, co
         A.ids2 = A.ids;
         A.ids = d_pa_ids;
     }
-    hipLaunchKernelGGL(k_leaf_flags, eg, eb, 0, s, A);
-    hipLaunchKernelGGL(k_groups, dim3(1), dim3(1024), 0, s, A);
-    hipLaunchKernelGGL(k_group_ids, eg, eb, 0, s, A);
+    if (d_custom_groups) {
+        // The PA tree above is the median-split tree; the EHSS groups are the caller's.
+        // Segment 0 (the root) is reused as one segment holding every atom.
+        BuildArgs C = A;
+        C.pseg = A.pseg2;
+        hipLaunchKernelGGL(k_reset_root, dim3(1), dim3(1), 0, s, C);
+        hipLaunchKernelGGL(k_custom_keys, eg, eb, 0, s, C, d_custom_groups);
+        const unsigned chunks = (unsigned)((n + kRankChunk - 1) / kRankChunk);
+        hipLaunchKernelGGL(k_rank, dim3(eg.x, chunks), eb, 0, s, C, 0);
+        hipLaunchKernelGGL(k_custom_place, eg, eb, 0, s, C, d_custom_groups);
+        hipLaunchKernelGGL(k_custom_offsets, eg, eb, 0, s, C, d_custom_groups, custom_ng);
+        hipLaunchKernelGGL(k_restore_root, dim3(1), dim3(1), 0, s, C);
+    } else {
+        hipLaunchKernelGGL(k_leaf_flags, eg, eb, 0, s, A);
+        hipLaunchKernelGGL(k_groups, dim3(1), dim3(1024), 0, s, A);
+        hipLaunchKernelGGL(k_group_ids, eg, eb, 0, s, A);
+    }
     EHSS_HIP(hipGetLastError());
     Ctl hc;
     EHSS_HIP(hipMemcpyAsync(&hc, ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, s));

@@ -53,39 +53,64 @@ struct alignas(64) NodePA {
     int32_t pad;
 };
 
-// Per-query grazing bookkeeping: a decision is "grazing" when a relative
-// perturbation of order kGrazeTol could flip it.
+// Grazing bookkeeping.  A decision is "grazing" when an input perturbation of
+// size tol (a distance) could flip it: tol starts at kGrazeTol times the scene
+// scale and grows along the path with a first-order bound on how far rounding
+// differences in the source could have moved the ray (see PathError).
 constexpr double kGrazeTol = 1e-9;
 
-// src/native/ehss_cpu.cpp:7-17.  Also returns, through *graze_t, the
-// along-ray distance of a near-threshold decision (or +inf when none):
-//   disc within kGrazeTol*r^2 of 0 (tangency), projection within
-//   kGrazeTol*|x| of 0, or near within kGrazeTol*r of the -64 eps r snap.
+// src/native/ehss_cpu.cpp:7-17.  Also lowers *graze_t to the along-ray distance of
+// a decision within tol of its threshold: tangency (|disc| <= 2 r tol, sphere not
+// behind) or the -64 eps r snap (|near - snap| <= tol).  The sign of the
+// projection can only flip the outcome together with a near-zero disc.
 __device__ __forceinline__ double entry64(const V3& o, const V3& d, double cx, double cy, double cz, double r,
-                                          double* graze_t) {
+                                          double tol, double* graze_t) {
     double x0 = o.x - cx, x1 = o.y - cy, x2 = o.z - cz;
     double aa = (d.x * d.x + d.y * d.y) + d.z * d.z;
     double projection = ((x0 * d.x + x1 * d.y) + x2 * d.z) / aa;
     double p0 = x0 - projection * d.x, p1 = x1 - projection * d.y, p2 = x2 - projection * d.z;
     double disc = r * r - ((p0 * p0 + p1 * p1) + p2 * p2);
-    double rr = r * r;
-    bool tangent = fabs(disc) <= kGrazeTol * rr;
+    const double tdisc = 2 * r * tol;
+    const bool tangent = fabs(disc) <= tdisc;
     if (disc < 0. || projection >= 0.) {
-        if (tangent && projection < 0.) *graze_t = fmin(*graze_t, -projection);
-        if (projection >= 0. && projection <= kGrazeTol * r && disc >= 0.) *graze_t = 0.;
+        if (tangent && projection < tol) *graze_t = fmin(*graze_t, fmax(-projection, 0.));
         return INFINITY;
     }
     double near = -projection - sqrt(smax(0., disc) / aa);
     double snap = -64 * kEps64 * r;
     if (tangent) *graze_t = fmin(*graze_t, -projection);
-    if (fabs(near - snap) <= kGrazeTol * r) *graze_t = 0.;
+    if (fabs(near - snap) <= tol) *graze_t = 0.;
     if (near < snap) return INFINITY;
     return smax(0., near);
 }
 
-// src/native/ehss_cpu.cpp:19-29
+// First-order bound on the displacement of a ray (position ep, direction ed)
+// caused by rounding-level differences in its source, propagated through free
+// flights and specular reflections: a flight of length t adds t*ed to ep; a
+// reflection at incidence cosine c off radius r maps ep -> ep / c and adds
+// 2 ep / r to ed.  Used only to widen the grazing tolerance.
+struct PathError {
+    double ep, ed;
+};
+
+__device__ __forceinline__ PathError path_error_start(double scale) {
+    return PathError{16 * kEps64 * scale, 16 * kEps64};
+}
+
+__device__ __forceinline__ double path_tol(const PathError& e, double scale) {
+    return fmax(kGrazeTol * scale, 8.0 * (e.ep + 2.0 * scale * e.ed));
+}
+
+__device__ __forceinline__ void path_error_reflect(PathError& e, double t, double cos_inc, double r, double scale) {
+    const double c = fmax(fabs(cos_inc), 1e-6);
+    const double ep = (e.ep + t * e.ed) / c + 4 * kEps64 * scale;
+    e.ed = e.ed + 2 * ep / r + 4 * kEps64;
+    e.ep = ep;
+}
+
+// src/native/ehss_cpu.cpp:19-29.  *near_out receives the slab entry distance.
 __device__ __forceinline__ bool box64(const V3& o, const V3& d, const double* lo, const double* hi,
-                                      double best) {
+                                      double best, double* near_out = nullptr) {
     double near = 0., far = INFINITY;
     const double oo[3] = {o.x, o.y, o.z};
     const double dd[3] = {d.x, d.y, d.z};
@@ -99,12 +124,13 @@ __device__ __forceinline__ bool box64(const V3& o, const V3& d, const double* lo
             far = smin(far, smax(a, b));
         }
     }
+    if (near_out) *near_out = (far >= near && far >= 0.) ? near : INFINITY;
     return far >= near && near <= best && far >= 0.;
 }
 
 // src/native/ehss_cpu.cpp:31-40 (in place on o, d).
-__device__ __forceinline__ void reflect64(V3& o, V3& d, double cx, double cy, double cz, double r,
-                                          double distance) {
+__device__ __forceinline__ double reflect64(V3& o, V3& d, double cx, double cy, double cz, double r,
+                                            double distance) {
     double n0 = o.x + distance * d.x - cx, n1 = o.y + distance * d.y - cy, n2 = o.z + distance * d.z - cz;
     double norm = sqrt((n0 * n0 + n1 * n1) + n2 * n2);
     n0 /= norm;
@@ -121,6 +147,7 @@ __device__ __forceinline__ void reflect64(V3& o, V3& d, double cx, double cy, do
     d.x /= norm;
     d.y /= norm;
     d.z /= norm;
+    return dot;  // incidence cosine (unit incoming direction)
 }
 
 // Nearest-atom update rule of src/native/ehss_cpu.cpp:67 (tie: lower id wins).

@@ -117,7 +117,8 @@ struct BuiltScene {
 };
 int build_scene_device(int32_t n, int32_t leaf_size, const double* d_centers, const double* d_radii, double4* d_sph,
                        int32_t* d_groups, int32_t* d_offsets, int32_t* d_eids, Node64* d_nodes, NodePA* d_pa,
-                       int32_t* d_pa_ids, BuiltScene* out, hipStream_t s);
+                       int32_t* d_pa_ids, const int32_t* d_custom_groups, int32_t custom_ng, BuiltScene* out,
+                       hipStream_t s);
 
 // Impact disk of ehss_state.py:46-47 computed on the host in NumPy's order:
 // center = sequential sum / n, radius = max(sqrt((dx^2 + dy^2) + dz^2) + r) * (1 + 1e-9).

@@ -1,11 +1,17 @@
-// ehss_query.h -- per-lane nearest-hit and any-hit queries over a device scene.
+// ehss_query.h -- nearest-hit and any-hit queries over a device scene.
 //
-// nearest_plain: ehss_cpu.cpp:59-72 (FP64 box tests, reference traversal and counters).
-// nearest_fast:  the same query through an FP32 filter with an exact FP64 re-test of
-//                every sphere the filter cannot exclude (identical result).
-// pa_plain:      pa_spatial.py:114-130 over the PA tree (FP64, reference counters).
-// pa_fast:       any-hit through the FP32 filter over the static tree, FP64 re-test
-//                of undecided spheres (identical hit flag).
+// nearest_plain:  ehss_cpu.cpp:59-72 (FP64 box tests, reference traversal and counters).
+// nearest_fast:   the same query through an FP32 filter with an exact FP64 re-test of
+//                 every sphere the filter cannot exclude (identical result).
+// nearest_packet: nearest_fast with a wave-uniform node sequence and scalar loads.
+// pa_plain:       pa_spatial.py:114-130 over the PA tree (FP64, reference counters).
+// pa_fast:        any-hit through the FP32 filter over the static tree, FP64 re-test
+//                 of undecided spheres (identical hit flag).
+//
+// Every query takes `tol`, the current grazing tolerance (a distance).  The
+// filter never excludes a sphere whose exact FP64 outcome lies within tol of a
+// decision threshold, so its grazing flag is conservative; the plain kernel
+// keeps the reference traversal and flags boxes it culls within tol of best.
 #pragma once
 #include "ehss_internal.h"
 
@@ -18,6 +24,13 @@ struct Query {
     double graze;   // smallest along-ray distance of a near-threshold sphere test
 };
 
+__device__ __forceinline__ void query_init(Query& q) {
+    q.atom = -1;
+    q.best = INFINITY;
+    q.second = INFINITY;
+    q.graze = INFINITY;
+}
+
 __device__ __forceinline__ void consider(Query& q, double t, int32_t wanted) {
     if (better(t, wanted, q.best, q.atom)) {
         q.second = q.best;
@@ -28,24 +41,25 @@ __device__ __forceinline__ void consider(Query& q, double t, int32_t wanted) {
     }
 }
 
-// A ray is grazing when a near-threshold sphere test lies at or before the
-// winning distance, or when the runner-up is within tolerance of the winner.
+// A query is grazing when a near-threshold sphere test lies at or before the
+// winning distance, or when the runner-up is within tol of the winner.
 __device__ __forceinline__ bool grazing(const Query& q, double tol) {
-    return q.graze <= q.best + tol || q.second - q.best <= tol;
+    return (q.graze < INFINITY && q.graze <= q.best + tol) || q.second - q.best <= tol;
 }
 
-__device__ __forceinline__ void nearest_plain(const SceneView& S, const V3& o, const V3& d, int32_t last, Query& q,
-                                              uint64_t& boxes, uint64_t& spheres) {
-    q.atom = -1;
-    q.best = INFINITY;
-    q.second = INFINITY;
-    q.graze = INFINITY;
+__device__ __forceinline__ void nearest_plain(const SceneView& S, const V3& o, const V3& d, int32_t last, double tol,
+                                              Query& q, uint64_t& boxes, uint64_t& spheres) {
+    query_init(q);
     int32_t node = 0;
     const int32_t nn = S.n_nodes;
     while (node < nn) {
         const Node64& n = S.nodes[node];
         ++boxes;
-        if (!box64(o, d, n.lo, n.hi, q.best)) {
+        double near;
+        if (!box64(o, d, n.lo, n.hi, q.best, &near)) {
+            // A box culled by the reference rule near > best may still hold a
+            // sphere within tol of best: count it as a potential near-tie.
+            if (near <= q.best + tol) q.graze = fmin(q.graze, near);
             node = n.end;
             continue;
         }
@@ -57,7 +71,7 @@ __device__ __forceinline__ void nearest_plain(const SceneView& S, const V3& o, c
                 if (wanted == last) continue;
                 ++spheres;
                 const double4 c = S.sph[wanted];
-                const double t = entry64(o, d, c.x, c.y, c.z, c.w, &q.graze);
+                const double t = entry64(o, d, c.x, c.y, c.z, c.w, tol, &q.graze);
                 consider(q, t, wanted);
             }
         }
@@ -100,14 +114,79 @@ __device__ __forceinline__ bool box32(const Ray32& r, const Node32& n, float bes
     return near <= far && near <= bestf;
 }
 
-__device__ __forceinline__ void nearest_fast(const SceneView& S, const V3& o, const V3& d, int32_t last, Query& q,
-                                             uint64_t& boxes, uint64_t& spheres, uint64_t& retests) {
-    q.atom = -1;
-    q.best = INFINITY;
-    q.second = INFINITY;
-    q.graze = INFINITY;
+// FP32 upper bound of best + tol used for box pruning.
+__device__ __forceinline__ float box_bound(const SceneView& S, double best, double tol) {
+    return (float)(best + tol) * (1.0f + 4.0f * kU32) + S.m_box;
+}
+
+// FP64 any-hit sphere test (inside_out_pa.py:93-101) with a flag for decisions an
+// input perturbation of size tol could flip: hit = (disc >= 0) and (exit > 0),
+// critical when disc is within 2 r tol of 0 while the exit condition (-b > 0 at
+// disc = 0) nearly holds, or when exit is within tol of 0 while disc nearly holds.
+__device__ __forceinline__ bool sphere_hit64_graze(const V3& o, const V3& d, double cx, double cy, double cz, double r,
+                                                   double tol, bool* critical) {
+    double x = o.x - cx, y = o.y - cy, z = o.z - cz;
+    double b = (x * d.x + y * d.y) + z * d.z;
+    double cc = ((x * x + y * y) + z * z) - r * r;
+    double disc = b * b - cc;
+    double exit = -b + sqrt(smax(0., disc));
+    const double tdisc = 2 * r * tol;
+    *critical = (fabs(disc) <= tdisc && -b > -tol) || (fabs(exit) <= tol && disc >= -tdisc);
+    return disc >= 0 && exit > 0;
+}
+
+// Any-hit PA decided during the first nearest query of an external ray
+// (origin outside every sphere).  A sphere whose FP32 test is a robust hit is a
+// robust PA hit; every sphere the query re-tests in FP64 also gets the FP64
+// any-hit test.  When the query ends without a robust PA hit but with a winner,
+// the caller runs the full any-hit query.
+struct PAState {
+    bool decided;   // a robust hit was found
+    bool hit;       // some FP64 any-hit test returned true
+    bool critical;  // some FP64 any-hit test was near a threshold
+};
+
+// Exact FP64 entry of one sphere (and its any-hit test when PA is pending).
+__device__ __forceinline__ void eval64(const SceneView& S, const V3& o, const V3& d, int32_t wanted, double tol,
+                                       Query& q, PAState* pa) {
+    const double4 c = S.sph[wanted];
+    const double t = entry64(o, d, c.x, c.y, c.z, c.w, tol, &q.graze);
+    consider(q, t, wanted);
+    if (pa && !pa->decided) {
+        bool crit;
+        const bool h = sphere_hit64_graze(o, d, c.x, c.y, c.z, c.w, tol, &crit);
+        pa->critical |= crit;
+        if (h) {
+            pa->hit = true;
+            if (!crit) pa->decided = true;
+        }
+    }
+}
+
+// FP32 classification of one sphere against the running FP64 best.  Returns true
+// when the exact FP64 entry must be evaluated: false only when the FP64 entry is
+// provably infinite or larger than best + tol, and no FP64 threshold test of this
+// sphere is within tol.
+__device__ __forceinline__ bool needs_exact(const SceneView& S, const Ray32& r, const float4& s, double best,
+                                            double tol, PAState* pa) {
+    const float x0 = r.ox - s.x, x1 = r.oy - s.y, x2 = r.oz - s.z;
+    const float p = x0 * r.dx + x1 * r.dy + x2 * r.dz;
+    const float ftol = (float)tol * 1.01f;
+    if (p >= S.m_p + ftol) return false;  // projection64 > tol: behind
+    const float p0 = x0 - p * r.dx, p1 = x1 - p * r.dy, p2 = x2 - p * r.dz;
+    const float disc = s.w * s.w - (p0 * p0 + p1 * p1 + p2 * p2);
+    const float mdisc = S.m_disc + 2.01f * s.w * ftol;
+    if (disc < -mdisc) return false;  // disc64 < -2 r tol: clear miss
+    if (pa && disc > mdisc && p <= -(S.m_p + ftol)) pa->decided = pa->hit = true;  // robust hit in both tests
+    const float tlo = (-p - S.m_p) - sqrtf(fmaxf(disc, 0.f) + S.m_disc);
+    return !((double)tlo > best + tol);  // entry64 > best + tol: cannot win, tie or graze
+}
+
+__device__ __forceinline__ void nearest_fast(const SceneView& S, const V3& o, const V3& d, int32_t last, double tol,
+                                             Query& q, uint64_t& boxes, uint64_t& spheres, uint64_t& retests,
+                                             PAState* pa = nullptr) {
+    query_init(q);
     const Ray32 r = ray32(S, o, d);
-    const float m_p = S.m_p, m_disc = S.m_disc, m_box = S.m_box;
     float bestf = INFINITY;
     int32_t node = 0;
     const int32_t nn = S.n_nodes;
@@ -125,48 +204,86 @@ __device__ __forceinline__ void nearest_fast(const SceneView& S, const V3& o, co
                 const int32_t wanted = S.ids[k];
                 if (wanted == last) continue;
                 ++spheres;
-                const float4 s = S.sph32[k];
-                const float x0 = r.ox - s.x, x1 = r.oy - s.y, x2 = r.oz - s.z;
-                const float p = x0 * r.dx + x1 * r.dy + x2 * r.dz;
-                if (p >= m_p) continue;  // projection64 > 0: miss
-                const float p0 = x0 - p * r.dx, p1 = x1 - p * r.dy, p2 = x2 - p * r.dz;
-                const float disc = s.w * s.w - (p0 * p0 + p1 * p1 + p2 * p2);
-                if (disc < -m_disc) continue;  // disc64 < 0: miss
-                const float tlo = (-p - m_p) - sqrtf(fmaxf(disc, 0.f) + m_disc);
-                if ((double)tlo > q.best) continue;  // entry64 > best: cannot win or tie
+                if (!needs_exact(S, r, S.sph32[k], q.best, tol, pa)) continue;
                 ++retests;
-                const double4 c = S.sph[wanted];
-                const double t = entry64(o, d, c.x, c.y, c.z, c.w, &q.graze);
                 const int32_t before = q.atom;
-                consider(q, t, wanted);
-                if (q.atom != before) bestf = (float)q.best * (1.0f + 4.0f * kU32) + m_box;
+                eval64(S, o, d, wanted, tol, q, pa);
+                if (q.atom != before) bestf = box_bound(S, q.best, tol);
             }
         }
         ++node;
     }
 }
 
-// FP64 any-hit sphere test (inside_out_pa.py:93-101) with a flag for decisions a
-// small perturbation could flip: hit = (disc >= 0) and (exit > 0), so it is
-// critical when disc is within tol r^2 of 0 while the exit condition (-b > 0 at
-// disc = 0) nearly holds, or when exit is within tol r of 0 while disc nearly
-// holds.
-__device__ __forceinline__ bool sphere_hit64_graze(const V3& o, const V3& d, double cx, double cy, double cz, double r,
-                                                   bool* critical) {
-    double x = o.x - cx, y = o.y - cy, z = o.z - cz;
-    double b = (x * d.x + y * d.y) + z * d.z;
-    double cc = ((x * x + y * y) + z * z) - r * r;
-    double disc = b * b - cc;
-    double exit = -b + sqrt(smax(0., disc));
-    const double tr = kGrazeTol * r, trr = kGrazeTol * r * r;
-    *critical = (fabs(disc) <= trr && -b > -tr) || (fabs(exit) <= tr && disc >= -trr);
-    return disc >= 0 && exit > 0;
+// Wave-uniform (packet) traversal: the 64 lanes walk one node sequence; a node is
+// entered when any live lane's inflated box test passes.  Node, leaf and sphere
+// records are read through the constant address space, so the compiler issues
+// scalar loads (one fetch per wave) for them.  Per-lane state and the exact
+// FP64 re-tests are those of nearest_fast, so every lane gets the same answer.
+typedef const __attribute__((address_space(4))) float* CFloat;
+typedef const __attribute__((address_space(4))) int32_t* CInt32;
+
+__device__ __forceinline__ Node32 load_node(const Node32* base, int32_t node) {
+    const CFloat f = (CFloat)(base + node);
+    const CInt32 w = (CInt32)(base + node);
+    Node32 n;
+    n.lo[0] = f[0];
+    n.lo[1] = f[1];
+    n.lo[2] = f[2];
+    n.hi[0] = f[3];
+    n.hi[1] = f[4];
+    n.hi[2] = f[5];
+    n.end = w[6];
+    n.leaf = w[7];
+    return n;
+}
+
+__device__ __forceinline__ float4 load_sphere32(const float4* base, int32_t k) {
+    const CFloat f = (CFloat)(base + k);
+    return make_float4(f[0], f[1], f[2], f[3]);
+}
+
+__device__ __forceinline__ void nearest_packet(const SceneView& S, const V3& o, const V3& d, int32_t last,
+                                               double tol, bool live, Query& q, uint64_t& boxes,
+                                               uint64_t& spheres, uint64_t& retests, PAState* pa = nullptr) {
+    query_init(q);
+    const Ray32 r = ray32(S, o, d);
+    const CInt32 offsets = (CInt32)S.offsets;
+    const CInt32 ids = (CInt32)S.ids;
+    float bestf = INFINITY;
+    int32_t node = 0;
+    const int32_t nn = S.n_nodes;
+    while (node < nn) {
+        const Node32 n = load_node(S.nodes32, node);
+        boxes += live ? 1 : 0;
+        const bool hit = live && box32(r, n, bestf);
+        if (__ballot(hit) == 0) {
+            node = n.end;
+            continue;
+        }
+        const int32_t g = n.leaf;
+        if (g >= 0) {
+            const int32_t k1 = offsets[g + 1];
+            for (int32_t k = offsets[g]; k < k1; ++k) {
+                const int32_t wanted = ids[k];
+                const float4 s = load_sphere32(S.sph32, k);
+                if (!hit || wanted == last) continue;
+                ++spheres;
+                if (!needs_exact(S, r, s, q.best, tol, pa)) continue;
+                ++retests;
+                const int32_t before = q.atom;
+                eval64(S, o, d, wanted, tol, q, pa);
+                if (q.atom != before) bestf = box_bound(S, q.best, tol);
+            }
+        }
+        ++node;
+    }
 }
 
 // pa_spatial.py:114-130.  Counters follow the reference (stop at the first hit);
 // after a near-threshold first hit the scan continues, uncounted, looking for a
 // robust hit so that the grazing flag does not depend on traversal order.
-__device__ __forceinline__ bool pa_plain(const SceneView& S, const V3& o, const V3& d, int32_t owner,
+__device__ __forceinline__ bool pa_plain(const SceneView& S, const V3& o, const V3& d, int32_t owner, double tol,
                                          uint64_t& boxes, uint64_t& tests, bool& graze) {
     int32_t node = 0;
     const int32_t stop = S.pa[0].end;
@@ -184,7 +301,7 @@ __device__ __forceinline__ bool pa_plain(const SceneView& S, const V3& o, const 
             if (counting) ++tests;
             const double4 c = S.sph[j];
             bool critical;
-            const bool h = sphere_hit64_graze(o, d, c.x, c.y, c.z, c.w, &critical);
+            const bool h = sphere_hit64_graze(o, d, c.x, c.y, c.z, c.w, tol, &critical);
             critical_seen |= critical;
             if (h) {
                 hit = true;
@@ -203,13 +320,14 @@ __device__ __forceinline__ bool pa_plain(const SceneView& S, const V3& o, const 
 
 // Any-hit through the FP32 filter on the static tree (any conservative tree gives
 // the same flag).  With x = o - c, b = x.d, cc = |x|^2 - r^2, disc = b^2 - cc:
-// definite miss when disc < -m; definite hit when disc > m and b <= -m_p
-// (then -b + sqrt(disc) > 0); definite miss when b >= m_p and cc > m (origin
-// outside, centre behind); otherwise the FP64 test decides.
-__device__ __forceinline__ bool pa_fast(const SceneView& S, const V3& o, const V3& d, int32_t owner,
+// clear miss when disc < -m; robust hit when disc > m and b <= -m_p (then
+// -b + sqrt(disc) > 0); clear miss when b >= m_p and cc > m (origin outside,
+// centre behind); otherwise the FP64 test decides.  Margins include tol.
+__device__ __forceinline__ bool pa_fast(const SceneView& S, const V3& o, const V3& d, int32_t owner, double tol,
                                         uint64_t& boxes, uint64_t& tests, uint64_t& retests, bool& graze) {
     const Ray32 r = ray32(S, o, d);
-    const float m_p = S.m_p, m = S.m_disc;
+    const float ftol = (float)tol * 1.01f;
+    const float m_p = S.m_p + ftol;
     int32_t node = 0;
     const int32_t nn = S.n_nodes;
     bool hit = false, critical_seen = false;
@@ -228,6 +346,7 @@ __device__ __forceinline__ bool pa_fast(const SceneView& S, const V3& o, const V
                 if (j == owner) continue;
                 ++tests;
                 const float4 s = S.sph32[k];
+                const float m = S.m_disc + 2.01f * s.w * ftol;
                 const float x0 = r.ox - s.x, x1 = r.oy - s.y, x2 = r.oz - s.z;
                 const float b = x0 * r.dx + x1 * r.dy + x2 * r.dz;
                 const float cc = (x0 * x0 + x1 * x1 + x2 * x2) - s.w * s.w;
@@ -237,11 +356,11 @@ __device__ __forceinline__ bool pa_fast(const SceneView& S, const V3& o, const V
                     graze = false;
                     return true;
                 }
-                if (b >= m_p && cc > m) continue;
+                if (b >= m_p && cc > m + 4.0f * (float)S.scale * ftol) continue;
                 ++retests;
                 const double4 c = S.sph[j];
                 bool critical;
-                const bool h = sphere_hit64_graze(o, d, c.x, c.y, c.z, c.w, &critical);
+                const bool h = sphere_hit64_graze(o, d, c.x, c.y, c.z, c.w, tol, &critical);
                 critical_seen |= critical;
                 if (h) {
                     hit = true;

@@ -40,7 +40,8 @@ using namespace ehss;
 
 namespace {
 
-int create_scene(int64_t n, const double* centers, const double* radii, int leaf_size, ehss_scene** out) {
+int create_scene(int64_t n, const double* centers, const double* radii, const int64_t* groups, int leaf_size,
+                 ehss_scene** out) {
     EHSS_TRY(ensure_init());
     EHSS_CHECK(out, "null output handle");
     *out = nullptr;
@@ -51,6 +52,22 @@ int create_scene(int64_t n, const double* centers, const double* radii, int leaf
         EHSS_CHECK(isfinite(centers[3 * i]) && isfinite(centers[3 * i + 1]) && isfinite(centers[3 * i + 2]) &&
                        isfinite(radii[i]) && radii[i] > 0,
                    "Finite centers and positive finite radii required (atom %lld)", (long long)i);
+    }
+    std::vector<int32_t> cg;
+    int32_t custom_ng = 0;
+    if (groups) {
+        // ehss_static.py:71-77: one integer group per atom, contiguous nonempty ids.
+        int64_t gmax = -1;
+        for (int64_t i = 0; i < n; ++i) {
+            EHSS_CHECK(groups[i] >= 0 && groups[i] < n, "Contiguous nonempty groups required");
+            gmax = std::max(gmax, groups[i]);
+        }
+        std::vector<char> seen((size_t)gmax + 1, 0);
+        for (int64_t i = 0; i < n; ++i) seen[groups[i]] = 1;
+        for (int64_t g = 0; g <= gmax; ++g) EHSS_CHECK(seen[g], "Contiguous nonempty groups required");
+        cg.resize((size_t)n);
+        for (int64_t i = 0; i < n; ++i) cg[i] = (int32_t)groups[i];
+        custom_ng = (int32_t)(gmax + 1);
     }
     ehss_scene* sc = new ehss_scene();
     struct Guard {
@@ -89,11 +106,17 @@ int create_scene(int64_t n, const double* centers, const double* radii, int leaf
     EHSS_TRY(sc->pa_ids.alloc(sizeof(int32_t) * n));
     EHSS_HIP(hipMemcpyAsync(sc->centers.p, centers, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
     EHSS_HIP(hipMemcpyAsync(sc->radii.p, radii, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    DevBuf dcg;
+    if (groups) {
+        EHSS_TRY(dcg.alloc(sizeof(int32_t) * n));
+        EHSS_HIP(hipMemcpyAsync(dcg.p, cg.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+    }
     BuiltScene built;
     EHSS_TRY(build_scene_device((int32_t)n, leaf_size, sc->centers.as<double>(), sc->radii.as<double>(),
                                 sc->sph.as<double4>(), sc->groups.as<int32_t>(), sc->offsets.as<int32_t>(),
                                 sc->eids.as<int32_t>(), sc->nodes.as<Node64>(), sc->pa.as<NodePA>(),
-                                sc->pa_ids.as<int32_t>(), &built, s));
+                                sc->pa_ids.as<int32_t>(), groups ? dcg.as<int32_t>() : nullptr, custom_ng, &built,
+                                s));
     sc->ng = built.n_groups;
     sc->nn = built.n_nodes;
     sc->pa_nn = built.pa_nodes;
@@ -140,7 +163,13 @@ int download_as_i64(const ehss::DevBuf& b, int64_t count, int64_t* dst, hipStrea
 extern "C" {
 
 int ehss_scene_create(int64_t n, const double* centers, const double* radii, int leaf_size, ehss_scene** out) {
-    return create_scene(n, centers, radii, leaf_size, out);
+    return create_scene(n, centers, radii, nullptr, leaf_size, out);
+}
+
+int ehss_scene_create_grouped(int64_t n, const double* centers, const double* radii, const int64_t* groups,
+                              int leaf_size, ehss_scene** out) {
+    EHSS_CHECK(groups, "null groups");
+    return create_scene(n, centers, radii, groups, leaf_size, out);
 }
 
 int ehss_scene_destroy(ehss_scene* scene) {

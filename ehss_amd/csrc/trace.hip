@@ -31,10 +31,11 @@ __global__ void __launch_bounds__(256) k_trace(SceneView S, TraceIO io) {
         uint8_t esc = 0, unres = 0;
         for (;;) {
             Query q;
+            // Outputs do not depend on the grazing tolerance; 0 keeps the filter tightest.
             if (kFast)
-                nearest_fast(S, o, d, last, q, boxes, spheres, retests);
+                nearest_fast(S, o, d, last, 0., q, boxes, spheres, retests);
             else
-                nearest_plain(S, o, d, last, q, boxes, spheres);
+                nearest_plain(S, o, d, last, 0., q, boxes, spheres);
             if (q.atom < 0) {
                 esc = 1;
                 break;
@@ -79,7 +80,7 @@ __global__ void __launch_bounds__(256) k_pa_query(SceneView S, PAIO io) {
         const V3 d = {io.d[3 * i], io.d[3 * i + 1], io.d[3 * i + 2]};
         const int32_t owner = io.owners ? (int32_t)io.owners[i] : -1;
         bool graze;
-        io.hits[i] = pa_plain(S, o, d, owner, boxes, tests, graze) ? 1 : 0;
+        io.hits[i] = pa_plain(S, o, d, owner, 0., boxes, tests, graze) ? 1 : 0;
     }
     const uint64_t b = (uint64_t)wave_sum_i64((int64_t)boxes);
     const uint64_t t = (uint64_t)wave_sum_i64((int64_t)tests);

@@ -100,7 +100,7 @@ class GPUScene:
     """Device-resident spheres with the static EHSS tree and the PA tree built
     on the GPU (ehss_scene_create)."""
 
-    def __init__(self, centers, radii, leaf_size: int = 4):
+    def __init__(self, centers, radii, leaf_size: int = 4, groups=None):
         c, r = _f64(centers), _f64(radii)
         if c.ndim != 2 or c.shape[1] != 3 or r.shape != (len(c),) or len(c) == 0:
             raise ValueError("Expected nonempty centers (N,3) and radii (N,).")
@@ -109,7 +109,13 @@ class GPUScene:
         self.centers, self.radii = c, r
         handle = ct.c_void_p()
         t0 = time.perf_counter()
-        L.call("ehss_scene_create", len(r), L.ptr(c), L.ptr(r), leaf_size, ct.byref(handle))
+        if groups is None:
+            L.call("ehss_scene_create", len(r), L.ptr(c), L.ptr(r), leaf_size, ct.byref(handle))
+        else:
+            g = _i64(groups)
+            if g.shape != (len(r),):
+                raise ValueError("One integer group per atom required")
+            L.call("ehss_scene_create_grouped", len(r), L.ptr(c), L.ptr(r), L.ptr(g), leaf_size, ct.byref(handle))
         self.build_s = time.perf_counter() - t0
         self._h = handle
         info = L.SceneInfo()
@@ -193,11 +199,14 @@ class GPUScene:
 
     # -- fused device solve ---------------------------------------------------------
     def solve(self, sobol: L.Sobol, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
-              keep_rays: bool = False, histogram: bool = False) -> dict:
+              keep_rays: bool = False, histogram: bool = False, variant: int = 0) -> dict:
+        """One fused device solve of 2^power rays.  `variant` selects the FP32
+        filter kernel (0 default, 1 per-lane, 2 packet first query, 3 packet
+        throughout); all variants return identical results."""
         totals = L.Totals()
         hist = np.zeros(cap + 1, np.int64) if histogram else None
         ledger = np.zeros(cap + 1, np.float64) if histogram else None
-        flags = (L.FLAG_PA if pa else 0) | (L.FLAG_KEEP_RAYS if keep_rays else 0)
+        flags = (L.FLAG_PA if pa else 0) | (L.FLAG_KEEP_RAYS if keep_rays else 0) | (int(variant) << L.VARIANT_SHIFT)
         L.call("ehss_solve", self.handle, ct.byref(sobol), int(power), int(cap), precision_code(precision),
                flags, ct.byref(totals), L.ptr(hist), L.ptr(ledger))
         out = totals.as_dict()

