@@ -92,6 +92,7 @@ _SIGS = {
     "ehss_scene_pa_query": (_INT, [_P, _I64, _INT] + [_P] * 5),
     "ehss_solve": (_INT, [_P, ct.POINTER(Sobol), _INT, _INT, _INT, ct.c_uint, ct.POINTER(Totals), _P, _P]),
     "ehss_scene_last_rays": (_INT, [_P, _I64, _P, _P, _P]),
+    "ehss_ray_map": (_INT, [ct.POINTER(Sobol), _INT, _P]),
     "ehss_source_rays": (_INT, [_P, ct.POINTER(Sobol), _INT, _P, _P, _P]),
 }
 

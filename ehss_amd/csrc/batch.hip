@@ -23,6 +23,7 @@ struct JobDev {
     SceneView view;
     SolveArgs args;
     ehss_sobol sob;
+    RayMap map;
     int64_t pkt_begin;  // first 64-ray packet of this job in the batch
     int64_t red_begin;  // first reduction block of this job
     int64_t red_blocks;
@@ -65,7 +66,8 @@ __device__ __forceinline__ int find_job(const JobDev* jobs, int n_jobs, int64_t 
 
 template <int kMode, int kPA>
 __global__ void __launch_bounds__(256) k_batch(const JobDev* jobs, int n_jobs, int64_t total_packets,
-                                               unsigned long long* next_packet) {
+                                               unsigned long long* next_packet, RayState* queue,
+                                               uint32_t* queue_count) {
     const int lane = threadIdx.x & 63;
     for (;;) {
         unsigned long long pkt = 0;
@@ -78,10 +80,57 @@ __global__ void __launch_bounds__(256) k_batch(const JobDev* jobs, int n_jobs, i
         const SceneView S = load_uniform(&jobs[j].view);
         const SolveArgs A = load_uniform(&jobs[j].args);
         const CSobol sob = (CSobol)&jobs[j].sob;
+        const CMap map = (CMap)&jobs[j].map;
         const int64_t begin = *(CI64)&jobs[j].pkt_begin;
         uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
-        solve_ray<kMode, kPA>(S, A, sob, (uint32_t)(((int64_t)pkt - begin) * 64 + lane), ctr);
+        solve_ray<kMode, kPA>(S, A, sob, map, (uint32_t)(((int64_t)pkt - begin) * 64 + lane), ctr, queue,
+                              queue_count, j);
         flush_counters(ctr, A.counters);
+    }
+}
+
+// Drains the batch queue; each queued ray carries its job, whose scene and
+// arguments are read from the job table.  Counters go to the ray's job.
+__global__ void __launch_bounds__(256) k_drain_batch(const JobDev* jobs, const RayState* queue,
+                                                     const uint32_t* queue_count, uint32_t* queue_head) {
+    const uint32_t len = __builtin_amdgcn_readfirstlane(*queue_count);
+    RayCtx R;
+    R.done = true;
+    int32_t job = 0;
+    bool have = false, exhausted = false;
+    const int lane = threadIdx.x & 63;
+    uint64_t boxes = 0, spheres = 0, retests = 0;  // of the current ray, flushed when it finishes
+    for (;;) {
+        const uint64_t idle = __ballot(!have);
+        if (idle && !exhausted) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(queue_head, (uint32_t)__popcll(idle));
+            base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
+            if (base + (uint32_t)__popcll(idle) >= len) exhausted = true;
+            if (!have) {
+                const uint32_t slot = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (slot < len) {
+                    job = ray_job(queue + slot);
+                    ray_load(queue + slot, jobs[job].view, R);
+                    have = true;
+                }
+            }
+        }
+        if (__ballot(have) == 0) break;
+        if (have) {
+            const JobDev& J = jobs[job];
+            Query q;
+            nearest_fast(J.view, R.o, R.d, R.last, R.tol, q, boxes, spheres, retests);
+            ray_process(J.view, J.args.cap, R, q);
+            if (R.done) {
+                ray_record(J.args, R);
+                atomicAdd(&J.args.counters[0], (unsigned long long)boxes);
+                atomicAdd(&J.args.counters[1], (unsigned long long)spheres);
+                if (retests) atomicAdd(&J.args.counters[2], (unsigned long long)retests);
+                boxes = spheres = retests = 0;
+                have = false;
+            }
+        }
     }
 }
 
@@ -115,8 +164,10 @@ struct ehss_batch {
     bool pa = false, use_graph = false, captured = false, dirty = true, keep = false;
     std::vector<JobDev> jobs;
     std::vector<ehss_scene*> scenes;
-    DevBuf jobs_dev, info, contrib, partials, words;
-    unsigned long long* next = nullptr;  // last word of `words`
+    DevBuf jobs_dev, info, contrib, partials, words, queue;
+    unsigned long long* next = nullptr;  // packet counter: word zero_words - 2 of `words`
+    uint32_t* qwords = nullptr;          // queue length and head: last word of `words`
+    int64_t rays = 0;
     int64_t total_packets = 0, total_red = 0, zero_words = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -133,21 +184,28 @@ int enqueue_batch(ehss_batch* b) {
                        b->zero_words);
     const JobDev* jd = b->jobs_dev.as<JobDev>();
     unsigned long long* next = b->next;
+    RayState* qbuf = b->variant == 4 ? b->queue.as<RayState>() : nullptr;
     const dim3 grid((unsigned)b->blocks), block(256);
 #define EHSS_BATCH(M)                                                                                         \
     do {                                                                                                      \
         if (b->pa)                                                                                            \
-            hipLaunchKernelGGL((k_batch<M, 1>), grid, block, 0, s, jd, b->n_jobs, b->total_packets, next);    \
+            hipLaunchKernelGGL((k_batch<M, 1>), grid, block, 0, s, jd, b->n_jobs, b->total_packets, next, qbuf,   \
+                               b->qwords);                                                                    \
         else                                                                                                  \
-            hipLaunchKernelGGL((k_batch<M, 0>), grid, block, 0, s, jd, b->n_jobs, b->total_packets, next);    \
+            hipLaunchKernelGGL((k_batch<M, 0>), grid, block, 0, s, jd, b->n_jobs, b->total_packets, next, qbuf,   \
+                               b->qwords);                                                                    \
     } while (0)
     switch (b->variant) {
         case 0: EHSS_BATCH(0); break;
         case 1: EHSS_BATCH(1); break;
         case 2: EHSS_BATCH(2); break;
-        default: EHSS_BATCH(3); break;
+        case 3: EHSS_BATCH(3); break;
+        default: EHSS_BATCH(4); break;
     }
 #undef EHSS_BATCH
+    if (b->variant == 4)
+        hipLaunchKernelGGL(k_drain_batch, dim3((unsigned)drain_blocks(b->rays)), dim3(256), 0, s, jd, qbuf, b->qwords,
+                           b->qwords + 1);
     hipLaunchKernelGGL(k_reduce_batch, dim3((unsigned)b->total_red), dim3(256), 0, s, jd, b->n_jobs);
     hipLaunchKernelGGL(k_finalize_batch, dim3((unsigned)b->n_jobs), dim3(256), 0, s, jd);
     EHSS_HIP(hipGetLastError());
@@ -175,7 +233,7 @@ int create_batch(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols
     int variant = (int)((flags >> EHSS_VARIANT_SHIFT) & 0xFu);
     if (precision == EHSS_PRECISION_FP64) variant = 0;
     else if (variant == 0) variant = 2;
-    EHSS_CHECK(variant >= 0 && variant <= 3, "unknown kernel variant %d", variant);
+    EHSS_CHECK(variant >= 0 && variant <= 4, "unknown kernel variant %d", variant);
     for (int j = 0; j < n_jobs; ++j) {
         EHSS_CHECK(scenes[j], "null scene for job %d", j);
         EHSS_CHECK(powers[j] >= 0 && powers[j] <= 26, "power of job %d out of range 0..26", j);
@@ -213,12 +271,15 @@ int create_batch(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols
     EHSS_TRY(b->partials.alloc(sizeof(Partial) * b->total_red));
     // words, then histogram and ledger bins in one zero-filled block; the packet
     // counter is its last word.
-    b->zero_words = (int64_t)kWords * n_jobs + 2 * bins + 1;
+    b->zero_words = (int64_t)kWords * n_jobs + 2 * bins + 2;
+    b->rays = rays;
+    if (variant == 4) EHSS_TRY(b->queue.alloc(sizeof(RayState) * rays));
     EHSS_TRY(b->words.alloc(sizeof(unsigned long long) * b->zero_words));
     unsigned long long* W = b->words.as<unsigned long long>();
     unsigned long long* H = W + (int64_t)kWords * n_jobs;
     unsigned long long* LG = H + bins;
-    b->next = W + (b->zero_words - 1);
+    b->next = W + (b->zero_words - 2);
+    b->qwords = reinterpret_cast<uint32_t*>(W + (b->zero_words - 1));
     b->jobs.resize(n_jobs);
     for (int j = 0; j < n_jobs; ++j) {
         ehss_scene* sc = scenes[j];
@@ -238,6 +299,7 @@ int create_batch(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols
         J.args.contrib = b->contrib.as<double>() + ray_off[j];
         J.args.counters = W + (int64_t)kWords * j;
         J.sob = sobols[j];
+        make_ray_map(J.sob, powers[j], J.map.col);
         J.pkt_begin = pkt_off[j];
         J.red_begin = red_off[j];
         J.red_blocks = (n + kReduceRays - 1) / kReduceRays;
@@ -353,6 +415,7 @@ int ehss_batch_set_sobol(ehss_batch* batch, int job, const ehss_sobol* sobol) {
     EHSS_CHECK(batch && sobol, "null batch/sobol");
     EHSS_CHECK(job >= 0 && job < batch->n_jobs, "job %d out of range", job);
     batch->jobs[job].sob = *sobol;
+    make_ray_map(*sobol, batch->jobs[job].args.power, batch->jobs[job].map.col);
     batch->dirty = true;
     return EHSS_OK;
 }

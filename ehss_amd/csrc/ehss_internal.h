@@ -131,6 +131,11 @@ struct RayRecords {
 };
 
 constexpr uint32_t kBounceMask = 0xFFFFFu;
+
+int64_t drain_blocks(int64_t n);
+
+// Wave-coherent ray map columns for a scrambled net of 2^m points (host).
+void make_ray_map(const ehss_sobol& sob, int m, uint32_t cols[32]);
 constexpr int kFlagShift = 20;
 
 }  // namespace ehss
@@ -145,7 +150,7 @@ struct ehss_scene {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // solve workspace (grow-only)
-    ehss::DevBuf ray_info, ray_contrib, partials, hist, ledger, counters;
+    ehss::DevBuf ray_info, ray_contrib, partials, hist, ledger, counters, queue;
     int64_t last_rays = 0;
     double last_weight = 0;
     int64_t device_bytes() const;

@@ -37,9 +37,42 @@ struct Partial {
     long long escaped, unresolved, pa_hits, ehss_hits, grazing, pad;
 };
 
-__device__ __forceinline__ uint32_t ray_index(uint32_t t, int power) {
-    if (power < 6) return t;
-    return ((t & 63u) << (power - 6)) | (t >> 6);
+// Wave-coherent ray map.  For n = 2^m rays the 64 rays of wave w are the points of
+// one elementary cell of the scrambled net: the top a_d output bits of every
+// coordinate d are fixed (sum a_d = m - 6).  Those bits are GF(2)-linear in the
+// Gray-code index g, so with T the invertible m x m matrix whose first m - 6 rows
+// are the corresponding rows of the scrambled generator matrices,
+// g = T^-1 (bits of w || bits of lane); col[k] is column k of T^-1.  The ray index
+// (record position) is the inverse Gray code of g.
+struct RayMap {
+    uint32_t col[32];
+};
+
+__device__ __forceinline__ uint32_t inverse_gray(uint32_t g) {
+    g ^= g >> 1;
+    g ^= g >> 2;
+    g ^= g >> 4;
+    g ^= g >> 8;
+    g ^= g >> 16;
+    return g;
+}
+
+typedef const __attribute__((address_space(4))) uint32_t* CMap;
+__device__ __forceinline__ uint32_t map_col(const RayMap& m, int k) { return m.col[k]; }
+__device__ __forceinline__ uint32_t map_col(CMap m, int k) { return m[k]; }
+
+// Gray index of thread t (wave t >> 6, lane t & 63) for a net of 2^power points.
+template <class Map>
+__device__ __forceinline__ uint32_t ray_gray(Map map, uint32_t t, int power) {
+    if (power < 6) return t ^ (t >> 1);
+    const uint32_t w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const uint32_t lane = t & 63u;
+    uint32_t gw = 0, gl = 0;
+    for (int k = 0; k < power - 6; ++k)
+        if ((w >> k) & 1u) gw ^= map_col(map, k);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) gl ^= map_col(map, power - 6 + k) & (0u - ((lane >> k) & 1u));
+    return gw ^ gl;
 }
 
 // Scrambled direction numbers read through the constant address space (scalar loads).
@@ -50,9 +83,9 @@ __device__ __forceinline__ uint32_t sob_shift(const ehss_sobol& s, int d) { retu
 __device__ __forceinline__ uint32_t sob_sv(CSobol s, int d, int j) { return s[d * 32 + j]; }
 __device__ __forceinline__ uint32_t sob_shift(CSobol s, int d) { return s[128 + d]; }
 
+// Point with Gray-code index g: q = shift ^ XOR_{bit j of g} sv[j].
 template <class Sob>
-__device__ __forceinline__ void sobol_point(Sob sob, uint32_t i, double u[4]) {
-    const uint32_t g = i ^ (i >> 1);
+__device__ __forceinline__ void sobol_point_gray(Sob sob, uint32_t g, double u[4]) {
     uint32_t q0 = sob_shift(sob, 0), q1 = sob_shift(sob, 1), q2 = sob_shift(sob, 2), q3 = sob_shift(sob, 3);
 #pragma unroll
     for (int j = 0; j < 30; ++j) {
@@ -110,110 +143,259 @@ __device__ __forceinline__ void source_ray(const double u[4], double cx, double 
 //   3 FP32 filter, packet traversal for every bounce.
 // Variants 1-3 return bit-identical per-ray results (the filter only skips
 // spheres whose exact FP64 outcome is already decided).
-template <int kMode, int kPA, class Sob>
-__device__ __forceinline__ void solve_ray(const SceneView& S, const SolveArgs& A, Sob sob, uint32_t t,
-                                          uint64_t* ctr) {
+// State of one ray between queries.
+struct RayCtx {
+    V3 o, d, inc;
+    PathError pe;
+    double tol;
+    uint32_t i, flags;
+    int32_t last, count;
+    bool done, escaped;
+};
+
+// Queue entry of a ray that is still bouncing (two-phase variant); `job` selects
+// the batch job (0 for a single solve).
+struct alignas(8) RayState {
+    double o[3], d[3], inc[3];
+    double ep, ed;
+    uint32_t i, flags;
+    int32_t last, count;
+    int32_t job, pad;
+};
+
+// One query result applied to the ray: escape, cap (unresolved) or reflection
+// (ehss_cpu.cpp:73-77), plus the grazing flag and the path error.
+__device__ __forceinline__ void ray_process(const SceneView& S, int32_t cap, RayCtx& R, const Query& q) {
+    if (grazing(q, R.tol)) R.flags |= EHSS_RAY_GRAZING;
+    if (q.atom < 0) {
+        R.escaped = true;
+        R.done = true;
+    } else if (R.count >= cap) {
+        R.flags |= EHSS_RAY_UNRESOLVED;
+        R.done = true;
+    } else {
+        const double4 c = S.sph[q.atom];
+        const double cos_inc = reflect64(R.o, R.d, c.x, c.y, c.z, c.w, q.best);
+        path_error_reflect(R.pe, q.best, cos_inc, c.w, S.scale);
+        R.tol = path_tol(R.pe, S.scale);
+        ++R.count;
+        R.last = q.atom;
+    }
+}
+
+// The ray's record (ehss_state.py:84-87): bounces and flags, w (1 - u.v) if escaped.
+__device__ __forceinline__ void ray_record(const SolveArgs& A, RayCtx& R) {
+    double contrib = 0.;
+    if (R.escaped) {
+        R.flags |= EHSS_RAY_ESCAPED;
+        // clip(1 - sum(incoming * outgoing), 0, 2) * w
+        double angle = 1 - ((R.inc.x * R.d.x + R.inc.y * R.d.y) + R.inc.z * R.d.z);
+        angle = angle < 0. ? 0. : (angle > 2. ? 2. : angle);
+        contrib = A.w * angle;
+    }
+    if (R.count > 0 || (R.flags & EHSS_RAY_UNRESOLVED)) R.flags |= EHSS_RAY_EHSS_HIT;
+    A.info[R.i] = (uint32_t)R.count | (R.flags << kFlagShift);
+    A.contrib[R.i] = contrib;
+}
+
+__device__ __forceinline__ void ray_save(const RayCtx& R, int32_t job, RayState* q) {
+    RayState st;
+    st.o[0] = R.o.x;
+    st.o[1] = R.o.y;
+    st.o[2] = R.o.z;
+    st.d[0] = R.d.x;
+    st.d[1] = R.d.y;
+    st.d[2] = R.d.z;
+    st.inc[0] = R.inc.x;
+    st.inc[1] = R.inc.y;
+    st.inc[2] = R.inc.z;
+    st.ep = R.pe.ep;
+    st.ed = R.pe.ed;
+    st.i = R.i;
+    st.flags = R.flags;
+    st.last = R.last;
+    st.count = R.count;
+    st.job = job;
+    st.pad = 0;
+    *q = st;
+}
+
+__device__ __forceinline__ int32_t ray_job(const RayState* q) { return q->job; }
+
+__device__ __forceinline__ int32_t ray_load(const RayState* q, const SceneView& S, RayCtx& R) {
+    const RayState st = *q;
+    R.o = V3{st.o[0], st.o[1], st.o[2]};
+    R.d = V3{st.d[0], st.d[1], st.d[2]};
+    R.inc = V3{st.inc[0], st.inc[1], st.inc[2]};
+    R.pe = PathError{st.ep, st.ed};
+    R.tol = path_tol(R.pe, S.scale);
+    R.i = st.i;
+    R.flags = st.flags;
+    R.last = st.last;
+    R.count = st.count;
+    R.done = false;
+    R.escaped = false;
+    return st.job;
+}
+
+// Number of wave-uniform queries a ray gets in the two-phase variant (4) before
+// it is queued for the per-lane drain kernel.
+constexpr int kPacketQueries = 2;
+
+// Kernel variants (template kMode):
+//   0 plain FP64: reference traversal, separate reference PA traversal, reference counters;
+//   1 FP32 filter, one ray per lane, PA fused into the first query;
+//   2 FP32 filter, wave-uniform (packet) first query with fused PA, later bounces per lane;
+//   3 FP32 filter, packet traversal for every bounce;
+//   4 FP32 filter, packet traversal for the first kPacketQueries queries, then the rays
+//     still bouncing are compacted into a queue (wave ballot, one atomic per wave)
+//     and finished by the persistent per-lane drain kernel.
+// Variants 1-4 return bit-identical per-ray results (the filter only skips spheres
+// whose exact FP64 outcome is already decided).
+template <int kMode, int kPA, class Sob, class Map>
+__device__ __forceinline__ void solve_ray(const SceneView& S, const SolveArgs& A, Sob sob, Map map, uint32_t t,
+                                          uint64_t* ctr, RayState* queue = nullptr, uint32_t* queue_count = nullptr,
+                                          int32_t job = 0) {
     const bool valid = t < (uint64_t)A.n;
     uint64_t boxes = 0, spheres = 0, retests = 0, pboxes = 0, ptests = 0, pretests = 0;
-    const uint32_t i = valid ? ray_index(t, A.power) : 0u;
-    V3 o = {0., 0., 0.}, d = {0., 0., 1.};
+    RayCtx R;
+    const uint32_t g = valid ? ray_gray(map, t, A.power) : 0u;
+    R.i = inverse_gray(g);
+    R.o = V3{0., 0., 0.};
+    R.d = V3{0., 0., 1.};
     if (valid) {
         double u[4];
-        sobol_point(sob, i, u);
-        source_ray(u, A.cx, A.cy, A.cz, A.radius, o, d);
+        sobol_point_gray(sob, g, u);
+        source_ray(u, A.cx, A.cy, A.cz, A.radius, R.o, R.d);
     }
-    const V3 inc = d;
-    uint32_t flags = 0;
-    int32_t last = -1;
-    int32_t count = 0;
-    bool done = !valid, escaped = false;
+    R.inc = R.d;
+    R.flags = 0;
+    R.last = -1;
+    R.count = 0;
+    R.done = !valid;
+    R.escaped = false;
+    R.pe = path_error_start(S.scale);
+    R.tol = path_tol(R.pe, S.scale);
     Query q;
-    PathError pe = path_error_start(S.scale);
-    double tol = path_tol(pe, S.scale);
-    auto process = [&]() {
-        if (grazing(q, tol)) flags |= EHSS_RAY_GRAZING;
-        if (q.atom < 0) {
-            escaped = true;
-            done = true;
-        } else if (count >= A.cap) {
-            flags |= EHSS_RAY_UNRESOLVED;
-            done = true;
-        } else {
-            const double4 c = S.sph[q.atom];
-            const double cos_inc = reflect64(o, d, c.x, c.y, c.z, c.w, q.best);
-            path_error_reflect(pe, q.best, cos_inc, c.w, S.scale);
-            tol = path_tol(pe, S.scale);
-            ++count;
-            last = q.atom;
-        }
-    };
+    bool record = valid;
     if (kMode == 0) {
         if (valid) {
             if (kPA) {
                 bool pgraze = false;
-                if (pa_plain(S, o, d, -1, tol, pboxes, ptests, pgraze)) flags |= EHSS_RAY_PA_HIT;
-                if (pgraze) flags |= EHSS_RAY_GRAZING;
+                if (pa_plain(S, R.o, R.d, -1, R.tol, pboxes, ptests, pgraze)) R.flags |= EHSS_RAY_PA_HIT;
+                if (pgraze) R.flags |= EHSS_RAY_GRAZING;
             }
-            while (!done) {
-                nearest_plain(S, o, d, last, tol, q, boxes, spheres);
-                process();
+            while (!R.done) {
+                nearest_plain(S, R.o, R.d, R.last, R.tol, q, boxes, spheres);
+                ray_process(S, A.cap, R, q);
             }
         }
     } else {
         PAState pa = {false, false, false};
         PAState* pap = kPA ? &pa : nullptr;
         if (kMode == 1) {
-            if (valid) nearest_fast(S, o, d, -1, tol, q, boxes, spheres, retests, pap);
+            if (valid) nearest_fast(S, R.o, R.d, -1, R.tol, q, boxes, spheres, retests, pap);
         } else {
-            nearest_packet(S, o, d, -1, tol, valid, q, boxes, spheres, retests, pap);
+            nearest_packet(S, R.o, R.d, -1, R.tol, valid, q, boxes, spheres, retests, pap);
         }
         if (valid) {
             if (kPA) {
                 bool hit = pa.hit, pgraze = false;
                 if (!pa.decided) {
                     if (q.atom >= 0)  // a winner without a robust any-hit: run the full any-hit query
-                        hit = pa_fast(S, o, d, -1, tol, pboxes, ptests, pretests, pgraze);
+                        hit = pa_fast(S, R.o, R.d, -1, R.tol, pboxes, ptests, pretests, pgraze);
                     else
                         pgraze = pa.critical;
                 }
-                if (hit) flags |= EHSS_RAY_PA_HIT;
-                if (pgraze) flags |= EHSS_RAY_GRAZING;
+                if (hit) R.flags |= EHSS_RAY_PA_HIT;
+                if (pgraze) R.flags |= EHSS_RAY_GRAZING;
             }
-            process();
+            ray_process(S, A.cap, R, q);
         }
-        if (kMode == 3) {
-            for (;;) {
-                const bool live = !done;
+        if (kMode == 3 || kMode == 4) {
+            for (int k = 1; kMode == 3 || k < kPacketQueries; ++k) {
+                const bool live = !R.done;
                 if (__ballot(live) == 0) break;
-                nearest_packet(S, o, d, last, tol, live, q, boxes, spheres, retests);
-                if (live) process();
+                nearest_packet(S, R.o, R.d, R.last, R.tol, live, q, boxes, spheres, retests);
+                if (live) ray_process(S, A.cap, R, q);
+            }
+            if (kMode == 4) {
+                const bool alive = !R.done;
+                const uint64_t m = __ballot(alive);
+                if (m) {
+                    uint32_t base = 0;
+                    if ((threadIdx.x & 63) == 0) base = atomicAdd(queue_count, (uint32_t)__popcll(m));
+                    base = __shfl(base, 0, 64);
+                    if (alive) {
+                        const uint64_t below = m & ((1ull << (threadIdx.x & 63)) - 1ull);
+                        ray_save(R, job, queue + base + (uint32_t)__popcll(below));
+                        record = false;  // finished by the drain kernel
+                    }
+                }
             }
         } else {
-            while (!done) {
-                nearest_fast(S, o, d, last, tol, q, boxes, spheres, retests);
-                process();
+            while (!R.done) {
+                nearest_fast(S, R.o, R.d, R.last, R.tol, q, boxes, spheres, retests);
+                ray_process(S, A.cap, R, q);
             }
         }
     }
-    if (valid) {
-        double contrib = 0.;
-        if (escaped) {
-            flags |= EHSS_RAY_ESCAPED;
-            // ehss_state.py:86-87: clip(1 - sum(incoming * outgoing), 0, 2) * w
-            double angle = 1 - ((inc.x * d.x + inc.y * d.y) + inc.z * d.z);
-            angle = angle < 0. ? 0. : (angle > 2. ? 2. : angle);
-            contrib = A.w * angle;
-        }
-        if (count > 0 || (flags & EHSS_RAY_UNRESOLVED)) flags |= EHSS_RAY_EHSS_HIT;
-        A.info[i] = (uint32_t)count | (flags << kFlagShift);
-        A.contrib[i] = contrib;
-    }
+    if (record) ray_record(A, R);
     ctr[0] += boxes;
     ctr[1] += spheres;
     ctr[2] += retests;
     ctr[3] += pboxes;
     ctr[4] += ptests;
     ctr[5] += pretests;
+}
+
+// Persistent per-lane drain of queued rays.  Idle lanes refill from the queue
+// (one atomic per wave); each live lane runs one query per iteration, so lanes
+// never wait for another lane's long path.  Every wave exits once the queue is
+// exhausted and its lanes are idle.  `scene_of(job)` / `args_of(job)` give the
+// scene and solve arguments of a queued ray's job.
+template <class SceneOf, class ArgsOf>
+__device__ __forceinline__ void drain_queue(const RayState* queue, uint32_t queue_len, uint32_t* queue_head,
+                                            SceneOf scene_of, ArgsOf args_of, uint64_t* ctr) {
+    RayCtx R;
+    R.done = true;
+    int32_t job = 0;
+    bool have = false;
+    const int lane = threadIdx.x & 63;
+    uint64_t boxes = 0, spheres = 0, retests = 0;
+    bool exhausted = false;  // wave-uniform: the queue head has passed the end
+    for (;;) {
+        const uint64_t idle = __ballot(!have);
+        if (idle && !exhausted) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(queue_head, (uint32_t)__popcll(idle));
+            base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
+            // One grab past the end at most per wave, so the head cannot wrap.
+            if (base + (uint32_t)__popcll(idle) >= queue_len) exhausted = true;
+            if (!have) {
+                const uint32_t slot = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (slot < queue_len) {
+                    job = ray_load(queue + slot, scene_of(ray_job(queue + slot)), R);
+                    have = true;
+                }
+            }
+        }
+        if (__ballot(have) == 0) break;
+        if (have) {
+            const SceneView& S = scene_of(job);
+            Query q;
+            nearest_fast(S, R.o, R.d, R.last, R.tol, q, boxes, spheres, retests);
+            const SolveArgs& A = args_of(job);
+            ray_process(S, A.cap, R, q);
+            if (R.done) {
+                ray_record(A, R);
+                have = false;
+            }
+        }
+    }
+    ctr[0] += boxes;
+    ctr[1] += spheres;
+    ctr[2] += retests;
 }
 
 // Adds per-lane counters into counters[0..5] with one atomic per wave (order-free).

@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <vector>
 
 #include "ehss_solve.h"
@@ -15,10 +16,52 @@ namespace ehss {
 namespace {
 
 template <int kMode, int kPA>
-__global__ void __launch_bounds__(256) k_solve(SceneView S, SolveArgs A, ehss_sobol sob) {
+__global__ void __launch_bounds__(256) k_solve(SceneView S, SolveArgs A, ehss_sobol sob, RayMap map,
+                                               RayState* queue, uint32_t* queue_count) {
     uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
-    solve_ray<kMode, kPA>(S, A, sob, blockIdx.x * blockDim.x + threadIdx.x, ctr);
+    solve_ray<kMode, kPA>(S, A, sob, map, blockIdx.x * blockDim.x + threadIdx.x, ctr, queue, queue_count, 0);
     flush_counters(ctr, A.counters);
+}
+
+// Persistent per-lane drain of the rays the two-phase variant queued.  With
+// kLds the FP32 filter tree, leaf offsets, slot ids and FP32 spheres are first
+// copied into LDS, so every node and sphere fetch of the per-lane traversal is an
+// LDS access instead of a dependent L2 round trip.
+template <int kLds>
+__global__ void __launch_bounds__(256) k_drain(SceneView S, SolveArgs A, const RayState* queue,
+                                               const uint32_t* queue_count, uint32_t* queue_head) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
+    const uint32_t len = __builtin_amdgcn_readfirstlane(*queue_count);
+    if (len == 0) return;
+    SceneView L = S;
+    if (kLds) {
+        Node32* nodes = reinterpret_cast<Node32*>(lds);
+        float4* sph = reinterpret_cast<float4*>(nodes + S.n_nodes);
+        int32_t* offsets = reinterpret_cast<int32_t*>(sph + S.n_atoms);
+        int32_t* ids = offsets + (S.n_groups + 1);
+        for (int32_t k = threadIdx.x; k < S.n_nodes; k += blockDim.x) nodes[k] = S.nodes32[k];
+        for (int32_t k = threadIdx.x; k < S.n_atoms; k += blockDim.x) {
+            sph[k] = S.sph32[k];
+            ids[k] = S.ids[k];
+        }
+        for (int32_t k = threadIdx.x; k <= S.n_groups; k += blockDim.x) offsets[k] = S.offsets[k];
+        __syncthreads();
+        L.nodes32 = nodes;
+        L.sph32 = sph;
+        L.offsets = offsets;
+        L.ids = ids;
+    }
+    drain_queue(
+        queue, len, queue_head, [&](int32_t) -> const SceneView& { return L; },
+        [&](int32_t) -> const SolveArgs& { return A; }, ctr);
+    flush_counters(ctr, A.counters);
+}
+
+// LDS bytes of a scene staged by k_drain<1>.
+size_t drain_lds_bytes(const SceneView& v) {
+    return sizeof(Node32) * v.n_nodes + sizeof(float4) * v.n_atoms + sizeof(int32_t) * (v.n_groups + 1) +
+           sizeof(int32_t) * v.n_atoms;
 }
 
 __global__ void k_zero(unsigned long long* p, int64_t n) {
@@ -42,7 +85,8 @@ __global__ void k_source_rays(int64_t n, double cx, double cy, double cz, double
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double u[4];
-    sobol_point(sob, (uint32_t)i, u);
+    const uint32_t ii = (uint32_t)i;
+    sobol_point_gray(sob, ii ^ (ii >> 1), u);
     V3 o, d;
     source_ray(u, cx, cy, cz, R, o, d);
     if (u_out)
@@ -61,6 +105,120 @@ __global__ void k_source_rays(int64_t n, double cx, double cy, double cz, double
 
 }  // namespace
 
+// Builds the wave-coherent ray map (see RayMap).  Row p (MSB-first) of the
+// scrambled generator matrix of dimension d, restricted to the first m index
+// bits, is bit (29 - p) of sv[d][j] for j < m.  The cell widths a_d are chosen
+// as balanced as possible (angles first) among partitions whose stacked rows are
+// linearly independent; without one, the identity map (lanes = low index bits)
+// is used.
+void make_ray_map(const ehss_sobol& sob, int m, uint32_t cols[32]) {
+    for (int k = 0; k < 32; ++k) cols[k] = 0;
+    if (m < 6) {
+        for (int k = 0; k < m; ++k) cols[k] = 1u << k;
+        return;
+    }
+    const int cells = m - 6;
+    auto row = [&](int d, int p) {
+        uint32_t r = 0;
+        for (int j = 0; j < m; ++j) r |= ((sob.sv[d][j] >> (29 - p)) & 1u) << j;
+        return r;
+    };
+    // Candidate partitions, most balanced first; ties favour the angular
+    // coordinates (dimension 1 = azimuth, 3 = impact phase).
+    std::vector<std::array<int, 4>> parts;
+    for (int a0 = 0; a0 <= cells; ++a0)
+        for (int a1 = 0; a0 + a1 <= cells; ++a1)
+            for (int a2 = 0; a0 + a1 + a2 <= cells; ++a2) {
+                const int a3 = cells - a0 - a1 - a2;
+                if (a0 > 30 || a1 > 30 || a2 > 30 || a3 > 30) continue;
+                parts.push_back({a0, a1, a2, a3});
+            }
+    auto cost = [](const std::array<int, 4>& a) {
+        const int mx = std::max({a[0], a[1], a[2], a[3]}), mn = std::min({a[0], a[1], a[2], a[3]});
+        return (mx - mn) * 16 - (a[1] + a[3]);
+    };
+    std::stable_sort(parts.begin(), parts.end(),
+                     [&](const std::array<int, 4>& x, const std::array<int, 4>& y) { return cost(x) < cost(y); });
+    for (const auto& a : parts) {
+        // T rows: cell rows first, then unit rows completing the basis.
+        uint32_t T[32];
+        int nr = 0;
+        for (int d = 0; d < 4; ++d)
+            for (int p = 0; p < a[d]; ++p) T[nr++] = row(d, p);
+        // Gaussian elimination on a copy to test rank and find pivot columns.
+        uint32_t E[32];
+        for (int k = 0; k < nr; ++k) E[k] = T[k];
+        uint32_t pivots = 0;
+        int rank = 0;
+        for (int c = 0; c < m && rank < nr; ++c) {
+            int sel = -1;
+            for (int k = rank; k < nr; ++k)
+                if ((E[k] >> c) & 1u) {
+                    sel = k;
+                    break;
+                }
+            if (sel < 0) continue;
+            std::swap(E[rank], E[sel]);
+            for (int k = 0; k < nr; ++k)
+                if (k != rank && ((E[k] >> c) & 1u)) E[k] ^= E[rank];
+            pivots |= 1u << c;
+            ++rank;
+        }
+        if (rank < nr) continue;
+        for (int c = 0; c < m; ++c)
+            if (!((pivots >> c) & 1u)) T[nr++] = 1u << c;  // completes to full rank
+        // Invert T (m x m over GF(2)): [T | I] -> [I | T^-1]; rows as bit masks.
+        uint32_t A[32], B[32];
+        for (int k = 0; k < m; ++k) {
+            A[k] = T[k];
+            B[k] = 1u << k;
+        }
+        bool ok = true;
+        for (int c = 0; c < m && ok; ++c) {
+            int sel = -1;
+            for (int k = c; k < m; ++k)
+                if ((A[k] >> c) & 1u) {
+                    sel = k;
+                    break;
+                }
+            if (sel < 0) {
+                ok = false;
+                break;
+            }
+            std::swap(A[c], A[sel]);
+            std::swap(B[c], B[sel]);
+            for (int k = 0; k < m; ++k)
+                if (k != c && ((A[k] >> c) & 1u)) {
+                    A[k] ^= A[c];
+                    B[k] ^= B[c];
+                }
+        }
+        if (!ok) continue;
+        // Row c of T^-1 is B[c]; column k of T^-1 collects bit k of every row.
+        for (int k = 0; k < m; ++k) {
+            uint32_t col = 0;
+            for (int c = 0; c < m; ++c) col |= ((B[c] >> k) & 1u) << c;
+            cols[k] = col;
+        }
+        return;
+    }
+    for (int k = 0; k < m; ++k) cols[k] = k < cells ? 1u << (k + 6) : 1u << (k - cells);
+}
+
+// Persistent drain grid: enough waves to keep every SIMD busy, never more than
+// the rays that could be queued.
+int64_t drain_blocks(int64_t n) {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            cus = prop.multiProcessorCount;
+        if (cus <= 0) cus = 256;
+    }
+    return std::max<int64_t>(1, std::min<int64_t>((int64_t)cus * 8, (n + 255) / 256));
+}
+
 int launch_zero(unsigned long long* p, int64_t n, hipStream_t s) {
     if (n <= 0) return EHSS_OK;
     const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
@@ -77,8 +235,10 @@ namespace {
 
 // Workspace layout in scene->counters: [0..5] kernel counters, [6] max bounces,
 // [8..] final Partial.
-constexpr int kCounterWords = 8 + (int)(sizeof(Partial) / 8);
+constexpr int kQueueWord = 8 + (int)(sizeof(Partial) / 8);  // queue length, queue head (two u32)
+constexpr int kCounterWords = kQueueWord + 1;
 constexpr int kDefaultVariant = 2;
+constexpr size_t kDrainLdsLimit = 144 * 1024;  // of the 160 KB per CU
 
 int solve_impl(ehss_scene* sc, const ehss_sobol* sob, int power, int cap, int precision, unsigned flags,
                ehss_totals* totals, int64_t* histogram, double* ledger_out) {
@@ -118,22 +278,47 @@ int solve_impl(ehss_scene* sc, const ehss_sobol* sob, int power, int cap, int pr
     int variant = (int)((flags >> EHSS_VARIANT_SHIFT) & 0xFu);
     if (precision == EHSS_PRECISION_FP64) variant = 0;
     else if (variant == 0) variant = kDefaultVariant;
-    EHSS_CHECK(variant >= 0 && variant <= 3, "unknown kernel variant %d", variant);
+    EHSS_CHECK(variant >= 0 && variant <= 4, "unknown kernel variant %d", variant);
+    RayState* qbuf = nullptr;
+    uint32_t* qcount = reinterpret_cast<uint32_t*>(ctr + kQueueWord);
+    if (variant == 4) {
+        EHSS_TRY(sc->queue.reserve(sizeof(RayState) * n));
+        qbuf = sc->queue.as<RayState>();
+    }
+    RayMap rmap;
+    make_ray_map(*sob, power, rmap.col);
     EHSS_HIP(hipEventRecord(sc->ev0, s));
 #define EHSS_LAUNCH(M)                                                                          \
     do {                                                                                        \
         if (pa)                                                                                 \
-            hipLaunchKernelGGL((k_solve<M, 1>), grid, block, 0, s, sc->view, A, *sob);          \
+            hipLaunchKernelGGL((k_solve<M, 1>), grid, block, 0, s, sc->view, A, *sob, rmap, qbuf, qcount); \
         else                                                                                    \
-            hipLaunchKernelGGL((k_solve<M, 0>), grid, block, 0, s, sc->view, A, *sob);          \
+            hipLaunchKernelGGL((k_solve<M, 0>), grid, block, 0, s, sc->view, A, *sob, rmap, qbuf, qcount); \
     } while (0)
     switch (variant) {
         case 0: EHSS_LAUNCH(0); break;
         case 1: EHSS_LAUNCH(1); break;
         case 2: EHSS_LAUNCH(2); break;
-        default: EHSS_LAUNCH(3); break;
+        case 3: EHSS_LAUNCH(3); break;
+        default: EHSS_LAUNCH(4); break;
     }
 #undef EHSS_LAUNCH
+    if (variant == 4) {
+        const size_t lds = drain_lds_bytes(sc->view);
+        if (lds <= kDrainLdsLimit) {
+            static bool attr = false;
+            if (!attr) {
+                EHSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_drain<1>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDrainLdsLimit));
+                attr = true;
+            }
+            hipLaunchKernelGGL(k_drain<1>, dim3((unsigned)drain_blocks(n)), dim3(256), lds, s, sc->view, A, qbuf,
+                               qcount, qcount + 1);
+        } else {
+            hipLaunchKernelGGL(k_drain<0>, dim3((unsigned)drain_blocks(n)), dim3(256), 0, s, sc->view, A, qbuf,
+                               qcount, qcount + 1);
+        }
+    }
     EHSS_HIP(hipGetLastError());
     EHSS_HIP(hipEventRecord(sc->ev1, s));
     hipLaunchKernelGGL(k_reduce, dim3((unsigned)nblocks), dim3(256), 0, s, n, cap, A.w, A.info, A.contrib,
@@ -189,6 +374,13 @@ int solve_impl(ehss_scene* sc, const ehss_sobol* sob, int power, int cap, int pr
 }  // namespace
 
 extern "C" {
+
+int ehss_ray_map(const ehss_sobol* sobol, int power, uint32_t* cols) {
+    EHSS_CHECK(sobol && cols, "null sobol/cols");
+    EHSS_CHECK(power >= 0 && power <= 30, "power %d out of range 0..30", power);
+    make_ray_map(*sobol, power, cols);
+    return EHSS_OK;
+}
 
 int ehss_solve(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, int precision, unsigned flags,
                ehss_totals* totals, int64_t* histogram, double* ledger) {

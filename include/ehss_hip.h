@@ -157,6 +157,12 @@ typedef struct {
     double kernel_ms;      /* device time of the trace kernel(s) (HIP events) */
 } ehss_totals;
 
+/* Host-only: the wave-coherent ray map used by ehss_solve for 2^power rays
+ * (cols[32]; no reference counterpart).  Thread t = 64 w + l traces the ray with
+ * Gray-code index g = XOR of cols[k] over set bits k of (w | l << (power - 6)),
+ * so the 64 rays of a wave share one elementary cell of the scrambled net. */
+int ehss_ray_map(const ehss_sobol* sobol, int power, uint32_t* cols);
+
 /* Synchronous solve of 2^power rays.  histogram (cap+1 int64) and ledger
  * (cap+1 double) may be NULL. */
 int ehss_solve(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, int precision, unsigned flags,

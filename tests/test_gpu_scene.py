@@ -118,7 +118,8 @@ def test_plain_and_fp32_fixup_kernels_are_bit_identical(oracle, name):
     c, r = scenes.globule(650, 1) if name == "globule650" else scenes.SCENES[name]()
     with GPUScene(c, r) as sc:
         runs = {}
-        for prec, variant in (("fp64", 0), ("fp32-fixup", 1), ("fp32-fixup", 2), ("fp32-fixup", 3)):
+        for prec, variant in (("fp64", 0), ("fp32-fixup", 1), ("fp32-fixup", 2), ("fp32-fixup", 3),
+                              ("fp32-fixup", 4)):
             say("kernels", name, prec, variant)
             res = sc.solve(scramble(7), 16, 64, precision=prec, pa=True, keep_rays=True, histogram=True,
                            variant=variant)
