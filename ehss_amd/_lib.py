@@ -25,7 +25,11 @@ PRECISIONS = {"fp64": PRECISION_FP64, "fp32-fixup": PRECISION_FP32_FIXUP}
 
 FLAG_PA = 1
 FLAG_KEEP_RAYS = 2
+FLAG_GRAPH = 4
+FLAG_TIMESTAMPS = 8
 VARIANT_SHIFT = 8
+COOP_SHIFT = 12
+GROUP_SHIFT = 19
 
 RAY_ESCAPED = 0x1
 RAY_UNRESOLVED = 0x2
@@ -92,6 +96,7 @@ _SIGS = {
     "ehss_scene_pa_query": (_INT, [_P, _I64, _INT] + [_P] * 5),
     "ehss_solve": (_INT, [_P, ct.POINTER(Sobol), _INT, _INT, _INT, ct.c_uint, ct.POINTER(Totals), _P, _P]),
     "ehss_scene_last_rays": (_INT, [_P, _I64, _P, _P, _P]),
+    "ehss_scene_last_ray_times": (_INT, [_P, _I64, _P, _P]),
     "ehss_ray_map": (_INT, [ct.POINTER(Sobol), _INT, _P]),
     "ehss_source_rays": (_INT, [_P, ct.POINTER(Sobol), _INT, _P, _P, _P]),
 }

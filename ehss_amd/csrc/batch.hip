@@ -90,48 +90,23 @@ __global__ void __launch_bounds__(256) k_batch(const JobDev* jobs, int n_jobs, i
 }
 
 // Drains the batch queue; each queued ray carries its job, whose scene and
-// arguments are read from the job table.  Counters go to the ray's job.
+// arguments are read from the job table.  Work counters go to the ray's job.
 __global__ void __launch_bounds__(256) k_drain_batch(const JobDev* jobs, const RayState* queue,
-                                                     const uint32_t* queue_count, uint32_t* queue_head) {
+                                                     const uint32_t* queue_count, uint32_t* queue_head, int32_t coop,
+                                                     int32_t group) {
     const uint32_t len = __builtin_amdgcn_readfirstlane(*queue_count);
-    RayCtx R;
-    R.done = true;
-    int32_t job = 0;
-    bool have = false, exhausted = false;
-    const int lane = threadIdx.x & 63;
-    uint64_t boxes = 0, spheres = 0, retests = 0;  // of the current ray, flushed when it finishes
-    for (;;) {
-        const uint64_t idle = __ballot(!have);
-        if (idle && !exhausted) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(queue_head, (uint32_t)__popcll(idle));
-            base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
-            if (base + (uint32_t)__popcll(idle) >= len) exhausted = true;
-            if (!have) {
-                const uint32_t slot = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-                if (slot < len) {
-                    job = ray_job(queue + slot);
-                    ray_load(queue + slot, jobs[job].view, R);
-                    have = true;
-                }
-            }
-        }
-        if (__ballot(have) == 0) break;
-        if (have) {
-            const JobDev& J = jobs[job];
-            Query q;
-            nearest_fast(J.view, R.o, R.d, R.last, R.tol, q, boxes, spheres, retests);
-            ray_process(J.view, J.args.cap, R, q);
-            if (R.done) {
-                ray_record(J.args, R);
-                atomicAdd(&J.args.counters[0], (unsigned long long)boxes);
-                atomicAdd(&J.args.counters[1], (unsigned long long)spheres);
-                if (retests) atomicAdd(&J.args.counters[2], (unsigned long long)retests);
-                boxes = spheres = retests = 0;
-                have = false;
-            }
-        }
-    }
+    auto scene_of = [&](int32_t j) -> const SceneView& { return jobs[j].view; };
+    auto args_of = [&](int32_t j) -> const SolveArgs& { return jobs[j].args; };
+    auto account = [&](int32_t j, uint64_t b, uint64_t sp, uint64_t r) {
+        unsigned long long* c = jobs[j].args.counters;
+        if (b) atomicAdd(&c[0], (unsigned long long)b);
+        if (sp) atomicAdd(&c[1], (unsigned long long)sp);
+        if (r) atomicAdd(&c[2], (unsigned long long)r);
+    };
+    if (group > 1)
+        drain_groups(queue, len, queue_head, group, scene_of, args_of, account);
+    else
+        drain_queue(queue, len, queue_head, coop, scene_of, args_of, account);
 }
 
 __global__ void __launch_bounds__(256) k_reduce_batch(const JobDev* jobs, int n_jobs) {
@@ -160,7 +135,7 @@ constexpr int kWords = 16;
 using namespace ehss;
 
 struct ehss_batch {
-    int n_jobs = 0, precision = 0, variant = 0, blocks = 0;
+    int n_jobs = 0, precision = 0, variant = 0, blocks = 0, coop = 0, group = 1;
     bool pa = false, use_graph = false, captured = false, dirty = true, keep = false;
     std::vector<JobDev> jobs;
     std::vector<ehss_scene*> scenes;
@@ -205,7 +180,7 @@ int enqueue_batch(ehss_batch* b) {
 #undef EHSS_BATCH
     if (b->variant == 4)
         hipLaunchKernelGGL(k_drain_batch, dim3((unsigned)drain_blocks(b->rays)), dim3(256), 0, s, jd, qbuf, b->qwords,
-                           b->qwords + 1);
+                           b->qwords + 1, b->coop, b->group);
     hipLaunchKernelGGL(k_reduce_batch, dim3((unsigned)b->total_red), dim3(256), 0, s, jd, b->n_jobs);
     hipLaunchKernelGGL(k_finalize_batch, dim3((unsigned)b->n_jobs), dim3(256), 0, s, jd);
     EHSS_HIP(hipGetLastError());
@@ -232,7 +207,7 @@ int create_batch(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols
     EHSS_CHECK(precision == EHSS_PRECISION_FP64 || precision == EHSS_PRECISION_FP32_FIXUP, "unknown precision");
     int variant = (int)((flags >> EHSS_VARIANT_SHIFT) & 0xFu);
     if (precision == EHSS_PRECISION_FP64) variant = 0;
-    else if (variant == 0) variant = 2;
+    else if (variant == 0) variant = 4;
     EHSS_CHECK(variant >= 0 && variant <= 4, "unknown kernel variant %d", variant);
     for (int j = 0; j < n_jobs; ++j) {
         EHSS_CHECK(scenes[j], "null scene for job %d", j);
@@ -249,6 +224,11 @@ int create_batch(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols
     b->variant = variant;
     b->pa = (flags & EHSS_FLAG_PA) != 0;
     b->use_graph = (flags & EHSS_FLAG_GRAPH) != 0;
+    b->coop = coop_from_flags(flags);
+    int64_t all_rays = 0;
+    for (int j = 0; j < n_jobs; ++j) all_rays += 1LL << powers[j];
+    b->group = group_from_flags(flags, all_rays);
+    EHSS_CHECK(b->coop <= kWave, "tail threshold %d out of range 0..64", b->coop);
     b->scenes.assign(scenes, scenes + n_jobs);
     int64_t rays = 0, bins = 0;
     std::vector<int64_t> ray_off(n_jobs), bin_off(n_jobs), red_off(n_jobs), pkt_off(n_jobs);
@@ -298,6 +278,8 @@ int create_batch(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols
         J.args.info = b->info.as<uint32_t>() + ray_off[j];
         J.args.contrib = b->contrib.as<double>() + ray_off[j];
         J.args.counters = W + (int64_t)kWords * j;
+        J.args.coop = b->coop;
+        J.args.group = b->group;
         J.sob = sobols[j];
         make_ray_map(J.sob, powers[j], J.map.col);
         J.pkt_begin = pkt_off[j];

@@ -72,7 +72,7 @@ namespace {
 
 // Host-side description of a caller tree (ehss_trace arrays) on the device.
 struct CompatScene {
-    DevBuf nodes, nodes32, offsets, ids, sph, sph32;
+    DevBuf nodes, nodes32, offsets, ids, sph, sph32, sphs;
     SceneView view;
 };
 
@@ -156,9 +156,12 @@ int upload_compat(CompatScene& cs, int64_t nn, const double* lo, const double* h
     if (fast) {
         EHSS_TRY(cs.nodes32.alloc(sizeof(Node32) * nn));
         EHSS_TRY(cs.sph32.alloc(sizeof(float4) * std::max<int64_t>(slots, 1)));
-        EHSS_TRY(launch_make_filter(v, cs.nodes32.as<Node32>(), cs.sph32.as<float4>(), s));
+        EHSS_TRY(cs.sphs.alloc(sizeof(double4) * std::max<int64_t>(slots, 1)));
+        EHSS_TRY(launch_make_filter(v, cs.nodes32.as<Node32>(), cs.sph32.as<float4>(), cs.sphs.as<double4>(),
+                                    nullptr, s));
         v.nodes32 = cs.nodes32.as<Node32>();
         v.sph32 = cs.sph32.as<float4>();
+        v.sphs = cs.sphs.as<double4>();
     }
     // Hold the staging vectors until the copies are done.
     EHSS_HIP(hipStreamSynchronize(s));

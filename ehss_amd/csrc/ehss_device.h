@@ -15,6 +15,8 @@ namespace ehss {
 constexpr double kEps64 = 2.220446049250313e-16;  // DBL_EPSILON
 constexpr float kU32 = 5.9604645e-08f;            // 2^-24, FP32 unit roundoff
 constexpr int kInfo = 0;
+constexpr int kWave = 64;                         // wavefront width on gfx950
+constexpr int kFrontier = 8 * kWave;              // cooperative-query subtree roots, at most
 
 // std::max / std::min semantics: the first argument wins unless strictly beaten.
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }

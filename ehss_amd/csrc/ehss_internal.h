@@ -67,8 +67,12 @@ struct SceneView {
     const int32_t* ids = nullptr;     // slot -> atom
     const double4* sph = nullptr;     // atom -> {x, y, z, r}, FP64
     const float4* sph32 = nullptr;    // slot -> {x - Cx, y - Cy, z - Cz, r}, FP32
+    const double4* sphs = nullptr;    // slot -> sph[ids[slot]] (FP64, slot order)
+    const Node32* leaves32 = nullptr; // group -> FP32 filter box of its leaf; end = first slot, leaf = slots
     const NodePA* pa = nullptr;       // PA tree
     const int32_t* pa_ids = nullptr;  // PA id slots
+    const int32_t* frontier = nullptr; // roots of <= kFrontier disjoint subtrees covering every leaf
+    int32_t n_frontier = 0;
     double cx = 0, cy = 0, cz = 0;    // filter origin (scene centre)
     double scale = 1;                 // bound on |coordinates - centre| and radii
     float m_box = 0, m_p = 0, m_disc = 0;  // FP32 filter margins
@@ -79,7 +83,10 @@ struct SceneView {
 void set_filter_margins(SceneView& v);
 
 // Builds the FP32 filter arrays (nodes32, sph32) from the FP64 ones, on the device.
-int launch_make_filter(const SceneView& v, Node32* nodes32, float4* sph32, hipStream_t s);
+// Also writes the slot-ordered FP64 spheres and, when leaves32 is not null, the
+// per-group leaf boxes.
+int launch_make_filter(const SceneView& v, Node32* nodes32, float4* sph32, double4* sphs, Node32* leaves32,
+                       hipStream_t s);
 
 // Per-ray trace of caller rays (AoS p, d as in ehss_trace).
 struct TraceIO {
@@ -145,13 +152,14 @@ struct ehss_scene {
     int32_t n = 0, ng = 0, nn = 0, pa_nn = 0, leaf_size = 4, max_depth = 0;
     double center[3] = {0, 0, 0};
     double radius = 0, scale = 1;
-    ehss::DevBuf centers, radii, sph, groups, offsets, eids, nodes, nodes32, sph32, pa, pa_ids;
+    ehss::DevBuf centers, radii, sph, groups, offsets, eids, nodes, nodes32, sph32, sphs, leaves32, pa, pa_ids,
+        frontier;
     ehss::SceneView view;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // solve workspace (grow-only)
-    ehss::DevBuf ray_info, ray_contrib, partials, hist, ledger, counters, queue;
-    int64_t last_rays = 0;
+    ehss::DevBuf ray_info, ray_contrib, partials, hist, ledger, counters, queue, ray_times;
+    int64_t last_rays = 0, last_times = 0;
     double last_weight = 0;
     int64_t device_bytes() const;
 };

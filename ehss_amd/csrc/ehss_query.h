@@ -147,9 +147,8 @@ struct PAState {
 };
 
 // Exact FP64 entry of one sphere (and its any-hit test when PA is pending).
-__device__ __forceinline__ void eval64(const SceneView& S, const V3& o, const V3& d, int32_t wanted, double tol,
+__device__ __forceinline__ void eval64(const V3& o, const V3& d, const double4& c, int32_t wanted, double tol,
                                        Query& q, PAState* pa) {
-    const double4 c = S.sph[wanted];
     const double t = entry64(o, d, c.x, c.y, c.z, c.w, tol, &q.graze);
     consider(q, t, wanted);
     if (pa && !pa->decided) {
@@ -207,7 +206,7 @@ __device__ __forceinline__ void nearest_fast(const SceneView& S, const V3& o, co
                 if (!needs_exact(S, r, S.sph32[k], q.best, tol, pa)) continue;
                 ++retests;
                 const int32_t before = q.atom;
-                eval64(S, o, d, wanted, tol, q, pa);
+                eval64(o, d, S.sphs[k], wanted, tol, q, pa);
                 if (q.atom != before) bestf = box_bound(S, q.best, tol);
             }
         }
@@ -243,6 +242,13 @@ __device__ __forceinline__ float4 load_sphere32(const float4* base, int32_t k) {
     return make_float4(f[0], f[1], f[2], f[3]);
 }
 
+typedef const __attribute__((address_space(4))) double* CDouble;
+
+__device__ __forceinline__ double4 load_sphere64(const double4* base, int32_t k) {
+    const CDouble f = (CDouble)(base + k);
+    return make_double4(f[0], f[1], f[2], f[3]);
+}
+
 __device__ __forceinline__ void nearest_packet(const SceneView& S, const V3& o, const V3& d, int32_t last,
                                                double tol, bool live, Query& q, uint64_t& boxes,
                                                uint64_t& spheres, uint64_t& retests, PAState* pa = nullptr) {
@@ -272,12 +278,102 @@ __device__ __forceinline__ void nearest_packet(const SceneView& S, const V3& o, 
                 if (!needs_exact(S, r, s, q.best, tol, pa)) continue;
                 ++retests;
                 const int32_t before = q.atom;
-                eval64(S, o, d, wanted, tol, q, pa);
+                eval64(o, d, load_sphere64(S.sphs, k), wanted, tol, q, pa);
                 if (q.atom != before) bestf = box_bound(S, q.best, tol);
             }
         }
         ++node;
     }
+}
+
+// Cooperative nearest query: one ray, a group of G lanes (G a power of two <= 64,
+// groups aligned within the wave).  The ray's state must be the same on every
+// lane of the group.  Lane `sub` of the group walks the subtrees rooted at
+// frontier entries sub, sub + G, ... with the nearest_fast rules and its own
+// running best (small scenes have every leaf in the frontier, so each lane scans
+// leaf boxes with four independent loads in flight); the group then merges its
+// per-lane queries.  Every sphere that can win (exact entry within best + tol) is
+// evaluated in FP64 on some lane, and the winner is the lexicographic minimum of
+// (entry, atom id), so the answer equals nearest_fast's; `second` and `graze`
+// stay conservative.
+__device__ __forceinline__ void group_merge_query(Query& q, int G) {
+    for (int off = G >> 1; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(q.best, off, kWave);
+        const int32_t oa = __shfl_xor(q.atom, off, kWave);
+        const double os = __shfl_xor(q.second, off, kWave);
+        const double og = __shfl_xor(q.graze, off, kWave);
+        q.second = fmin(fmin(q.second, os), fmax(q.best, ob));
+        if (better(ob, oa, q.best, q.atom)) {
+            q.best = ob;
+            q.atom = oa;
+        }
+        q.graze = fmin(q.graze, og);
+    }
+}
+
+// Spheres of one leaf (slots first .. first + count - 1) against the lane's query.
+__device__ __forceinline__ void scan_leaf(const SceneView& S, const Ray32& r, const V3& o, const V3& d,
+                                          int32_t first, int32_t count, int32_t last, double tol, Query& q,
+                                          float& bestf, uint64_t& spheres, uint64_t& retests) {
+    for (int32_t k = first; k < first + count; ++k) {
+        const int32_t wanted = S.ids[k];
+        if (wanted == last) continue;
+        ++spheres;
+        if (!needs_exact(S, r, S.sph32[k], q.best, tol, nullptr)) continue;
+        ++retests;
+        const int32_t before = q.atom;
+        eval64(o, d, S.sphs[k], wanted, tol, q, nullptr);
+        if (q.atom != before) bestf = box_bound(S, q.best, tol);
+    }
+}
+
+__device__ __forceinline__ void nearest_group(const SceneView& S, const V3& o, const V3& d, int32_t last, double tol,
+                                              int G, int sub, Query& q, uint64_t& boxes, uint64_t& spheres,
+                                              uint64_t& retests) {
+    query_init(q);
+    const Ray32 r = ray32(S, o, d);
+    float bestf = INFINITY;
+    const int32_t ng = S.n_groups;
+    if (ng <= kFrontier) {
+        // Every leaf box, four independent loads in flight per lane.
+        for (int32_t g0 = sub; g0 < ng; g0 += 4 * G) {
+            Node32 leaf[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (g0 + u * G < ng) leaf[u] = S.leaves32[g0 + u * G];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (g0 + u * G >= ng) break;
+                ++boxes;
+                if (box32(r, leaf[u], bestf))
+                    scan_leaf(S, r, o, d, leaf[u].end, leaf[u].leaf, last, tol, q, bestf, spheres, retests);
+            }
+        }
+    } else {
+        for (int32_t e = sub; e < S.n_frontier; e += G) {
+            int32_t node = S.frontier[e];
+            const int32_t stop = S.nodes32[node].end;
+            while (node < stop) {
+                const Node32 n = S.nodes32[node];
+                ++boxes;
+                if (!box32(r, n, bestf)) {
+                    node = n.end;
+                    continue;
+                }
+                const int32_t g = n.leaf;
+                if (g >= 0) scan_leaf(S, r, o, d, S.offsets[g], S.offsets[g + 1] - S.offsets[g], last, tol, q, bestf,
+                                      spheres, retests);
+                ++node;
+            }
+        }
+    }
+    group_merge_query(q, G);
+}
+
+// The whole wave on one ray.
+__device__ __forceinline__ void nearest_coop(const SceneView& S, const V3& o, const V3& d, int32_t last, double tol,
+                                             Query& q, uint64_t& boxes, uint64_t& spheres, uint64_t& retests) {
+    nearest_group(S, o, d, last, tol, kWave, threadIdx.x & (kWave - 1), q, boxes, spheres, retests);
 }
 
 // pa_spatial.py:114-130.  Counters follow the reference (stop at the first hit);
@@ -358,7 +454,7 @@ __device__ __forceinline__ bool pa_fast(const SceneView& S, const V3& o, const V
                 }
                 if (b >= m_p && cc > m + 4.0f * (float)S.scale * ftol) continue;
                 ++retests;
-                const double4 c = S.sph[j];
+                const double4 c = S.sphs[k];
                 bool critical;
                 const bool h = sphere_hit64_graze(o, d, c.x, c.y, c.z, c.w, tol, &critical);
                 critical_seen |= critical;

@@ -30,6 +30,9 @@ struct SolveArgs {
     uint32_t* info;
     double* contrib;
     unsigned long long* counters;  // boxes, spheres, retests, pa boxes, pa tests, pa retests
+    int32_t coop;                  // a wave with at most this many live rays finishes them cooperatively
+    int32_t group;                 // lanes per ray in the queue drain (power of two 1..64)
+    long long* times;              // optional per-ray completion time stamps (device wall clock)
 };
 
 struct Partial {
@@ -163,6 +166,13 @@ struct alignas(8) RayState {
     int32_t job, pad;
 };
 
+// Grazing tolerance of the ray's next query.  The flag is sticky and no
+// physical output depends on tol, so a ray already flagged queries with tol 0:
+// its filter then prunes as tightly as on a short path.
+__device__ __forceinline__ double ray_tol(const RayCtx& R, double scale) {
+    return (R.flags & EHSS_RAY_GRAZING) ? 0. : path_tol(R.pe, scale);
+}
+
 // One query result applied to the ray: escape, cap (unresolved) or reflection
 // (ehss_cpu.cpp:73-77), plus the grazing flag and the path error.
 __device__ __forceinline__ void ray_process(const SceneView& S, int32_t cap, RayCtx& R, const Query& q) {
@@ -177,7 +187,7 @@ __device__ __forceinline__ void ray_process(const SceneView& S, int32_t cap, Ray
         const double4 c = S.sph[q.atom];
         const double cos_inc = reflect64(R.o, R.d, c.x, c.y, c.z, c.w, q.best);
         path_error_reflect(R.pe, q.best, cos_inc, c.w, S.scale);
-        R.tol = path_tol(R.pe, S.scale);
+        R.tol = ray_tol(R, S.scale);
         ++R.count;
         R.last = q.atom;
     }
@@ -196,6 +206,7 @@ __device__ __forceinline__ void ray_record(const SolveArgs& A, RayCtx& R) {
     if (R.count > 0 || (R.flags & EHSS_RAY_UNRESOLVED)) R.flags |= EHSS_RAY_EHSS_HIT;
     A.info[R.i] = (uint32_t)R.count | (R.flags << kFlagShift);
     A.contrib[R.i] = contrib;
+    if (A.times) A.times[R.i] = (long long)wall_clock64();
 }
 
 __device__ __forceinline__ void ray_save(const RayCtx& R, int32_t job, RayState* q) {
@@ -228,7 +239,7 @@ __device__ __forceinline__ int32_t ray_load(const RayState* q, const SceneView& 
     R.d = V3{st.d[0], st.d[1], st.d[2]};
     R.inc = V3{st.inc[0], st.inc[1], st.inc[2]};
     R.pe = PathError{st.ep, st.ed};
-    R.tol = path_tol(R.pe, S.scale);
+    R.tol = ray_tol(R, S.scale);
     R.i = st.i;
     R.flags = st.flags;
     R.last = st.last;
@@ -236,6 +247,57 @@ __device__ __forceinline__ int32_t ray_load(const RayState* q, const SceneView& 
     R.done = false;
     R.escaped = false;
     return st.job;
+}
+
+// Tail threshold from ehss_solve / ehss_batch_create flags (bits 12..18).
+constexpr int kDefaultCoop = 8;
+inline int coop_from_flags(unsigned flags) {
+    const int f = (int)((flags >> EHSS_COOP_SHIFT) & 0x7Fu);
+    return f == 0 ? kDefaultCoop : f - 1;
+}
+
+// Lanes per ray of the queue drain from flags (bits 19..21: log2 + 1).  The
+// default gives every queued ray of a small solve the whole wave, and two rays
+// per wave from 2^18 rays up, where the queue is long enough to fill the chip.
+inline int group_from_flags(unsigned flags, int64_t rays) {
+    const int f = (int)((flags >> EHSS_GROUP_SHIFT) & 0x7u);
+    if (f == 0) return rays >= (1 << 18) ? 32 : 64;
+    return 1 << (f - 1);
+}
+
+// Value of lane `src` (wave-uniform) on every lane.
+__device__ __forceinline__ int32_t lane_value(int32_t x, int src) { return __builtin_amdgcn_readlane(x, src); }
+__device__ __forceinline__ double lane_value(double x, int src) {
+    const long long b = __double_as_longlong(x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), src);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Tail mode: the wave's live rays (lanes with !R.done) are finished one at a time,
+// every query spread over the whole wave (nearest_coop).  A long path then costs
+// one short cooperative query per bounce instead of one full per-lane traversal
+// while 63 lanes idle.  All lanes of the wave call this; `job` is the lane's job,
+// `account(job, boxes, spheres, retests)` receives each lane's share of a query's
+// work.  Results are those of the per-lane queries.
+template <class SceneOf, class ArgsOf, class Account>
+__device__ __forceinline__ void finish_coop(RayCtx& R, int32_t job, SceneOf scene_of, ArgsOf args_of,
+                                            Account account) {
+    const int lane = threadIdx.x & (kWave - 1);
+    for (;;) {
+        const uint64_t m = __ballot(!R.done);
+        if (m == 0) break;
+        const int src = __builtin_amdgcn_readfirstlane(__builtin_ctzll(m));
+        const int32_t j = lane_value(job, src);
+        const SceneView& S = scene_of(j);
+        const V3 o{lane_value(R.o.x, src), lane_value(R.o.y, src), lane_value(R.o.z, src)};
+        const V3 d{lane_value(R.d.x, src), lane_value(R.d.y, src), lane_value(R.d.z, src)};
+        uint64_t boxes = 0, spheres = 0, retests = 0;
+        Query q;
+        nearest_coop(S, o, d, lane_value(R.last, src), lane_value(R.tol, src), q, boxes, spheres, retests);
+        account(j, boxes, spheres, retests);
+        if (lane == src) ray_process(S, args_of(j).cap, R, q);
+    }
 }
 
 // Number of wave-uniform queries a ray gets in the two-phase variant (4) before
@@ -275,7 +337,7 @@ __device__ __forceinline__ void solve_ray(const SceneView& S, const SolveArgs& A
     R.done = !valid;
     R.escaped = false;
     R.pe = path_error_start(S.scale);
-    R.tol = path_tol(R.pe, S.scale);
+    R.tol = ray_tol(R, S.scale);
     Query q;
     bool record = valid;
     if (kMode == 0) {
@@ -312,10 +374,22 @@ __device__ __forceinline__ void solve_ray(const SceneView& S, const SolveArgs& A
             }
             ray_process(S, A.cap, R, q);
         }
+        auto scene_of = [&](int32_t) -> const SceneView& { return S; };
+        auto args_of = [&](int32_t) -> const SolveArgs& { return A; };
+        auto account = [&](int32_t, uint64_t b, uint64_t s, uint64_t r) {
+            boxes += b;
+            spheres += s;
+            retests += r;
+        };
         if (kMode == 3 || kMode == 4) {
             for (int k = 1; kMode == 3 || k < kPacketQueries; ++k) {
                 const bool live = !R.done;
-                if (__ballot(live) == 0) break;
+                const uint64_t m = __ballot(live);
+                if (m == 0) break;
+                if (kMode == 3 && __popcll(m) <= (unsigned)A.coop) {
+                    finish_coop(R, 0, scene_of, args_of, account);
+                    break;
+                }
                 nearest_packet(S, R.o, R.d, R.last, R.tol, live, q, boxes, spheres, retests);
                 if (live) ray_process(S, A.cap, R, q);
             }
@@ -334,9 +408,17 @@ __device__ __forceinline__ void solve_ray(const SceneView& S, const SolveArgs& A
                 }
             }
         } else {
-            while (!R.done) {
-                nearest_fast(S, R.o, R.d, R.last, R.tol, q, boxes, spheres, retests);
-                ray_process(S, A.cap, R, q);
+            for (;;) {
+                const uint64_t m = __ballot(!R.done);
+                if (m == 0) break;
+                if (__popcll(m) <= (unsigned)A.coop) {
+                    finish_coop(R, 0, scene_of, args_of, account);
+                    break;
+                }
+                if (!R.done) {
+                    nearest_fast(S, R.o, R.d, R.last, R.tol, q, boxes, spheres, retests);
+                    ray_process(S, A.cap, R, q);
+                }
             }
         }
     }
@@ -351,25 +433,28 @@ __device__ __forceinline__ void solve_ray(const SceneView& S, const SolveArgs& A
 
 // Persistent per-lane drain of queued rays.  Idle lanes refill from the queue
 // (one atomic per wave); each live lane runs one query per iteration, so lanes
-// never wait for another lane's long path.  Every wave exits once the queue is
-// exhausted and its lanes are idle.  `scene_of(job)` / `args_of(job)` give the
-// scene and solve arguments of a queued ray's job.
-template <class SceneOf, class ArgsOf>
+// never wait for another lane's long path.  Once the queue is exhausted and at
+// most `coop` rays are left in the wave, they are finished cooperatively.  Every
+// wave exits once the queue is exhausted and its lanes are idle.
+// `scene_of(job)` / `args_of(job)` give the scene and solve arguments of a queued
+// ray's job; `account(job, boxes, spheres, retests)` receives the work of each
+// finished ray (per lane) and of each cooperative query (each lane's share).
+template <class SceneOf, class ArgsOf, class Account>
 __device__ __forceinline__ void drain_queue(const RayState* queue, uint32_t queue_len, uint32_t* queue_head,
-                                            SceneOf scene_of, ArgsOf args_of, uint64_t* ctr) {
+                                            int32_t coop, SceneOf scene_of, ArgsOf args_of, Account account) {
     RayCtx R;
     R.done = true;
     int32_t job = 0;
     bool have = false;
-    const int lane = threadIdx.x & 63;
-    uint64_t boxes = 0, spheres = 0, retests = 0;
+    const int lane = threadIdx.x & (kWave - 1);
+    uint64_t boxes = 0, spheres = 0, retests = 0;  // of the lane's current ray
     bool exhausted = false;  // wave-uniform: the queue head has passed the end
     for (;;) {
         const uint64_t idle = __ballot(!have);
         if (idle && !exhausted) {
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(queue_head, (uint32_t)__popcll(idle));
-            base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
+            base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, kWave));
             // One grab past the end at most per wave, so the head cannot wrap.
             if (base + (uint32_t)__popcll(idle) >= queue_len) exhausted = true;
             if (!have) {
@@ -380,7 +465,15 @@ __device__ __forceinline__ void drain_queue(const RayState* queue, uint32_t queu
                 }
             }
         }
-        if (__ballot(have) == 0) break;
+        const uint64_t live = __ballot(have);
+        if (live == 0) break;
+        if (exhausted && __popcll(live) <= (unsigned)coop) {
+            // Nothing left to refill from: finish the wave's rays cooperatively.
+            if (have) account(job, boxes, spheres, retests);
+            finish_coop(R, job, scene_of, args_of, account);
+            if (have) ray_record(args_of(job), R);
+            break;
+        }
         if (have) {
             const SceneView& S = scene_of(job);
             Query q;
@@ -389,13 +482,73 @@ __device__ __forceinline__ void drain_queue(const RayState* queue, uint32_t queu
             ray_process(S, A.cap, R, q);
             if (R.done) {
                 ray_record(A, R);
+                account(job, boxes, spheres, retests);
+                boxes = spheres = retests = 0;
                 have = false;
             }
         }
     }
-    ctr[0] += boxes;
-    ctr[1] += spheres;
-    ctr[2] += retests;
+}
+
+// Sum over the G lanes of an aligned group, on every lane of the group.
+__device__ __forceinline__ uint64_t group_sum(uint64_t v, int G) {
+    for (int off = G >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Queue drain with G lanes per ray (G >= 2).  Every lane of a group holds the
+// group's ray state and applies the same query result to it, so the copies stay
+// identical; each query is a nearest_group over the group's lanes.  An idle
+// group takes the next queued ray (one atomic per wave per refill round), so a
+// long path holds only its own group.  `account` receives each finished ray's
+// work (summed over the group, on the group's first lane).
+template <class SceneOf, class ArgsOf, class Account>
+__device__ __forceinline__ void drain_groups(const RayState* queue, uint32_t queue_len, uint32_t* queue_head,
+                                             int G, SceneOf scene_of, ArgsOf args_of, Account account) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int sub = lane & (G - 1);
+    const int leader = lane - sub;
+    RayCtx R;
+    R.done = true;
+    int32_t job = 0;
+    bool have = false;       // the same on every lane of a group
+    bool exhausted = false;  // wave-uniform: the queue head has passed the end
+    uint64_t boxes = 0, spheres = 0, retests = 0;  // this lane's share of the current ray
+    for (;;) {
+        const uint64_t idle = __ballot(!have && sub == 0);  // one bit per idle group, at its first lane
+        if (idle && !exhausted) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(queue_head, (uint32_t)__popcll(idle));
+            base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, kWave));
+            // One grab past the end at most per wave, so the head cannot wrap.
+            if (base + (uint32_t)__popcll(idle) >= queue_len) exhausted = true;
+            const uint32_t mine = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+            const uint32_t slot = (uint32_t)__shfl((int)mine, leader, kWave);
+            if (!have && slot < queue_len) {
+                job = ray_load(queue + slot, scene_of(ray_job(queue + slot)), R);
+                have = true;
+            }
+        }
+        if (__ballot(have) == 0) break;
+        if (have) {
+            const SceneView& S = scene_of(job);
+            Query q;
+            nearest_group(S, R.o, R.d, R.last, R.tol, G, sub, q, boxes, spheres, retests);
+            const SolveArgs& A = args_of(job);
+            ray_process(S, A.cap, R, q);
+            if (R.done) {
+                boxes = group_sum(boxes, G);
+                spheres = group_sum(spheres, G);
+                retests = group_sum(retests, G);
+                if (sub == 0) {
+                    ray_record(A, R);
+                    account(job, boxes, spheres, retests);
+                }
+                boxes = spheres = retests = 0;
+                have = false;
+            }
+        }
+    }
 }
 
 // Adds per-lane counters into counters[0..5] with one atomic per wave (order-free).

@@ -30,7 +30,7 @@ void enclosing_disk(int64_t n, const double* centers, const double* radii, doubl
 int64_t ehss_scene::device_bytes() const {
     const ehss::DevBuf* bufs[] = {&centers, &radii, &sph,   &groups, &offsets, &eids,    &nodes,   &nodes32,
                                   &sph32,   &pa,    &pa_ids, &ray_info, &ray_contrib, &partials, &hist,  &ledger,
-                                  &counters, &queue};
+                                  &counters, &queue, &frontier, &sphs, &leaves32, &ray_times};
     int64_t total = 0;
     for (const ehss::DevBuf* b : bufs) total += (int64_t)b->bytes;
     return total;
@@ -39,6 +39,38 @@ int64_t ehss_scene::device_bytes() const {
 using namespace ehss;
 
 namespace {
+
+// Roots of the subtrees one wave splits a cooperative query over: starting from
+// the root, every internal node is replaced by its two children (preorder: the
+// left child is node + 1, the right child starts where the left subtree ends)
+// while the list stays within kFrontier entries (eight per lane).
+int build_frontier(ehss_scene* sc, hipStream_t s) {
+    std::vector<Node64> nodes((size_t)sc->nn);
+    EHSS_HIP(hipMemcpyAsync(nodes.data(), sc->nodes.p, sizeof(Node64) * sc->nn, hipMemcpyDeviceToHost, s));
+    EHSS_HIP(hipStreamSynchronize(s));
+    std::vector<int32_t> f{0}, next;
+    for (;;) {
+        next.clear();
+        bool split = false;
+        for (const int32_t v : f) {
+            if (nodes[v].leaf >= 0) {
+                next.push_back(v);
+            } else {
+                next.push_back(v + 1);
+                next.push_back(nodes[v + 1].end);
+                split = true;
+            }
+        }
+        if (!split || next.size() > (size_t)kFrontier) break;
+        f.swap(next);
+    }
+    EHSS_TRY(sc->frontier.alloc(sizeof(int32_t) * f.size()));
+    EHSS_HIP(hipMemcpyAsync(sc->frontier.p, f.data(), sizeof(int32_t) * f.size(), hipMemcpyHostToDevice, s));
+    EHSS_HIP(hipStreamSynchronize(s));  // f is a local
+    sc->view.frontier = sc->frontier.as<int32_t>();
+    sc->view.n_frontier = (int32_t)f.size();
+    return EHSS_OK;
+}
 
 int create_scene(int64_t n, const double* centers, const double* radii, const int64_t* groups, int leaf_size,
                  ehss_scene** out) {
@@ -139,9 +171,15 @@ int create_scene(int64_t n, const double* centers, const double* radii, const in
     set_filter_margins(v);
     EHSS_TRY(sc->nodes32.alloc(sizeof(Node32) * sc->nn));
     EHSS_TRY(sc->sph32.alloc(sizeof(float4) * n));
-    EHSS_TRY(launch_make_filter(v, sc->nodes32.as<Node32>(), sc->sph32.as<float4>(), s));
+    EHSS_TRY(sc->sphs.alloc(sizeof(double4) * n));
+    EHSS_TRY(sc->leaves32.alloc(sizeof(Node32) * sc->ng));
+    EHSS_TRY(launch_make_filter(v, sc->nodes32.as<Node32>(), sc->sph32.as<float4>(), sc->sphs.as<double4>(),
+                                sc->leaves32.as<Node32>(), s));
     v.nodes32 = sc->nodes32.as<Node32>();
     v.sph32 = sc->sph32.as<float4>();
+    v.sphs = sc->sphs.as<double4>();
+    v.leaves32 = sc->leaves32.as<Node32>();
+    EHSS_TRY(build_frontier(sc, s));
     EHSS_HIP(hipStreamSynchronize(s));
     *out = sc;
     sc = nullptr;  // release the guard

@@ -23,10 +23,11 @@ __global__ void __launch_bounds__(256) k_solve(SceneView S, SolveArgs A, ehss_so
     flush_counters(ctr, A.counters);
 }
 
-// Persistent per-lane drain of the rays the two-phase variant queued.  With
-// kLds the FP32 filter tree, leaf offsets, slot ids and FP32 spheres are first
-// copied into LDS, so every node and sphere fetch of the per-lane traversal is an
-// LDS access instead of a dependent L2 round trip.
+// Persistent drain of the rays the two-phase variant queued: one lane per ray
+// (per-lane tree traversal, cooperative tail) or A.group lanes per ray.  With
+// kLds the FP32 filter tree, the leaf boxes, leaf offsets, slot ids and FP32
+// spheres are first copied into LDS, so every node, leaf and sphere fetch of the
+// drain is an LDS access instead of a dependent L2 round trip.
 template <int kLds>
 __global__ void __launch_bounds__(256) k_drain(SceneView S, SolveArgs A, const RayState* queue,
                                                const uint32_t* queue_count, uint32_t* queue_head) {
@@ -37,10 +38,12 @@ __global__ void __launch_bounds__(256) k_drain(SceneView S, SolveArgs A, const R
     SceneView L = S;
     if (kLds) {
         Node32* nodes = reinterpret_cast<Node32*>(lds);
-        float4* sph = reinterpret_cast<float4*>(nodes + S.n_nodes);
+        Node32* leaves = nodes + S.n_nodes;
+        float4* sph = reinterpret_cast<float4*>(leaves + S.n_groups);
         int32_t* offsets = reinterpret_cast<int32_t*>(sph + S.n_atoms);
         int32_t* ids = offsets + (S.n_groups + 1);
         for (int32_t k = threadIdx.x; k < S.n_nodes; k += blockDim.x) nodes[k] = S.nodes32[k];
+        for (int32_t k = threadIdx.x; k < S.n_groups; k += blockDim.x) leaves[k] = S.leaves32[k];
         for (int32_t k = threadIdx.x; k < S.n_atoms; k += blockDim.x) {
             sph[k] = S.sph32[k];
             ids[k] = S.ids[k];
@@ -48,20 +51,29 @@ __global__ void __launch_bounds__(256) k_drain(SceneView S, SolveArgs A, const R
         for (int32_t k = threadIdx.x; k <= S.n_groups; k += blockDim.x) offsets[k] = S.offsets[k];
         __syncthreads();
         L.nodes32 = nodes;
+        L.leaves32 = leaves;
         L.sph32 = sph;
         L.offsets = offsets;
         L.ids = ids;
     }
-    drain_queue(
-        queue, len, queue_head, [&](int32_t) -> const SceneView& { return L; },
-        [&](int32_t) -> const SolveArgs& { return A; }, ctr);
+    auto scene_of = [&](int32_t) -> const SceneView& { return L; };
+    auto args_of = [&](int32_t) -> const SolveArgs& { return A; };
+    auto account = [&](int32_t, uint64_t b, uint64_t sp, uint64_t r) {
+        ctr[0] += b;
+        ctr[1] += sp;
+        ctr[2] += r;
+    };
+    if (A.group > 1)
+        drain_groups(queue, len, queue_head, A.group, scene_of, args_of, account);
+    else
+        drain_queue(queue, len, queue_head, A.coop, scene_of, args_of, account);
     flush_counters(ctr, A.counters);
 }
 
 // LDS bytes of a scene staged by k_drain<1>.
 size_t drain_lds_bytes(const SceneView& v) {
-    return sizeof(Node32) * v.n_nodes + sizeof(float4) * v.n_atoms + sizeof(int32_t) * (v.n_groups + 1) +
-           sizeof(int32_t) * v.n_atoms;
+    return sizeof(Node32) * (v.n_nodes + v.n_groups) + sizeof(float4) * v.n_atoms +
+           sizeof(int32_t) * (v.n_groups + 1) + sizeof(int32_t) * v.n_atoms;
 }
 
 __global__ void k_zero(unsigned long long* p, int64_t n) {
@@ -237,7 +249,7 @@ namespace {
 // [8..] final Partial.
 constexpr int kQueueWord = 8 + (int)(sizeof(Partial) / 8);  // queue length, queue head (two u32)
 constexpr int kCounterWords = kQueueWord + 1;
-constexpr int kDefaultVariant = 2;
+constexpr int kDefaultVariant = 4;
 constexpr size_t kDrainLdsLimit = 144 * 1024;  // of the 160 KB per CU
 
 int solve_impl(ehss_scene* sc, const ehss_sobol* sob, int power, int cap, int precision, unsigned flags,
@@ -273,6 +285,14 @@ int solve_impl(ehss_scene* sc, const ehss_sobol* sob, int power, int cap, int pr
     A.info = sc->ray_info.as<uint32_t>();
     A.contrib = sc->ray_contrib.as<double>();
     A.counters = ctr;
+    A.coop = coop_from_flags(flags);
+    A.group = group_from_flags(flags, n);
+    A.times = nullptr;
+    if (flags & EHSS_FLAG_TIMESTAMPS) {
+        EHSS_TRY(sc->ray_times.reserve(sizeof(long long) * n));
+        A.times = sc->ray_times.as<long long>();
+    }
+    EHSS_CHECK(A.coop <= kWave, "tail threshold %d out of range 0..64", A.coop);
     const bool pa = (flags & EHSS_FLAG_PA) != 0;
     const dim3 grid((unsigned)((n + 255) / 256)), block(256);
     int variant = (int)((flags >> EHSS_VARIANT_SHIFT) & 0xFu);
@@ -367,6 +387,7 @@ int solve_impl(ehss_scene* sc, const ehss_sobol* sob, int power, int cap, int pr
     if (ledger_out)
         for (int k = 0; k <= cap; ++k) ledger_out[k] = (double)h_led[k] / kLedgerScale * A.w;
     sc->last_rays = (flags & EHSS_FLAG_KEEP_RAYS) ? n : 0;
+    sc->last_times = (flags & EHSS_FLAG_TIMESTAMPS) ? n : 0;
     sc->last_weight = A.w;
     return EHSS_OK;
 }
@@ -385,6 +406,23 @@ int ehss_ray_map(const ehss_sobol* sobol, int power, uint32_t* cols) {
 int ehss_solve(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, int precision, unsigned flags,
                ehss_totals* totals, int64_t* histogram, double* ledger) {
     return solve_impl(scene, sobol, power, cap, precision, flags, totals, histogram, ledger);
+}
+
+int ehss_scene_last_ray_times(ehss_scene* scene, int64_t n, int64_t* ticks, double* tick_hz) {
+    EHSS_CHECK(scene && ticks, "null scene/ticks");
+    EHSS_CHECK(scene->last_times > 0, "no time stamps kept: solve with EHSS_FLAG_TIMESTAMPS first");
+    EHSS_CHECK(n == scene->last_times, "record count %lld != %lld rays of the last solve", (long long)n,
+               (long long)scene->last_times);
+    hipStream_t s = scene->stream;
+    EHSS_HIP(hipMemcpyAsync(ticks, scene->ray_times.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, s));
+    EHSS_HIP(hipStreamSynchronize(s));
+    if (tick_hz) {
+        int dev = 0, khz = 0;
+        EHSS_HIP(hipGetDevice(&dev));
+        EHSS_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        *tick_hz = 1e3 * (double)khz;
+    }
+    return EHSS_OK;
 }
 
 int ehss_scene_last_rays(ehss_scene* scene, int64_t n, uint32_t* flags, double* contrib, int32_t* bounces) {

@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(256) k_pa_query(SceneView S, PAIO io) {
 
 // FP32 filter arrays: boxes rounded outward and inflated by m_box; spheres in
 // slot order, recentred on the scene centre.
-__global__ void k_make_filter(SceneView S, Node32* nodes32, float4* sph32) {
+__global__ void k_make_filter(SceneView S, Node32* nodes32, float4* sph32, double4* sphs, Node32* leaves32) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < S.n_nodes) {
         const Node64& n = S.nodes[i];
@@ -106,10 +106,16 @@ __global__ void k_make_filter(SceneView S, Node32* nodes32, float4* sph32) {
         m.end = n.end;
         m.leaf = n.leaf;
         nodes32[i] = m;
+        if (leaves32 && n.leaf >= 0) {
+            m.end = S.offsets[n.leaf];
+            m.leaf = S.offsets[n.leaf + 1] - m.end;
+            leaves32[n.leaf] = m;
+        }
     }
     if (i < S.n_atoms) {
         const double4 s = S.sph[S.ids[i]];
         sph32[i] = make_float4((float)(s.x - S.cx), (float)(s.y - S.cy), (float)(s.z - S.cz), (float)s.w);
+        sphs[i] = s;
     }
 }
 
@@ -122,10 +128,11 @@ void set_filter_margins(SceneView& v) {
     v.m_disc = (float)(1024.0 * u * v.scale * v.scale);
 }
 
-int launch_make_filter(const SceneView& v, Node32* nodes32, float4* sph32, hipStream_t s) {
+int launch_make_filter(const SceneView& v, Node32* nodes32, float4* sph32, double4* sphs, Node32* leaves32,
+                       hipStream_t s) {
     const int32_t n = v.n_nodes > v.n_atoms ? v.n_nodes : v.n_atoms;
     if (n == 0) return EHSS_OK;
-    hipLaunchKernelGGL(k_make_filter, dim3((n + 255) / 256), dim3(256), 0, s, v, nodes32, sph32);
+    hipLaunchKernelGGL(k_make_filter, dim3((n + 255) / 256), dim3(256), 0, s, v, nodes32, sph32, sphs, leaves32);
     EHSS_HIP(hipGetLastError());
     return EHSS_OK;
 }

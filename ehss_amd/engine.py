@@ -96,6 +96,22 @@ def precision_code(precision) -> int:
     return int(precision)
 
 
+def coop_flags(coop: int | None, group: int | None = None) -> int:
+    """Flag bits of a tail threshold and of the drain's lanes per ray (None:
+    library defaults)."""
+    flags = 0
+    if coop is not None:
+        if not 0 <= int(coop) <= 64:
+            raise ValueError("coop must be in 0..64")
+        flags |= (int(coop) + 1) << L.COOP_SHIFT
+    if group is not None:
+        g = int(group)
+        if g < 1 or g > 64 or g & (g - 1):
+            raise ValueError("group must be a power of two in 1..64")
+        flags |= (g.bit_length()) << L.GROUP_SHIFT
+    return flags
+
+
 class GPUScene:
     """Device-resident spheres with the static EHSS tree and the PA tree built
     on the GPU (ehss_scene_create)."""
@@ -199,14 +215,20 @@ class GPUScene:
 
     # -- fused device solve ---------------------------------------------------------
     def solve(self, sobol: L.Sobol, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
-              keep_rays: bool = False, histogram: bool = False, variant: int = 0) -> dict:
+              keep_rays: bool = False, histogram: bool = False, variant: int = 0, coop: int | None = None,
+              timestamps: bool = False, group: int | None = None) -> dict:
         """One fused device solve of 2^power rays.  `variant` selects the FP32
         filter kernel (0 default, 1 per-lane, 2 packet first query, 3 packet
-        throughout); all variants return identical results."""
+        throughout, 4 packet first queries then queue + persistent drain);
+        `coop` is the tail threshold (a wave with at most that many live rays
+        finishes them with wave-cooperative queries; 0 off, None default);
+        `group` is the lanes per ray of variant 4's drain.  All settings return
+        identical per-ray results."""
         totals = L.Totals()
         hist = np.zeros(cap + 1, np.int64) if histogram else None
         ledger = np.zeros(cap + 1, np.float64) if histogram else None
         flags = (L.FLAG_PA if pa else 0) | (L.FLAG_KEEP_RAYS if keep_rays else 0) | (int(variant) << L.VARIANT_SHIFT)
+        flags |= coop_flags(coop, group) | (L.FLAG_TIMESTAMPS if timestamps else 0)
         L.call("ehss_solve", self.handle, ct.byref(sobol), int(power), int(cap), precision_code(precision),
                flags, ct.byref(totals), L.ptr(hist), L.ptr(ledger))
         out = totals.as_dict()
@@ -224,6 +246,14 @@ class GPUScene:
                     escaped=(flags & L.RAY_ESCAPED) != 0, unresolved=(flags & L.RAY_UNRESOLVED) != 0,
                     pa_hit=(flags & L.RAY_PA_HIT) != 0, ehss_hit=(flags & L.RAY_EHSS_HIT) != 0,
                     grazing=(flags & L.RAY_GRAZING) != 0)
+
+    def last_ray_times(self, n: int) -> np.ndarray:
+        """Completion time of every ray of the last solve run with
+        timestamps=True, in microseconds after the first ray finished."""
+        ticks = np.empty(n, np.int64)
+        hz = ct.c_double(0.0)
+        L.call("ehss_scene_last_ray_times", self.handle, n, L.ptr(ticks), ct.byref(hz))
+        return (ticks - ticks.min()) * (1e6 / hz.value)
 
     def source_rays(self, sobol: L.Sobol, power: int) -> dict:
         n = 1 << power

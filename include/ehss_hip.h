@@ -50,11 +50,24 @@ extern "C" {
 #define EHSS_FLAG_PA 1u        /* also run the BVH any-hit PA on the same rays */
 #define EHSS_FLAG_KEEP_RAYS 2u /* keep per-ray records for ehss_scene_last_rays */
 #define EHSS_FLAG_GRAPH 4u     /* batch: capture the launch sequence in a hipGraph and replay it */
+#define EHSS_FLAG_TIMESTAMPS 8u /* record each ray's completion time (ehss_scene_last_ray_times) */
 /* Kernel variant of the FP32 filter path in flag bits 8..11 (0 = default):
  * 1 one ray per lane, 2 wave-uniform first query then per-lane bounces,
- * 3 wave-uniform traversal for every bounce.  All return identical results. */
+ * 3 wave-uniform traversal for every bounce, 4 wave-uniform first queries, then
+ * the rays still bouncing are compacted into a queue drained by a persistent
+ * kernel.  All return identical per-ray results. */
 #define EHSS_VARIANT_SHIFT 8
 #define EHSS_VARIANT(v) ((unsigned)(v) << EHSS_VARIANT_SHIFT)
+/* Tail threshold in flag bits 12..18: a wave left with at most n live rays
+ * finishes them one at a time with the whole wave on each query.  Field value 0
+ * selects the default; EHSS_COOP(0) turns the tail mode off.  n <= 64. */
+#define EHSS_COOP_SHIFT 12
+#define EHSS_COOP(n) (((unsigned)(n) + 1u) << EHSS_COOP_SHIFT)
+/* Variant 4: lanes per queued ray in the drain, 2^(k-1) for field value k in
+ * bits 19..21 (0 = default).  With one lane per ray the drain traverses the tree
+ * per lane; with more, each query is split over the ray's lanes. */
+#define EHSS_GROUP_SHIFT 19
+#define EHSS_GROUP_LOG2(k) (((unsigned)(k) + 1u) << EHSS_GROUP_SHIFT)
 
 /* Library / device. */
 int ehss_abi_version(void);
@@ -177,6 +190,11 @@ int ehss_solve(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, i
 #define EHSS_RAY_EHSS_HIT 0x8u
 #define EHSS_RAY_GRAZING 0x10u
 int ehss_scene_last_rays(ehss_scene* scene, int64_t n, uint32_t* flags, double* angle, int32_t* bounces);
+
+/* Per-ray completion time stamps of the last solve run with EHSS_FLAG_TIMESTAMPS
+ * (device wall clock ticks, ray-index order) and the clock rate in Hz.  A
+ * diagnostic of where a solve spends its time; no reference counterpart. */
+int ehss_scene_last_ray_times(ehss_scene* scene, int64_t n, int64_t* ticks, double* tick_hz);
 
 /* Device-generated source rays of a Sobol net (for parity tests of the source
  * mapping): origins/directions n x 3, u n x 4 (any may be NULL). */

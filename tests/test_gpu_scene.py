@@ -118,20 +118,25 @@ def test_plain_and_fp32_fixup_kernels_are_bit_identical(oracle, name):
     c, r = scenes.globule(650, 1) if name == "globule650" else scenes.SCENES[name]()
     with GPUScene(c, r) as sc:
         runs = {}
-        for prec, variant in (("fp64", 0), ("fp32-fixup", 1), ("fp32-fixup", 2), ("fp32-fixup", 3),
-                              ("fp32-fixup", 4)):
-            say("kernels", name, prec, variant)
+        # coop None: default tail threshold; 0: tail mode off; 64: every wave
+        # switches to cooperative queries as soon as it may.  group: lanes per
+        # ray in variant 4's drain (1: per-lane traversal).
+        kernels = [("fp64", 0, None, None)] + [("fp32-fixup", v, c, None) for v in (1, 2, 3, 4)
+                                               for c in (None, 0, 64)]
+        kernels += [("fp32-fixup", 4, c, g) for c in (0, 64) for g in (1, 2, 16, 64)]
+        for prec, variant, coop, group in kernels:
+            say("kernels", name, prec, variant, coop, group)
             res = sc.solve(scramble(7), 16, 64, precision=prec, pa=True, keep_rays=True, histogram=True,
-                           variant=variant)
-            runs[(prec, variant)] = (res, sc.last_rays(1 << 16))
-        a = runs[("fp64", 0)]
+                           variant=variant, coop=coop, group=group)
+            runs[(prec, variant, coop, group)] = (res, sc.last_rays(1 << 16))
+        a = runs[("fp64", 0, None, None)]
         for key, b in runs.items():
             assert a[0]["omega"] == b[0]["omega"] and a[0]["pa"] == b[0]["pa"], key
             np.testing.assert_array_equal(a[1]["contributions"], b[1]["contributions"], err_msg=str(key))
             phys = ~np.uint32(16)  # every flag bit except the kernel-specific grazing diagnostic
             np.testing.assert_array_equal(a[1]["flags"] & phys, b[1]["flags"] & phys, err_msg=str(key))
             np.testing.assert_array_equal(a[0]["histogram"], b[0]["histogram"], err_msg=str(key))
-        b = runs[("fp32-fixup", 2)]
+        b = runs[("fp32-fixup", 2, None, None)]
         # determinism: repeat
         again = sc.solve(scramble(7), 16, 64, precision="fp32-fixup", pa=True)
         assert again["omega"] == b[0]["omega"]
