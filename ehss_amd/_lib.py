@@ -95,10 +95,19 @@ _SIGS = {
     "ehss_scene_trace": (_INT, [_P, _I64, _INT, _INT, _INT] + [_P] * 8),
     "ehss_scene_pa_query": (_INT, [_P, _I64, _INT] + [_P] * 5),
     "ehss_solve": (_INT, [_P, ct.POINTER(Sobol), _INT, _INT, _INT, ct.c_uint, ct.POINTER(Totals), _P, _P]),
+    "ehss_solve_async": (_INT, [_P, ct.POINTER(Sobol), _INT, _INT, _INT, ct.c_uint]),
+    "ehss_solve_query": (_INT, [_P, ct.POINTER(_INT)]),
+    "ehss_solve_wait": (_INT, [_P, ct.POINTER(Totals), _P, _P]),
     "ehss_scene_last_rays": (_INT, [_P, _I64, _P, _P, _P]),
     "ehss_scene_last_ray_times": (_INT, [_P, _I64, _P, _P]),
     "ehss_ray_map": (_INT, [ct.POINTER(Sobol), _INT, _P]),
     "ehss_source_rays": (_INT, [_P, ct.POINTER(Sobol), _INT, _P, _P, _P]),
+    "ehss_batch_create": (_INT, [_INT, _P, _P, _P, _P, _INT, ct.c_uint, _INT, ct.POINTER(_P)]),
+    "ehss_batch_set_sobol": (_INT, [_P, _INT, ct.POINTER(Sobol)]),
+    "ehss_batch_launch": (_INT, [_P]),
+    "ehss_batch_query": (_INT, [_P, ct.POINTER(_INT)]),
+    "ehss_batch_wait": (_INT, [_P, _P, _P, _P]),
+    "ehss_batch_destroy": (_INT, [_P]),
 }
 
 _lib = None

@@ -53,6 +53,19 @@ int DevBuf::reserve(size_t n) {
     return alloc(n);
 }
 
+int HostBuf::reserve(size_t n) {
+    if (n <= bytes && p) return EHSS_OK;
+    release();
+    if (n == 0) return EHSS_OK;
+    hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return hip_error(e, "hipHostMalloc", __FILE__, __LINE__);
+    }
+    bytes = n;
+    return EHSS_OK;
+}
+
 int ensure_init() {
     std::lock_guard<std::mutex> lock(g_init_mutex);
     if (g_stream) return hipSetDevice(g_device) == hipSuccess ? EHSS_OK : set_error(EHSS_ERR_HIP, "hipSetDevice");

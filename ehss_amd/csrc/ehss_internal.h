@@ -56,6 +56,25 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// Pinned host buffer (hipHostMalloc), grow-only; device-to-host copies into it
+// are asynchronous.
+struct HostBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf() { release(); }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int reserve(size_t n);
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
 // ---------------------------------------------------------------------------
 // Device view of a scene (everything the kernels read).
 // ---------------------------------------------------------------------------
@@ -156,7 +175,14 @@ struct ehss_scene {
         frontier;
     ehss::SceneView view;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;
+    // pending asynchronous solve (at most one per scene)
+    bool pending = false;
+    int64_t p_rays = 0;
+    int p_cap = 0;
+    unsigned p_flags = 0;
+    double p_weight = 0;
+    ehss::HostBuf host_words;  // counters, histogram and ledger of the pending solve
     // solve workspace (grow-only)
     ehss::DevBuf ray_info, ray_contrib, partials, hist, ledger, counters, queue, ray_times;
     int64_t last_rays = 0, last_times = 0;

@@ -108,6 +108,7 @@ int create_scene(int64_t n, const double* centers, const double* radii, const in
             if (p) {
                 if (p->ev0) (void)hipEventDestroy(p->ev0);
                 if (p->ev1) (void)hipEventDestroy(p->ev1);
+                if (p->ev_done) (void)hipEventDestroy(p->ev_done);
                 if (p->stream) (void)hipStreamDestroy(p->stream);
                 delete p;
             }
@@ -126,6 +127,7 @@ int create_scene(int64_t n, const double* centers, const double* radii, const in
     EHSS_HIP(hipStreamCreateWithFlags(&sc->stream, hipStreamNonBlocking));
     EHSS_HIP(hipEventCreate(&sc->ev0));
     EHSS_HIP(hipEventCreate(&sc->ev1));
+    EHSS_HIP(hipEventCreateWithFlags(&sc->ev_done, hipEventDisableTiming));
     hipStream_t s = sc->stream;
     EHSS_TRY(sc->centers.alloc(sizeof(double) * 3 * n));
     EHSS_TRY(sc->radii.alloc(sizeof(double) * n));
@@ -215,6 +217,7 @@ int ehss_scene_destroy(ehss_scene* scene) {
     if (scene->stream) (void)hipStreamSynchronize(scene->stream);
     if (scene->ev0) (void)hipEventDestroy(scene->ev0);
     if (scene->ev1) (void)hipEventDestroy(scene->ev1);
+    if (scene->ev_done) (void)hipEventDestroy(scene->ev_done);
     if (scene->stream) (void)hipStreamDestroy(scene->stream);
     delete scene;
     return EHSS_OK;

@@ -252,9 +252,12 @@ constexpr int kCounterWords = kQueueWord + 1;
 constexpr int kDefaultVariant = 4;
 constexpr size_t kDrainLdsLimit = 144 * 1024;  // of the 160 KB per CU
 
-int solve_impl(ehss_scene* sc, const ehss_sobol* sob, int power, int cap, int precision, unsigned flags,
-               ehss_totals* totals, int64_t* histogram, double* ledger_out) {
-    EHSS_CHECK(sc && sob && totals, "null scene/sobol/totals");
+// Enqueues one solve on the scene's stream, with the copies of its totals,
+// histogram and ledger into the scene's pinned host buffers; solve_collect
+// waits for them.  One solve per scene may be pending.
+int solve_enqueue(ehss_scene* sc, const ehss_sobol* sob, int power, int cap, int precision, unsigned flags) {
+    EHSS_CHECK(sc && sob, "null scene/sobol");
+    if (sc->pending) return set_error(EHSS_ERR_BUSY, "a solve is already pending on this scene");
     EHSS_CHECK(power >= 0 && power <= 30, "power %d out of range 0..30", power);
     EHSS_CHECK(cap >= 1 && cap <= (int)kBounceMask - 1, "cap %d out of range 1..%u", cap, kBounceMask - 1);
     EHSS_CHECK(precision == EHSS_PRECISION_FP64 || precision == EHSS_PRECISION_FP32_FIXUP, "unknown precision %d",
@@ -347,22 +350,54 @@ int solve_impl(ehss_scene* sc, const ehss_sobol* sob, int power, int cap, int pr
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, sc->partials.as<Partial>(), nblocks,
                        reinterpret_cast<Partial*>(ctr + 8));
     EHSS_HIP(hipGetLastError());
-    unsigned long long host_ctr[kCounterWords];
-    EHSS_HIP(hipMemcpyAsync(host_ctr, ctr, sizeof host_ctr, hipMemcpyDeviceToHost, s));
-    std::vector<unsigned long long> h_hist(histogram ? cap + 1 : 0), h_led(ledger_out ? cap + 1 : 0);
-    if (histogram)
-        EHSS_HIP(hipMemcpyAsync(h_hist.data(), sc->hist.p, sizeof(unsigned long long) * (cap + 1),
-                                hipMemcpyDeviceToHost, s));
-    if (ledger_out)
-        EHSS_HIP(hipMemcpyAsync(h_led.data(), sc->ledger.p, sizeof(unsigned long long) * (cap + 1),
-                                hipMemcpyDeviceToHost, s));
-    EHSS_HIP(hipStreamSynchronize(s));
+    EHSS_TRY(sc->host_words.reserve(sizeof(unsigned long long) * (kCounterWords + 2 * (cap + 1))));
+    unsigned long long* hw = sc->host_words.as<unsigned long long>();
+    EHSS_HIP(hipMemcpyAsync(hw, ctr, sizeof(unsigned long long) * kCounterWords, hipMemcpyDeviceToHost, s));
+    EHSS_HIP(hipMemcpyAsync(hw + kCounterWords, sc->hist.p, sizeof(unsigned long long) * (cap + 1),
+                            hipMemcpyDeviceToHost, s));
+    EHSS_HIP(hipMemcpyAsync(hw + kCounterWords + (cap + 1), sc->ledger.p, sizeof(unsigned long long) * (cap + 1),
+                            hipMemcpyDeviceToHost, s));
+    EHSS_HIP(hipEventRecord(sc->ev_done, s));
+    sc->pending = true;
+    sc->p_rays = n;
+    sc->p_cap = cap;
+    sc->p_flags = flags;
+    sc->p_weight = A.w;
+    return EHSS_OK;
+}
+
+int solve_query(ehss_scene* sc, int* done) {
+    EHSS_CHECK(sc && done, "null scene/done");
+    if (!sc->pending) return set_error(EHSS_ERR_INVALID, "no solve pending on this scene");
+    const hipError_t e = hipEventQuery(sc->ev_done);
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return EHSS_OK;
+    }
+    EHSS_HIP(e);
+    *done = 1;
+    return EHSS_OK;
+}
+
+// Waits for the pending solve and fills totals (and, if not NULL, the cap + 1
+// histogram and ledger bins).
+int solve_collect(ehss_scene* sc, ehss_totals* totals, int64_t* histogram, double* ledger_out) {
+    EHSS_CHECK(sc && totals, "null scene/totals");
+    if (!sc->pending) return set_error(EHSS_ERR_INVALID, "no solve pending on this scene");
+    sc->pending = false;
+    EHSS_HIP(hipEventSynchronize(sc->ev_done));
     float ms = 0.f;
     EHSS_HIP(hipEventElapsedTime(&ms, sc->ev0, sc->ev1));
+    const unsigned long long* hw = sc->host_words.as<unsigned long long>();
+    const int cap = sc->p_cap;
+    const int64_t n = sc->p_rays;
+    const double w = sc->p_weight;
+    const bool pa = (sc->p_flags & EHSS_FLAG_PA) != 0;
     Partial fin;
-    memcpy(&fin, host_ctr + 8, sizeof fin);
+    memcpy(&fin, hw + 8, sizeof fin);
     ehss_totals& T = *totals;
-    T.weight = A.w;
+    memset(&T, 0, sizeof T);
+    T.weight = w;
     T.omega = fin.s + fin.c;
     T.omega_err = fin.c;
     T.rays = n;
@@ -371,24 +406,26 @@ int solve_impl(ehss_scene* sc, const ehss_sobol* sob, int power, int cap, int pr
     T.pa_hits = pa ? fin.pa_hits : 0;
     T.ehss_hits = fin.ehss_hits;
     T.grazing_rays = fin.grazing;
-    T.pa = pa ? A.w * (double)fin.pa_hits : 0.;
-    T.pa_from_ehss = A.w * (double)fin.ehss_hits;
-    T.residual_mass = A.w * (double)fin.unresolved;
+    T.pa = pa ? w * (double)fin.pa_hits : 0.;
+    T.pa_from_ehss = w * (double)fin.ehss_hits;
+    T.residual_mass = w * (double)fin.unresolved;
     T.tail_bound = 2 * T.residual_mass;
-    T.box_tests = (int64_t)host_ctr[0];
-    T.sphere_tests = (int64_t)host_ctr[1];
-    T.fp64_retests = (int64_t)(host_ctr[2] + host_ctr[5]);
-    T.pa_box_tests = (int64_t)host_ctr[3];
-    T.pa_sphere_tests = (int64_t)host_ctr[4];
-    T.max_bounces = (int64_t)(int)host_ctr[6];
+    T.box_tests = (int64_t)hw[0];
+    T.sphere_tests = (int64_t)hw[1];
+    T.fp64_retests = (int64_t)(hw[2] + hw[5]);
+    T.pa_box_tests = (int64_t)hw[3];
+    T.pa_sphere_tests = (int64_t)hw[4];
+    T.max_bounces = (int64_t)(int)hw[6];
     T.kernel_ms = ms;
+    const unsigned long long* h_hist = hw + kCounterWords;
+    const unsigned long long* h_led = h_hist + (cap + 1);
     if (histogram)
         for (int k = 0; k <= cap; ++k) histogram[k] = (int64_t)h_hist[k];
     if (ledger_out)
-        for (int k = 0; k <= cap; ++k) ledger_out[k] = (double)h_led[k] / kLedgerScale * A.w;
-    sc->last_rays = (flags & EHSS_FLAG_KEEP_RAYS) ? n : 0;
-    sc->last_times = (flags & EHSS_FLAG_TIMESTAMPS) ? n : 0;
-    sc->last_weight = A.w;
+        for (int k = 0; k <= cap; ++k) ledger_out[k] = (double)h_led[k] / kLedgerScale * w;
+    sc->last_rays = (sc->p_flags & EHSS_FLAG_KEEP_RAYS) ? n : 0;
+    sc->last_times = (sc->p_flags & EHSS_FLAG_TIMESTAMPS) ? n : 0;
+    sc->last_weight = w;
     return EHSS_OK;
 }
 
@@ -405,11 +442,24 @@ int ehss_ray_map(const ehss_sobol* sobol, int power, uint32_t* cols) {
 
 int ehss_solve(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, int precision, unsigned flags,
                ehss_totals* totals, int64_t* histogram, double* ledger) {
-    return solve_impl(scene, sobol, power, cap, precision, flags, totals, histogram, ledger);
+    EHSS_CHECK(totals, "null totals");
+    EHSS_TRY(solve_enqueue(scene, sobol, power, cap, precision, flags));
+    return solve_collect(scene, totals, histogram, ledger);
+}
+
+int ehss_solve_async(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, int precision, unsigned flags) {
+    return solve_enqueue(scene, sobol, power, cap, precision, flags);
+}
+
+int ehss_solve_query(ehss_scene* scene, int* done) { return solve_query(scene, done); }
+
+int ehss_solve_wait(ehss_scene* scene, ehss_totals* totals, int64_t* histogram, double* ledger) {
+    return solve_collect(scene, totals, histogram, ledger);
 }
 
 int ehss_scene_last_ray_times(ehss_scene* scene, int64_t n, int64_t* ticks, double* tick_hz) {
     EHSS_CHECK(scene && ticks, "null scene/ticks");
+    if (scene->pending) return set_error(EHSS_ERR_BUSY, "a solve is pending on this scene");
     EHSS_CHECK(scene->last_times > 0, "no time stamps kept: solve with EHSS_FLAG_TIMESTAMPS first");
     EHSS_CHECK(n == scene->last_times, "record count %lld != %lld rays of the last solve", (long long)n,
                (long long)scene->last_times);
@@ -427,6 +477,7 @@ int ehss_scene_last_ray_times(ehss_scene* scene, int64_t n, int64_t* ticks, doub
 
 int ehss_scene_last_rays(ehss_scene* scene, int64_t n, uint32_t* flags, double* contrib, int32_t* bounces) {
     EHSS_CHECK(scene, "null scene");
+    if (scene->pending) return set_error(EHSS_ERR_BUSY, "a solve is pending on this scene");
     EHSS_CHECK(scene->last_rays > 0, "no per-ray records kept: solve with EHSS_FLAG_KEEP_RAYS first");
     EHSS_CHECK(n == scene->last_rays, "record count %lld != %lld rays of the last solve", (long long)n,
                (long long)scene->last_rays);

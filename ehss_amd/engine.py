@@ -139,6 +139,12 @@ class GPUScene:
         self.info = {name: (list(getattr(info, name)) if name == "center" else getattr(info, name))
                      for name, _ in L.SceneInfo._fields_}
 
+    def device_bytes(self) -> int:
+        """Device memory the scene holds now (trees + solve workspace)."""
+        info = L.SceneInfo()
+        L.call("ehss_scene_get_info", self.handle, ct.byref(info))
+        return int(info.device_bytes)
+
     # -- lifetime -------------------------------------------------------------
     def close(self):
         if getattr(self, "_h", None):
@@ -237,6 +243,32 @@ class GPUScene:
             out["ledger"] = ledger
         return out
 
+    def solve_async(self, sobol: L.Sobol, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
+                    variant: int = 0, coop: int | None = None, group: int | None = None) -> None:
+        """Enqueue solve() on the scene's stream and return at once; collect
+        with solve_wait().  Solves of different scenes overlap on the device."""
+        flags = (L.FLAG_PA if pa else 0) | (int(variant) << L.VARIANT_SHIFT) | coop_flags(coop, group)
+        L.call("ehss_solve_async", self.handle, ct.byref(sobol), int(power), int(cap), precision_code(precision),
+               flags)
+        self._pending_cap = int(cap)
+
+    def solve_done(self) -> bool:
+        flag = ct.c_int(0)
+        L.call("ehss_solve_query", self.handle, ct.byref(flag))
+        return bool(flag.value)
+
+    def solve_wait(self, histogram: bool = False) -> dict:
+        cap = getattr(self, "_pending_cap", 0)
+        totals = L.Totals()
+        hist = np.zeros(cap + 1, np.int64) if histogram else None
+        ledger = np.zeros(cap + 1, np.float64) if histogram else None
+        L.call("ehss_solve_wait", self.handle, ct.byref(totals), L.ptr(hist), L.ptr(ledger))
+        out = totals.as_dict()
+        if histogram:
+            out["histogram"] = hist
+            out["ledger"] = ledger
+        return out
+
     def last_rays(self, n: int) -> dict:
         flags = np.empty(n, np.uint32)
         contrib = np.empty(n, np.float64)
@@ -260,3 +292,89 @@ class GPUScene:
         u, o, d = np.empty((n, 4)), np.empty((n, 3)), np.empty((n, 3))
         L.call("ehss_source_rays", self.handle, ct.byref(sobol), int(power), L.ptr(u), L.ptr(o), L.ptr(d))
         return dict(u=u, origins=o, directions=d)
+
+
+class GPUBatch:
+    """Many fused solves -- different scenes, scrambles, ray counts and caps -- in
+    one persistent launch (ehss_batch_*).  Each job's totals are bit-identical to
+    the same solve run alone (GPUScene.solve): every ray writes its own record
+    and each job is reduced in its own fixed order.
+
+    With graph=True the launch sequence is captured once into a hipGraph and
+    replayed by later launches; set_sobol() changes a job's scramble between
+    launches (graph or not).  launch() is asynchronous; done() polls and wait()
+    collects the per-job results."""
+
+    def __init__(self, scenes, sobols, powers, caps, precision="fp32-fixup", pa: bool = True, graph: bool = False,
+                 variant: int = 0, coop: int | None = None, group: int | None = None, blocks_per_cu: int = 0):
+        n = len(scenes)
+        if n == 0 or not (len(sobols) == len(powers) == len(caps) == n):
+            raise ValueError("one scene, scramble, power and cap per job required")
+        self.scenes = list(scenes)  # keep the scenes alive while the batch exists
+        self.caps = [int(c) for c in caps]
+        self.powers = [int(p) for p in powers]
+        handles = (ct.c_void_p * n)(*[s.handle for s in self.scenes])
+        sob = (L.Sobol * n)(*sobols)
+        pw = np.asarray(self.powers, np.int32)
+        cp = np.asarray(self.caps, np.int32)
+        flags = (L.FLAG_PA if pa else 0) | (L.FLAG_GRAPH if graph else 0) | (int(variant) << L.VARIANT_SHIFT)
+        flags |= coop_flags(coop, group)
+        h = ct.c_void_p()
+        L.call("ehss_batch_create", n, handles, sob, L.ptr(pw), L.ptr(cp), precision_code(precision), flags,
+               int(blocks_per_cu), ct.byref(h))
+        self._h = h
+        self.n_jobs = n
+
+    @property
+    def handle(self):
+        if not getattr(self, "_h", None):
+            raise RuntimeError("batch is closed")
+        return self._h
+
+    def set_sobol(self, job: int, sobol: L.Sobol) -> None:
+        L.call("ehss_batch_set_sobol", self.handle, int(job), ct.byref(sobol))
+
+    def launch(self) -> None:
+        L.call("ehss_batch_launch", self.handle)
+
+    def done(self) -> bool:
+        flag = ct.c_int(0)
+        L.call("ehss_batch_query", self.handle, ct.byref(flag))
+        return bool(flag.value)
+
+    def wait(self, histogram: bool = False) -> list[dict]:
+        totals = (L.Totals * self.n_jobs)()
+        bins = sum(c + 1 for c in self.caps)
+        hist = np.zeros(bins, np.int64) if histogram else None
+        ledger = np.zeros(bins, np.float64) if histogram else None
+        L.call("ehss_batch_wait", self.handle, totals, L.ptr(hist), L.ptr(ledger))
+        out, off = [], 0
+        for j in range(self.n_jobs):
+            d = totals[j].as_dict()
+            if histogram:
+                d["histogram"] = hist[off:off + self.caps[j] + 1].copy()
+                d["ledger"] = ledger[off:off + self.caps[j] + 1].copy()
+            off += self.caps[j] + 1
+            out.append(d)
+        return out
+
+    def run(self, histogram: bool = False) -> list[dict]:
+        self.launch()
+        return self.wait(histogram)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.fn("ehss_batch_destroy")(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

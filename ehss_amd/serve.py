@@ -1,0 +1,232 @@
+"""Request loop for CCS serving on one GPU.
+
+Each request is a structure (centres, radii).  The loop
+
+* admits a request when it has arrived, a slot is free and its device-memory
+  footprint (scene trees + solve workspace) fits the budget left -- the scene
+  and its trees are built on the device at admission and stay resident until
+  the request completes;
+* keeps every admitted request's next solve in flight on the request's own
+  stream (ehss_solve_async), so solves of different structures and sizes
+  overlap on the device and a new request joins as soon as it is admitted
+  (continuous batching, not a queue of whole jobs);
+* chooses the ray count per request adaptively: independent scrambled-Sobol
+  replicates of 2^power rays (seeds replicate_seed(seed, k), the reference's
+  per-replicate rule) until the relative standard error of the mean CCS is at
+  most target_rel (and at least min_reps replicates), capped at max_reps;
+* records arrival, admission and completion times, from which p50/p99 latency
+  and CCS/s follow.
+
+A request's result depends only on its structure, seed and solve settings: the
+kernels and the reduction are deterministic, so the same request gives the same
+bits alone and in any mix.
+"""
+from __future__ import annotations
+
+import collections
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# Device bytes per atom of a scene (ehss_scene_create): centres 24, radii 8,
+# FP64 spheres 32, groups 4, offsets 4, slot ids 4, FP64 tree <= 2 x 64, PA tree
+# <= 2 x 64, PA ids 4, FP32 tree <= 2 x 32, FP32 spheres 16, slot-ordered FP64
+# spheres 32, leaf boxes <= 32.
+SCENE_BYTES_PER_ATOM = 24 + 8 + 32 + 4 + 4 + 4 + 128 + 128 + 4 + 64 + 16 + 32 + 32
+SCENE_BYTES_FIXED = 64 * 2 + 4 * 8 * 64 + 4096  # PA root slack, frontier, small buffers
+# Solve workspace per ray: record word 4, contribution 8, queue entry 112.
+SOLVE_BYTES_PER_RAY = 4 + 8 + 112
+SOLVE_BYTES_FIXED = 64 * 1024
+
+
+def footprint_bytes(n_atoms: int, power: int, cap: int) -> int:
+    """Upper bound on the device memory one admitted request holds."""
+    rays = 1 << int(power)
+    reduce_blocks = rays // 1024 + 1
+    return (SCENE_BYTES_PER_ATOM * int(n_atoms) + SCENE_BYTES_FIXED + SOLVE_BYTES_PER_RAY * rays
+            + 64 * reduce_blocks + 16 * (int(cap) + 1) + SOLVE_BYTES_FIXED)
+
+
+@dataclass
+class Request:
+    rid: int
+    centers: np.ndarray
+    radii: np.ndarray
+    arrival: float = 0.0  # seconds after the start of the stream
+    seed: int = 0
+
+
+@dataclass
+class Result:
+    rid: int
+    atoms: int
+    ccs: float = math.nan          # mean Omega over replicates (A^2)
+    rel_se: float = math.nan       # standard error of the mean / mean
+    omegas: list = field(default_factory=list)
+    pa: list = field(default_factory=list)
+    unresolved: int = 0
+    rays: int = 0
+    arrival: float = 0.0
+    admitted: float = math.nan
+    finished: float = math.nan
+    error: str | None = None
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+@dataclass
+class _Live:
+    req: Request
+    res: Result
+    scene: object
+    footprint: int
+
+
+def _rel_se(values) -> float:
+    v = np.asarray(values, dtype=np.float64)
+    if len(v) < 2 or v.mean() == 0:
+        return math.inf
+    return float(v.std(ddof=1) / math.sqrt(len(v)) / abs(v.mean()))
+
+
+class Server:
+    """Continuous-batching CCS server over device-resident scenes.
+
+    scene_factory(centers, radii) must return an object with the GPUScene
+    asynchronous interface (solve_async, solve_done, solve_wait, close);
+    scramble(seed) and replicate_seed(seed, k) give the per-replicate nets."""
+
+    def __init__(self, budget_bytes: int, max_inflight: int = 16, power: int = 12, cap: int = 64,
+                 target_rel: float = 0.01, min_reps: int = 4, max_reps: int = 64, precision: str = "fp32-fixup",
+                 pa: bool = True, scene_factory=None, scramble=None, replicate_seed=None, poll_s: float = 5e-5):
+        if budget_bytes <= 0 or max_inflight < 1 or min_reps < 2 or max_reps < min_reps:
+            raise ValueError("invalid server limits")
+        if scene_factory is None or scramble is None or replicate_seed is None:
+            from .engine import GPUScene, replicate_seed as rs, scramble as sc
+            scene_factory = scene_factory or GPUScene
+            scramble = scramble or sc
+            replicate_seed = replicate_seed or rs
+        self.budget = int(budget_bytes)
+        self.max_inflight = int(max_inflight)
+        self.power, self.cap = int(power), int(cap)
+        self.target_rel, self.min_reps, self.max_reps = float(target_rel), int(min_reps), int(max_reps)
+        self.precision, self.pa = precision, bool(pa)
+        self.scene_factory, self.scramble, self.replicate_seed = scene_factory, scramble, replicate_seed
+        self.poll_s = poll_s
+        self.peak_bytes = 0
+        self.peak_inflight = 0
+
+    def footprint(self, req: Request) -> int:
+        return footprint_bytes(len(req.radii), self.power, self.cap)
+
+    def _launch(self, live: _Live) -> None:
+        k = len(live.res.omegas)
+        live.scene.solve_async(self.scramble(self.replicate_seed(live.req.seed, k)), self.power, self.cap,
+                               precision=self.precision, pa=self.pa)
+
+    def _converged(self, res: Result) -> bool:
+        k = len(res.omegas)
+        if k >= self.max_reps:
+            return True
+        return k >= self.min_reps and _rel_se(res.omegas) <= self.target_rel
+
+    def run(self, requests, clock=time.perf_counter, sleep=time.sleep) -> list[Result]:
+        """Serve `requests` (arrival times relative to the call); returns one
+        Result per request, in completion order."""
+        pending = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
+        waiting: collections.deque = collections.deque()
+        live: dict[int, _Live] = {}
+        done: list[Result] = []
+        used = 0
+        t0 = clock()
+        while pending or waiting or live:
+            now = clock() - t0
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+            # Admission: first come first served, by device-memory footprint.
+            while waiting and len(live) < self.max_inflight:
+                req = waiting[0]
+                fp = self.footprint(req)
+                if fp > self.budget:
+                    waiting.popleft()
+                    done.append(Result(req.rid, len(req.radii), arrival=req.arrival, finished=clock() - t0,
+                                       error=f"footprint {fp} B exceeds the device budget {self.budget} B"))
+                    continue
+                if used + fp > self.budget:
+                    break
+                waiting.popleft()
+                res = Result(req.rid, len(req.radii), arrival=req.arrival, admitted=clock() - t0)
+                try:
+                    scene = self.scene_factory(req.centers, req.radii)
+                except Exception as exc:  # invalid structure: report, keep serving
+                    res.error, res.finished = str(exc), clock() - t0
+                    done.append(res)
+                    continue
+                item = _Live(req, res, scene, fp)
+                used += fp
+                live[req.rid] = item
+                self._launch(item)
+            self.peak_bytes = max(self.peak_bytes, used)
+            self.peak_inflight = max(self.peak_inflight, len(live))
+            # Progress: collect every finished solve, relaunch or complete.
+            progressed = False
+            for rid in list(live):
+                item = live[rid]
+                if not item.scene.solve_done():
+                    continue
+                progressed = True
+                out = item.scene.solve_wait()
+                res = item.res
+                res.omegas.append(out["omega"])
+                res.pa.append(out["pa"])
+                res.unresolved += int(out["unresolved"])
+                res.rays += int(out["rays"])
+                if self._converged(res):
+                    res.ccs = float(np.mean(res.omegas))
+                    res.rel_se = _rel_se(res.omegas)
+                    res.finished = clock() - t0
+                    item.scene.close()
+                    used -= item.footprint
+                    del live[rid]
+                    done.append(res)
+                else:
+                    self._launch(item)
+            if not progressed:
+                if not live and not waiting and pending:
+                    sleep(max(0.0, min(pending[0].arrival - (clock() - t0), 0.01)))
+                else:
+                    sleep(self.poll_s)
+        return done
+
+
+def latency_summary(results) -> dict:
+    """p50/p99 latency (s), CCS/s over the stream, and counts."""
+    ok = [r for r in results if r.error is None]
+    lat = np.array([r.latency for r in ok]) if ok else np.zeros(0)
+    span = (max(r.finished for r in ok) - min(r.arrival for r in ok)) if ok else 0.0
+    return dict(requests=len(results), served=len(ok), rejected=len(results) - len(ok),
+                p50_s=float(np.percentile(lat, 50)) if len(lat) else math.nan,
+                p99_s=float(np.percentile(lat, 99)) if len(lat) else math.nan,
+                mean_s=float(lat.mean()) if len(lat) else math.nan,
+                ccs_per_s=len(ok) / span if span > 0 else math.nan,
+                rays=int(sum(r.rays for r in ok)),
+                max_rel_se=float(max(r.rel_se for r in ok)) if ok else math.nan)
+
+
+def poisson_stream(bases, rate_hz: float, count: int, sigma: float = 0.3, seed: int = 0) -> list[Request]:
+    """`count` requests with Poisson arrivals at `rate_hz`; each is a conformer
+    (Gaussian jitter sigma, A) of a base structure drawn uniformly from `bases`
+    (a list of (centers, radii))."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate_hz, size=count))
+    out = []
+    for k in range(count):
+        c, r = bases[int(rng.integers(0, len(bases)))]
+        jitter = rng.normal(scale=sigma, size=np.shape(c))
+        out.append(Request(rid=k, centers=np.asarray(c) + jitter, radii=np.asarray(r), arrival=float(t[k]),
+                           seed=int(rng.integers(0, 2 ** 31))))
+    return out

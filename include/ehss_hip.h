@@ -181,6 +181,15 @@ int ehss_ray_map(const ehss_sobol* sobol, int power, uint32_t* cols);
 int ehss_solve(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, int precision, unsigned flags,
                ehss_totals* totals, int64_t* histogram, double* ledger);
 
+/* The same solve split for overlapped use (no reference counterpart; used by the
+ * request loop): ehss_solve_async enqueues it on the scene's stream and returns;
+ * ehss_solve_query sets *done without blocking; ehss_solve_wait blocks and
+ * returns what ehss_solve returns.  One solve may be pending per scene
+ * (EHSS_ERR_BUSY otherwise); solves of different scenes run concurrently. */
+int ehss_solve_async(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, int precision, unsigned flags);
+int ehss_solve_query(ehss_scene* scene, int* done);
+int ehss_solve_wait(ehss_scene* scene, ehss_totals* totals, int64_t* histogram, double* ledger);
+
 /* Per-ray records of the last ehss_solve on this scene made with
  * EHSS_FLAG_KEEP_RAYS, in ray-index order: info bits (see below), angle
  * (1 - u.v clipped to [0,2], 0 unless escaped), bounces. */

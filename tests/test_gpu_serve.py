@@ -1,0 +1,76 @@
+"""Asynchronous solves and the request loop on the device: overlapped solves
+equal synchronous ones, every request's result is the same alone and in a
+mix, and the admission footprint bounds the device memory a request holds."""
+import numpy as np
+import pytest
+
+from ehss_amd import _lib as L
+from ehss_amd import molecules, serve
+from ehss_amd.engine import GPUScene, scramble
+from tests import scenes
+from tests.gpu_helpers import say
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("omega", "pa", "pa_from_ehss", "escaped", "unresolved", "pa_hits", "ehss_hits", "max_bounces", "rays")
+
+
+def test_async_solves_overlap_and_equal_synchronous():
+    items = [scenes.globule(650, 1), scenes.SCENES["c60"](), scenes.globule(300, 4)]
+    sc = [GPUScene(c, r) for c, r in items]
+    try:
+        sync = [s.solve(scramble(5 + k), 14, 64, histogram=True) for k, s in enumerate(sc)]
+        say("async enqueue")
+        for k, s in enumerate(sc):
+            s.solve_async(scramble(5 + k), 14, 64)
+        with pytest.raises(L.EHSSError, match="pending"):
+            sc[0].solve_async(scramble(1), 10, 64)
+        with pytest.raises(L.EHSSError, match="pending"):
+            sc[0].last_rays(1 << 14)
+        got = [s.solve_wait(histogram=True) for s in sc]
+        for a, b in zip(sync, got):
+            for k in KEYS:
+                assert a[k] == b[k], k
+            np.testing.assert_array_equal(a["histogram"], b["histogram"])
+        with pytest.raises(L.EHSSError, match="no solve pending"):
+            sc[1].solve_wait()
+        assert sc[2].solve(scramble(7), 12, 64)["omega"] > 0  # usable again
+    finally:
+        for s in sc:
+            s.close()
+
+
+def test_footprint_bounds_device_memory():
+    for n, seed in ((120, 3), (650, 1), (2000, 2)):
+        c, r = scenes.globule(n, seed)
+        with GPUScene(c, r) as s:
+            for power, cap in ((10, 64), (14, 128)):
+                s.solve(scramble(1), power, cap)
+                assert s.device_bytes() <= serve.footprint_bytes(n, power, cap), (n, power, cap)
+
+
+def _bases():
+    out = []
+    for n, seed in ((150, 21), (400, 22), (900, 23), (2500, 24)):
+        xyz, el = molecules.globule(n, seed)
+        out.append((xyz, molecules.radii_for(el)))
+    return out
+
+
+def test_results_are_the_same_alone_and_in_the_mix():
+    reqs = serve.poisson_stream(_bases(), rate_hz=400.0, count=16, seed=5)
+    kw = dict(max_inflight=8, power=11, cap=64, target_rel=0.02, min_reps=4, max_reps=24)
+    srv = serve.Server(1 << 34, **kw)
+    say("mix of", len(reqs))
+    mixed = {r.rid: r for r in srv.run(reqs)}
+    stats = serve.latency_summary(mixed.values())
+    say("mix", stats, "peak in flight", srv.peak_inflight)
+    assert stats["served"] == len(reqs)
+    assert srv.peak_inflight >= 2
+    for r in mixed.values():
+        assert r.error is None and (r.rel_se <= 0.02 or len(r.omegas) == 24)
+    for req in reqs[:4]:
+        alone = serve.Server(1 << 34, **kw)
+        (res,) = alone.run([serve.Request(req.rid, req.centers, req.radii, 0.0, req.seed)])
+        assert res.omegas == mixed[req.rid].omegas, req.rid
+        assert res.ccs == mixed[req.rid].ccs

@@ -1,0 +1,139 @@
+"""Request loop logic (admission by device-memory footprint, continuous batching,
+adaptive replicate count, latency accounting) on a simulated device: a fake
+scene whose solves take virtual time and return seeded values."""
+import math
+
+import numpy as np
+import pytest
+
+from ehss_amd import serve
+
+
+class Clock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        return self.t
+
+    def sleep(self, dt):
+        self.t += max(dt, 1e-6)
+
+
+class FakeScene:
+    """Solve time proportional to atoms; Omega = pi R^2 (1 + 0.03 z(seed))."""
+    live = 0
+
+    def __init__(self, clock, centers, radii):
+        if len(radii) == 0:
+            raise ValueError("Expected nonempty centers (N,3) and radii (N,).")
+        self.clock, self.n = clock, len(radii)
+        self.base = math.pi * (1.0 + float(np.max(np.linalg.norm(centers, axis=1)) + np.max(radii))) ** 2
+        self.busy_until, self.seed, self.closed = None, None, False
+        FakeScene.live += 1
+
+    def solve_async(self, sobol, power, cap, precision="fp32-fixup", pa=True):
+        assert self.busy_until is None, "one pending solve per scene"
+        self.seed, self.rays = sobol, 1 << power
+        self.busy_until = self.clock() + 1e-6 * self.n
+
+    def solve_done(self):
+        return self.busy_until is not None and self.clock() >= self.busy_until
+
+    def solve_wait(self):
+        assert self.solve_done()
+        self.busy_until = None
+        z = np.random.default_rng(self.seed).normal()
+        return dict(omega=self.base * (1 + 0.03 * z), pa=self.base, unresolved=0, rays=self.rays)
+
+    def close(self):
+        assert not self.closed
+        self.closed = True
+        FakeScene.live -= 1
+
+
+def make_server(clock, budget=1 << 40, **kw):
+    return serve.Server(budget, scene_factory=lambda c, r: FakeScene(clock, c, r), scramble=lambda s: s,
+                        replicate_seed=lambda s, k: (s + 104729 * k) % 2 ** 32, **kw)
+
+
+def stream(count=40, rate=2000.0, seed=3):
+    rng = np.random.default_rng(11)
+    bases = []
+    for n in (100, 400, 1500, 6000):
+        c = rng.normal(scale=n ** (1 / 3), size=(n, 3))
+        bases.append((c, np.full(n, 1.7)))
+    return serve.poisson_stream(bases, rate, count, seed=seed)
+
+
+def test_every_request_served_to_target_and_results_match_alone():
+    reqs = stream()
+    clock = Clock()
+    srv = make_server(clock, max_inflight=8, power=10, target_rel=0.01, min_reps=4, max_reps=200)
+    mixed = {r.rid: r for r in srv.run(reqs, clock=clock, sleep=clock.sleep)}
+    assert len(mixed) == len(reqs) and FakeScene.live == 0
+    assert srv.peak_inflight > 1  # solves of several requests were in flight together
+    for r in mixed.values():
+        assert r.error is None
+        assert r.rel_se <= 0.01 and len(r.omegas) >= 4
+        assert r.finished >= r.admitted >= r.arrival
+    for req in reqs[:6]:
+        c = Clock()
+        alone = make_server(c, max_inflight=8, power=10, target_rel=0.01, min_reps=4, max_reps=200)
+        (res,) = alone.run([serve.Request(req.rid, req.centers, req.radii, 0.0, req.seed)], clock=c, sleep=c.sleep)
+        assert res.omegas == mixed[req.rid].omegas and res.ccs == mixed[req.rid].ccs
+
+
+def test_admission_respects_the_device_budget():
+    reqs = stream(count=30, rate=1e6)  # all arrive at once
+    fps = sorted(serve.footprint_bytes(len(r.radii), 10, 64) for r in reqs)
+    budget = fps[-1] + fps[-2]  # room for the two largest at most
+    clock = Clock()
+    srv = make_server(clock, budget=budget, max_inflight=16, power=10)
+    res = srv.run(reqs, clock=clock, sleep=clock.sleep)
+    assert all(r.error is None for r in res)
+    assert srv.peak_bytes <= budget
+    assert srv.peak_inflight >= 2
+
+
+def test_oversized_request_is_rejected_and_the_rest_served():
+    reqs = stream(count=10)
+    big = serve.Request(99, np.zeros((50000, 3)), np.ones(50000), arrival=0.0, seed=1)
+    clock = Clock()
+    budget = max(serve.footprint_bytes(len(r.radii), 10, 64) for r in reqs) * 2
+    srv = make_server(clock, budget=budget, power=10)
+    res = {r.rid: r for r in srv.run(reqs + [big], clock=clock, sleep=clock.sleep)}
+    assert res[99].error and "budget" in res[99].error
+    assert all(res[r.rid].error is None for r in reqs)
+
+
+def test_invalid_structure_reports_an_error():
+    clock = Clock()
+    srv = make_server(clock, power=10)
+    (res,) = srv.run([serve.Request(0, np.zeros((0, 3)), np.zeros(0))], clock=clock, sleep=clock.sleep)
+    assert res.error and "nonempty" in res.error
+
+
+def test_max_reps_caps_a_noisy_request():
+    clock = Clock()
+    srv = make_server(clock, power=10, target_rel=1e-6, min_reps=3, max_reps=7)
+    (res,) = srv.run(stream(count=1), clock=clock, sleep=clock.sleep)
+    assert len(res.omegas) == 7 and res.rel_se > 1e-6
+
+
+def test_latency_summary_and_stream():
+    reqs = stream(count=50, rate=100.0)
+    arr = np.array([r.arrival for r in reqs])
+    assert np.all(np.diff(arr) > 0)
+    clock = Clock()
+    res = make_server(clock, power=10).run(reqs, clock=clock, sleep=clock.sleep)
+    s = serve.latency_summary(res)
+    assert s["served"] == 50 and s["p50_s"] <= s["p99_s"] and s["ccs_per_s"] > 0
+
+
+def test_limits_are_validated():
+    with pytest.raises(ValueError):
+        serve.Server(0, scene_factory=object, scramble=object, replicate_seed=object)
+    with pytest.raises(ValueError):
+        serve.Server(1 << 30, min_reps=5, max_reps=4, scene_factory=object, scramble=object,
+                     replicate_seed=object)
