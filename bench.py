@@ -1,14 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark of the EHSS/PA hot path on one MI355X.
 
-Workload (one step): a fused device solve of the 650-atom synthetic globular
+Workload of one step: a fused device solve of the 650-atom synthetic globular
 molecule (random walk, 1.5 A steps, 0.11 atoms/A^3, seed 1): 2^20 scrambled
-Sobol rays -> impact-disk source -> BVH any-hit PA -> reflect-until-escape
-trace (cap 64) -> FP64 compensated (1 - cos chi) reduction.  Inputs (the scene
-and its trees) are resident in HBM before the timed region; each step uses its
-own replicate seed.
+Sobol rays -> impact-disk source -> BVH any-hit PA -> reflect-until-escape trace
+(cap 64) -> FP64 compensated (1 - cos chi) reduction.  The scene and its trees
+are resident in HBM before the timed region; each step uses its own replicate
+seed.  `value` is traced rays per second of the whole step (host wall clock).
 
-Prints one JSON line (metric/value/roofline/cpu_baseline, plus extras).
+Side measurements (not part of `value`): the plain FP64 kernel on the same
+workload, CCS/s at 1 % relative RMSE (single stream, and with overlapped
+solves), a Poisson serving stream (p50/p99 latency), the 2000-atom ellipsoid at
+2^22 rays / cap 128 (FP32 error, re-test fraction, unresolved bound), the C60
+case, and the reference CPU kernel on the host cores.
+
+Prints one JSON line.  --traffic-probe runs only a fixed copy of known size and
+a few solves (the command profiled for HBM traffic).
 """
 from __future__ import annotations
 
@@ -26,42 +33,95 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "EHSS traced rays/s + CCS/s at 1% RMSE on 1 MI355X; p50/p99 serve latency"
-FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak (packed); 78.6 without packing
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
+HBM_PEAK_GBS = 8000.0
 POWER, CAP, SEED = 20, 64, 1
+COPY_BYTES = 256 << 20    # calibration copy of the traffic probe (bytes read = bytes written)
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_solve_globule650.txt"
+
+
+def say(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
 
 
 def flops_per_ray(box, sph, refl, pa_box, pa_sph, rays):
-    """Algorithmic FLOPs per traced ray: 12 per box test, 29 per sphere test, 47 per
-    reflection, 7 readout, plus 12 per PA box test and 19 per PA sphere test; the
-    counts are those of the reference traversal (leaf 4, stackless preorder)."""
+    """Algorithmic FLOPs per traced ray of the reference traversal (leaf 4,
+    stackless preorder): 12 per box test, 29 per sphere test, 47 per
+    reflection, 12 per PA box test, 19 per PA sphere test, 7 for the readout."""
     return (12 * box + 29 * sph + 47 * refl + 12 * pa_box + 19 * pa_sph) / rays + 7
 
 
-def rays_per_ccs_1pct(scene, scramble, replicate_seed, precision, reps=24):
+def read_traffic():
+    """HBM bytes per launch of the trace kernels from the committed rocprofv3
+    counter summary, or None."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        if line.startswith("bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def rays_per_ccs(scene, scramble, replicate_seed, reps=32, target=0.01):
     """Smallest power-of-two ray count whose relative SD of Omega over
-    independent scrambles is <= 1 % (one estimate = one CCS)."""
+    independent scrambles is <= target (one estimate = one CCS)."""
+    rel = math.inf
     for power in range(6, 21):
-        vals = [scene.solve(scramble(replicate_seed(100 + k, 0)), power, CAP, precision=precision, pa=False)["omega"]
+        vals = [scene.solve(scramble(replicate_seed(100 + k, 0)), power, CAP, pa=False)["omega"]
                 for k in range(reps)]
         rel = float(np.std(vals, ddof=1) / np.mean(vals))
-        if rel <= 0.01:
-            return 1 << power, rel
-    return 1 << 20, rel
+        if rel <= target:
+            return power, rel
+    return 20, rel
 
 
-def cpu_baseline(centers, radii):
-    """Reference CPU kernel (src/native/ehss_cpu.cpp compiled into oracle/_ref) on a
-    bounded sample of the same workload, all host cores; falls back to the
-    project's C restatement of the whole path when oracle/_ref is absent."""
+def ccs_rate_sequential(scene, sobols, power, precision):
+    t = time.perf_counter()
+    for s in sobols:
+        scene.solve(s, power, CAP, precision=precision, pa=False)
+    return len(sobols) / (time.perf_counter() - t)
+
+
+def ccs_rate_overlapped(scenes, sobols, power, precision):
+    """CCS estimates with one solve in flight per scene copy (each on its own
+    stream), refilled as they finish."""
+    it = iter(sobols)
+    live = []
+    count = 0
+    t = time.perf_counter()
+    for s in scenes:
+        nxt = next(it, None)
+        if nxt is None:
+            break
+        s.solve_async(nxt, power, CAP, precision=precision, pa=False)
+        live.append(s)
+    while live:
+        for s in list(live):
+            if s.solve_done():
+                s.solve_wait()
+                count += 1
+                nxt = next(it, None)
+                if nxt is None:
+                    live.remove(s)
+                else:
+                    s.solve_async(nxt, power, CAP, precision=precision, pa=False)
+        time.sleep(2e-5)
+    return count / (time.perf_counter() - t)
+
+
+def cpu_baseline(centers, radii, sample_power=21):
+    """Reference CPU kernel (src/native/ehss_cpu.cpp, built from the reference
+    sources into oracle/_ref) on a bounded sample of the same workload, all
+    host cores; the project's C restatement of the whole path when the
+    reference build is absent."""
     from oracle import oracle as O
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     ref = O.ref_lib()
-    sample_power = 18
     if ref is not None:
         src = O.external_source(centers, radii, sample_power, SEED)
         tree = O.static_tree(centers, radii, 4)
         times = []
-        for _ in range(5):
+        for _ in range(3):
             t0 = time.perf_counter()
             O.trace(tree, centers, radii, src["origins"], src["directions"], cap=CAP, record=False,
                     lib_handle=ref, threads=cores)
@@ -69,16 +129,34 @@ def cpu_baseline(centers, radii):
         t = float(np.median(times))
         return dict(value=(1 << sample_power) / t, unit="rays/s", cores=cores, kind="reference",
                     sample=f"reference ehss_trace (src/native/ehss_cpu.cpp, g++ -O2 -fopenmp) on 2^{sample_power} "
-                           f"rays of the same molecule/seed/cap, trace only (source/PA/readout excluded), "
-                           f"median of 5 warm runs, {cores} OpenMP threads")
+                           f"rays of the same molecule, seed and cap; trace only (source, PA and readout "
+                           f"excluded); median of 3 warm runs with {cores} OpenMP threads")
     times = []
     for _ in range(3):
         t0 = time.perf_counter()
-        O.solve_cpu(centers, radii, SEED, sample_power, CAP, threads=cores)
+        O.solve_cpu(centers, radii, SEED, sample_power - 2, CAP, threads=cores)
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
-    return dict(value=(1 << sample_power) / t, unit="rays/s", cores=cores, kind="port",
-                sample=f"oracle C restatement of the whole path on 2^{sample_power} rays, median of 3")
+    return dict(value=(1 << (sample_power - 2)) / t, unit="rays/s", cores=cores, kind="port",
+                sample=f"oracle C restatement of the whole path on 2^{sample_power - 2} rays, median of 3")
+
+
+def traffic_probe(args):
+    """The command profiled for HBM traffic: one copy of COPY_BYTES (the
+    calibration kernel), then a few solves of the workload."""
+    import torch
+    from ehss_amd import molecules
+    from ehss_amd.engine import GPUScene, scramble, replicate_seed
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    centers, radii, _ = molecules.workload("globule650")
+    with GPUScene(centers, radii) as scene:
+        for k in range(args.steps):
+            scene.solve(scramble(replicate_seed(SEED, k)), POWER, CAP, precision=args.precision, pa=True)
+    say("traffic probe done")
+    return 0
 
 
 def main():
@@ -88,27 +166,30 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="fp32-fixup", choices=["fp32-fixup", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--quick", action="store_true", help="skip the 1%% RMSE and plain-kernel side measurements")
+    ap.add_argument("--quick", action="store_true", help="skip the side measurements")
+    ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the EHSS/PA path here runs on a single MI355X; "
-              "multi-GPU sharding is a design note only (run with --gpus 1).", file=sys.stderr)
+        print("bench.py measures one GPU: the EHSS/PA path runs on a single MI355X here and its "
+              "multi-GPU sharding is a design note only; run with --gpus 1.", file=sys.stderr)
         return 2
+    if args.traffic_probe:
+        return traffic_probe(args)
 
     from ehss_amd import _lib as L
-    from ehss_amd import molecules
+    from ehss_amd import molecules, serve
     from ehss_amd.engine import GPUScene, scramble, replicate_seed
 
     centers, radii, meta = molecules.workload("globule650")
     dev = L.device_info()
-    print(f"[bench] device {dev['name']} CUs {dev['compute_units']}", file=sys.stderr, flush=True)
+    say(f"device {dev['name']} CUs {dev['compute_units']}")
     scene = GPUScene(centers, radii)
     seeds = [replicate_seed(SEED, k) for k in range(args.warmup + args.steps)]
     sobols = [scramble(s) for s in seeds]
     n = 1 << POWER
 
-    # Warmup, then exactly K timed steps bracketed by device synchronisation
-    # (every solve returns after its stream has drained).
+    # Warmup, then exactly K timed steps; every solve returns after its stream
+    # has drained, so the region is bracketed by device synchronisation.
     for k in range(args.warmup):
         scene.solve(sobols[k], POWER, CAP, precision=args.precision, pa=True)
     kernel_ms = []
@@ -119,7 +200,7 @@ def main():
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
     value = n * args.steps / elapsed
-    print(f"[bench] {args.steps} steps, {ms_per_step:.3f} ms/step, {value:.4g} rays/s", file=sys.stderr, flush=True)
+    say(f"{args.steps} steps, {ms_per_step:.3f} ms/step, {value:.4g} rays/s")
 
     # Algorithmic work per ray from the plain FP64 kernel, whose box/sphere
     # counters equal the reference traversal's (pinned by the parity tests).
@@ -129,39 +210,87 @@ def main():
                         plain["pa_sphere_tests"], n)
     avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
     achieved = fpr * n / avg_kernel_s / 1e12
-    traffic = None
-    tfile = ROOT / "profiles" / f"traffic_solve_{args.precision}_globule650.json"
-    if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+    kernels = ("k_solve<4,1> + k_drain<1>" if args.precision == "fp32-fixup" else "k_solve<0,1>")
     roofline = dict(bound="valu", achieved=round(achieved, 4), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=round(achieved / FP32_PEAK_TFLOPS, 6), traffic=traffic,
-                    kernel=f"k_solve<{'fast' if args.precision == 'fp32-fixup' else 'plain'},PA>",
-                    kernel_ms=round(float(np.mean(kernel_ms)), 4), flops_per_ray=round(fpr, 2),
-                    per_ray=dict(box=plain["box_tests"] / n, sphere=plain["sphere_tests"] / n, reflections=refl / n,
-                                 pa_box=plain["pa_box_tests"] / n, pa_sphere=plain["pa_sphere_tests"] / n))
+                    frac=round(achieved / FP32_PEAK_TFLOPS, 6), traffic=read_traffic(),
+                    kernel=kernels, kernel_ms=round(float(np.mean(kernel_ms)), 4), flops_per_ray=round(fpr, 2),
+                    per_ray=dict(box=plain["box_tests"] / n, sphere=plain["sphere_tests"] / n,
+                                 reflections=refl / n, pa_box=plain["pa_box_tests"] / n,
+                                 pa_sphere=plain["pa_sphere_tests"] / n))
+    if roofline["traffic"]:
+        roofline["traffic_gbs"] = round(roofline["traffic"] / avg_kernel_s / 1e9, 2)
+        roofline["traffic_frac_of_hbm_peak"] = round(roofline["traffic_gbs"] / HBM_PEAK_GBS, 6)
 
     extras = {}
     if not args.quick:
-        # Plain FP64 kernel on the same workload (the GPU baseline).
+        say("plain FP64 kernel")
         t1 = time.perf_counter()
         for k in range(20):
             scene.solve(sobols[k], POWER, CAP, precision="fp64", pa=True)
         plain_rate = 20 * n / (time.perf_counter() - t1)
-        rays_ccs, rel = rays_per_ccs_1pct(scene, scramble, replicate_seed, args.precision)
-        power_ccs = rays_ccs.bit_length() - 1
-        ccs_sobols = [scramble(replicate_seed(7, k)) for k in range(200)]
-        rates = {}
-        for prec in ("fp32-fixup", "fp64"):
-            t2 = time.perf_counter()
-            for s in ccs_sobols:
-                scene.solve(s, power_ccs, CAP, precision=prec, pa=False)
-            rates[prec] = len(ccs_sobols) / (time.perf_counter() - t2)
-        extras = dict(plain_fp64_rays_per_s=plain_rate, speedup_vs_plain_fp64=value / plain_rate,
-                      rays_per_ccs_1pct_rmse=rays_ccs, rel_sd_at_that_count=rel,
-                      ccs_per_s_single_stream=rates[args.precision], ccs_per_s_single_stream_plain_fp64=rates["fp64"],
-                      omega_angstrom2=res["omega"], fp64_retests_per_ray=res["fp64_retests"] / n,
-                      grazing_rays=res["grazing_rays"])
-    scene.close()
+        say("CCS at 1% RMSE")
+        power_ccs, rel = rays_per_ccs(scene, scramble, replicate_seed)
+        ccs_sobols = [scramble(replicate_seed(7, k)) for k in range(400)]
+        seq_fast = ccs_rate_sequential(scene, ccs_sobols, power_ccs, "fp32-fixup")
+        seq_plain = ccs_rate_sequential(scene, ccs_sobols[:100], power_ccs, "fp64")
+        copies = [GPUScene(centers, radii) for _ in range(8)]
+        ovl_fast = ccs_rate_overlapped(copies, ccs_sobols, power_ccs, "fp32-fixup")
+        ovl_plain = ccs_rate_overlapped(copies, ccs_sobols[:200], power_ccs, "fp64")
+        for c in copies:
+            c.close()
+        extras["globule650"] = dict(
+            plain_fp64_rays_per_s=plain_rate, speedup_vs_plain_fp64=value / plain_rate,
+            omega_A2=res["omega"], pa_A2=res["pa"], fp64_retests_per_ray=res["fp64_retests"] / n,
+            grazing_rays=res["grazing_rays"], unresolved_rays=res["unresolved"], tail_bound_A2=res["tail_bound"],
+            omega_fp32_fixup_minus_fp64=scene.solve(sobols[0], POWER, CAP, pa=True)["omega"] - plain["omega"])
+        extras["ccs_1pct"] = dict(
+            rays_per_ccs=1 << power_ccs, rel_sd_at_that_count=rel,
+            ccs_per_s_fp32_fixup_single_stream=seq_fast, ccs_per_s_plain_fp64_single_stream=seq_plain,
+            ccs_per_s_fp32_fixup_overlapped_8=ovl_fast, ccs_per_s_plain_fp64_overlapped_8=ovl_plain,
+            served_fp32_fixup_vs_plain_fp64=ovl_fast / seq_plain)
+        scene.close()
+
+        say("ellipsoid2000, 2^22 rays, cap 128")
+        c3, r3, _ = molecules.workload("ellipsoid2000")
+        with GPUScene(c3, r3) as s3:
+            a = s3.solve(scramble(2), 22, 128, precision="fp64", pa=True)
+            b = s3.solve(scramble(2), 22, 128, precision="fp32-fixup", pa=True)
+            t = time.perf_counter()
+            for k in range(5):
+                s3.solve(scramble(replicate_seed(2, k)), 22, 128, pa=True)
+            rate3 = 5 * (1 << 22) / (time.perf_counter() - t)
+        extras["ellipsoid2000"] = dict(
+            rays=1 << 22, cap=128, omega_fp64=a["omega"], omega_fp32_fixup=b["omega"],
+            fp32_rel_error_vs_fp64=abs(b["omega"] - a["omega"]) / a["omega"],
+            fp64_retest_fraction=b["fp64_retests"] / max(1, b["sphere_tests"]),
+            fp64_retests_per_ray=b["fp64_retests"] / (1 << 22), grazing_rays=b["grazing_rays"],
+            unresolved_rays=b["unresolved"], unresolved_bound_A2=b["tail_bound"], rays_per_s=rate3)
+
+        say("C60, 3 x 65536 rays")
+        c1, r1, _ = molecules.workload("c60")
+        with GPUScene(c1, r1) as s1:
+            vals = [s1.solve(scramble(replicate_seed(0, k)), 16, 64, pa=True) for k in range(3)]
+        extras["c60"] = dict(omega_mean=float(np.mean([v["omega"] for v in vals])),
+                             pa_mean=float(np.mean([v["pa"] for v in vals])),
+                             ms_per_repeat=float(np.mean([v["kernel_ms"] for v in vals])))
+
+        say("serving stream")
+        rng = np.random.default_rng(2024)
+        sizes = np.exp(rng.uniform(math.log(100), math.log(20000), size=6)).astype(int)
+        bases = []
+        for k, na in enumerate(sizes):
+            xyz, el = molecules.globule(int(na), 500 + k)
+            bases.append((xyz, molecules.radii_for(el)))
+        reqs = serve.poisson_stream(bases, rate_hz=200.0, count=120, seed=9)
+        srv = serve.Server(int(0.8 * dev["total_mem"]), max_inflight=16, power=12, cap=64, target_rel=0.01,
+                           min_reps=4, max_reps=64)
+        results = srv.run(reqs)
+        stats = serve.latency_summary(results)
+        extras["serving"] = dict(stats, rate_hz=200.0, requests=len(reqs), base_atoms=[int(s) for s in sizes],
+                                 rays_per_replicate=1 << 12, peak_inflight=srv.peak_inflight,
+                                 peak_admitted_bytes=srv.peak_bytes)
+    else:
+        scene.close()
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(centers, radii)
     line = dict(metric=METRIC, value=value, unit="rays/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
