@@ -186,6 +186,10 @@ struct ehss_scene {
     // solve workspace (grow-only)
     ehss::DevBuf ray_info, ray_contrib, partials, hist, ledger, counters, queue, ray_times;
     int64_t last_rays = 0, last_times = 0;
+    uint32_t last_map[32] = {0};  // ray map of the last solve (slot -> ray index)
+    int last_power = 0;
+    uint32_t p_map[32] = {0};
+    int p_power = 0;
     double last_weight = 0;
     int64_t device_bytes() const;
 };

@@ -151,7 +151,7 @@ struct RayCtx {
     V3 o, d, inc;
     PathError pe;
     double tol;
-    uint32_t i, flags;
+    uint32_t i, flags;  // i: record slot (thread index t; ray index = inverse Gray of the map's g(t))
     int32_t last, count;
     bool done, escaped;
 };
@@ -193,7 +193,10 @@ __device__ __forceinline__ void ray_process(const SceneView& S, int32_t cap, Ray
     }
 }
 
-// The ray's record (ehss_state.py:84-87): bounces and flags, w (1 - u.v) if escaped.
+// The ray's record (ehss_state.py:84-87): bounces and flags, w (1 - u.v) if escaped,
+// stored at the ray's slot.  Slots follow thread order, a fixed bijection with
+// ray indices (ehss_ray_map), so the fixed-order reduction over slots does not
+// depend on scheduling; host readback permutes to ray-index order.
 __device__ __forceinline__ void ray_record(const SolveArgs& A, RayCtx& R) {
     double contrib = 0.;
     if (R.escaped) {
@@ -322,7 +325,7 @@ __device__ __forceinline__ void solve_ray(const SceneView& S, const SolveArgs& A
     uint64_t boxes = 0, spheres = 0, retests = 0, pboxes = 0, ptests = 0, pretests = 0;
     RayCtx R;
     const uint32_t g = valid ? ray_gray(map, t, A.power) : 0u;
-    R.i = inverse_gray(g);
+    R.i = t;  // record slot: thread order, so a wave's record writes are contiguous
     R.o = V3{0., 0., 0.};
     R.d = V3{0., 0., 1.};
     if (valid) {

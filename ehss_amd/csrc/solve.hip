@@ -252,6 +252,27 @@ constexpr int kCounterWords = kQueueWord + 1;
 constexpr int kDefaultVariant = 4;
 constexpr size_t kDrainLdsLimit = 144 * 1024;  // of the 160 KB per CU
 
+// Ray index of record slot t (host copy of the device mapping: Gray index from
+// the ray map, then inverse Gray code).
+uint32_t slot_ray(const uint32_t cols[32], int power, uint32_t t) {
+    uint32_t g = 0;
+    if (power < 6) {
+        g = t ^ (t >> 1);
+    } else {
+        const uint32_t w = t >> 6, lane = t & 63u;
+        for (int k = 0; k < power - 6; ++k)
+            if ((w >> k) & 1u) g ^= cols[k];
+        for (int k = 0; k < 6; ++k)
+            if ((lane >> k) & 1u) g ^= cols[power - 6 + k];
+    }
+    g ^= g >> 1;
+    g ^= g >> 2;
+    g ^= g >> 4;
+    g ^= g >> 8;
+    g ^= g >> 16;
+    return g;
+}
+
 // Enqueues one solve on the scene's stream, with the copies of its totals,
 // histogram and ledger into the scene's pinned host buffers; solve_collect
 // waits for them.  One solve per scene may be pending.
@@ -359,6 +380,8 @@ int solve_enqueue(ehss_scene* sc, const ehss_sobol* sob, int power, int cap, int
                             hipMemcpyDeviceToHost, s));
     EHSS_HIP(hipEventRecord(sc->ev_done, s));
     sc->pending = true;
+    memcpy(sc->p_map, rmap.col, sizeof sc->p_map);
+    sc->p_power = power;
     sc->p_rays = n;
     sc->p_cap = cap;
     sc->p_flags = flags;
@@ -426,6 +449,8 @@ int solve_collect(ehss_scene* sc, ehss_totals* totals, int64_t* histogram, doubl
     sc->last_rays = (sc->p_flags & EHSS_FLAG_KEEP_RAYS) ? n : 0;
     sc->last_times = (sc->p_flags & EHSS_FLAG_TIMESTAMPS) ? n : 0;
     sc->last_weight = w;
+    memcpy(sc->last_map, sc->p_map, sizeof sc->last_map);
+    sc->last_power = sc->p_power;
     return EHSS_OK;
 }
 
@@ -464,8 +489,10 @@ int ehss_scene_last_ray_times(ehss_scene* scene, int64_t n, int64_t* ticks, doub
     EHSS_CHECK(n == scene->last_times, "record count %lld != %lld rays of the last solve", (long long)n,
                (long long)scene->last_times);
     hipStream_t s = scene->stream;
-    EHSS_HIP(hipMemcpyAsync(ticks, scene->ray_times.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, s));
+    std::vector<int64_t> slots((size_t)n);
+    EHSS_HIP(hipMemcpyAsync(slots.data(), scene->ray_times.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, s));
     EHSS_HIP(hipStreamSynchronize(s));
+    for (int64_t t = 0; t < n; ++t) ticks[slot_ray(scene->last_map, scene->last_power, (uint32_t)t)] = slots[t];
     if (tick_hz) {
         int dev = 0, khz = 0;
         EHSS_HIP(hipGetDevice(&dev));
@@ -483,12 +510,16 @@ int ehss_scene_last_rays(ehss_scene* scene, int64_t n, uint32_t* flags, double* 
                (long long)scene->last_rays);
     hipStream_t s = scene->stream;
     std::vector<uint32_t> info((size_t)n);
+    std::vector<double> ang(contrib ? (size_t)n : 0);
     EHSS_HIP(hipMemcpyAsync(info.data(), scene->ray_info.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
-    if (contrib) EHSS_HIP(hipMemcpyAsync(contrib, scene->ray_contrib.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    if (contrib)
+        EHSS_HIP(hipMemcpyAsync(ang.data(), scene->ray_contrib.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     EHSS_HIP(hipStreamSynchronize(s));
-    for (int64_t i = 0; i < n; ++i) {
-        if (flags) flags[i] = info[i] >> kFlagShift;
-        if (bounces) bounces[i] = (int32_t)(info[i] & kBounceMask);
+    for (int64_t t = 0; t < n; ++t) {  // slot order -> ray-index order
+        const uint32_t i = slot_ray(scene->last_map, scene->last_power, (uint32_t)t);
+        if (flags) flags[i] = info[t] >> kFlagShift;
+        if (bounces) bounces[i] = (int32_t)(info[t] & kBounceMask);
+        if (contrib) contrib[i] = ang[t];
     }
     return EHSS_OK;
 }
