@@ -96,6 +96,8 @@ _SIGS = {
     "ehss_scene_pa_query": (_INT, [_P, _I64, _INT] + [_P] * 5),
     "ehss_solve": (_INT, [_P, ct.POINTER(Sobol), _INT, _INT, _INT, ct.c_uint, ct.POINTER(Totals), _P, _P]),
     "ehss_solve_async": (_INT, [_P, ct.POINTER(Sobol), _INT, _INT, _INT, ct.c_uint]),
+    "ehss_solve_many": (_INT, [_P, _INT, _P, _INT, _INT, _INT, ct.c_uint, _P, _P, _P]),
+    "ehss_solve_many_async": (_INT, [_P, _INT, _P, _INT, _INT, _INT, ct.c_uint]),
     "ehss_solve_query": (_INT, [_P, ct.POINTER(_INT)]),
     "ehss_solve_wait": (_INT, [_P, ct.POINTER(Totals), _P, _P]),
     "ehss_scene_last_rays": (_INT, [_P, _I64, _P, _P, _P]),

@@ -178,13 +178,15 @@ struct ehss_scene {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;
     // pending asynchronous solve (at most one per scene)
     bool pending = false;
+    int p_reps = 1;
     int64_t p_rays = 0;
     int p_cap = 0;
     unsigned p_flags = 0;
     double p_weight = 0;
-    ehss::HostBuf host_words;  // counters, histogram and ledger of the pending solve
+    ehss::HostBuf host_words;  // counters, histograms and ledgers of the pending launch
+    ehss::HostBuf host_table;  // scrambles and ray maps of the pending launch (staging)
     // solve workspace (grow-only)
-    ehss::DevBuf ray_info, ray_contrib, partials, hist, ledger, counters, queue, ray_times;
+    ehss::DevBuf ray_info, ray_contrib, partials, hist, ledger, counters, queue, ray_times, tables;
     int64_t last_rays = 0, last_times = 0;
     uint32_t last_map[32] = {0};  // ray map of the last solve (slot -> ray index)
     int last_power = 0;

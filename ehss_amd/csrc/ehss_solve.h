@@ -35,10 +35,15 @@ struct SolveArgs {
     long long* times;              // optional per-ray completion time stamps (device wall clock)
 };
 
+// Words per replicate in the solve workspace: [0..5] kernel counters, [6] max
+// bounces, [8..15] final Partial.
+constexpr int kCounterWords = 16;
+
 struct Partial {
     double s, c;
     long long escaped, unresolved, pa_hits, ehss_hits, grazing, pad;
 };
+static_assert(sizeof(Partial) == 8 * 8, "Partial occupies words 8..15 of a replicate's counters");
 
 // Wave-coherent ray map.  For n = 2^m rays the 64 rays of wave w are the points of
 // one elementary cell of the scrambled net: the top a_d output bits of every

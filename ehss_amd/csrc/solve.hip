@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <array>
+#include <mutex>
 #include <vector>
 
 #include "ehss_solve.h"
@@ -15,21 +16,38 @@ namespace ehss {
 
 namespace {
 
-template <int kMode, int kPA>
-__global__ void __launch_bounds__(256) k_solve(SceneView S, SolveArgs A, ehss_sobol sob, RayMap map,
-                                               RayState* queue, uint32_t* queue_count) {
-    uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
-    solve_ray<kMode, kPA>(S, A, sob, map, blockIdx.x * blockDim.x + threadIdx.x, ctr, queue, queue_count, 0);
-    flush_counters(ctr, A.counters);
+// Replicate r of a multi-replicate solve: its slices of the record, counter,
+// histogram and ledger arrays (replicates are laid out one after another).
+__device__ __forceinline__ SolveArgs replicate_args(const SolveArgs& A, int32_t r) {
+    SolveArgs B = A;
+    B.info = A.info + (int64_t)r * A.n;
+    B.contrib = A.contrib + (int64_t)r * A.n;
+    B.counters = A.counters + (int64_t)r * kCounterWords;
+    if (A.times) B.times = A.times + (int64_t)r * A.n;
+    return B;
 }
 
-// Persistent drain of the rays the two-phase variant queued: one lane per ray
+// One thread per (ray slot, replicate): blockIdx.y is the replicate, whose
+// scrambled net and ray map are read (scalar loads) from device tables.
+template <int kMode, int kPA>
+__global__ void __launch_bounds__(256) k_solve(SceneView S, SolveArgs A, const ehss_sobol* sobs, const RayMap* maps,
+                                               RayState* queue, uint32_t* queue_count) {
+    const int32_t r = (int32_t)blockIdx.y;
+    const SolveArgs B = replicate_args(A, r);
+    uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
+    solve_ray<kMode, kPA>(S, B, (CSobol)(sobs + r), (CMap)(maps + r), blockIdx.x * blockDim.x + threadIdx.x, ctr,
+                          queue, queue_count, r);
+    flush_counters(ctr, B.counters);
+}
+
+// Persistent drain of the rays the two-phase variant queued (every replicate's
+// rays share one queue; a ray's job is its replicate): one lane per ray
 // (per-lane tree traversal, cooperative tail) or A.group lanes per ray.  With
 // kLds the FP32 filter tree, the leaf boxes, leaf offsets, slot ids and FP32
 // spheres are first copied into LDS, so every node, leaf and sphere fetch of the
 // drain is an LDS access instead of a dependent L2 round trip.
 template <int kLds>
-__global__ void __launch_bounds__(256) k_drain(SceneView S, SolveArgs A, const RayState* queue,
+__global__ void __launch_bounds__(256) k_drain(SceneView S, SolveArgs A, int32_t replicates, const RayState* queue,
                                                const uint32_t* queue_count, uint32_t* queue_head) {
     extern __shared__ __align__(16) unsigned char lds[];
     uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
@@ -57,17 +75,26 @@ __global__ void __launch_bounds__(256) k_drain(SceneView S, SolveArgs A, const R
         L.ids = ids;
     }
     auto scene_of = [&](int32_t) -> const SceneView& { return L; };
-    auto args_of = [&](int32_t) -> const SolveArgs& { return A; };
-    auto account = [&](int32_t, uint64_t b, uint64_t sp, uint64_t r) {
-        ctr[0] += b;
-        ctr[1] += sp;
-        ctr[2] += r;
+    auto args_of = [&](int32_t j) -> SolveArgs { return replicate_args(A, j); };
+    // One replicate: per-lane sums flushed once; several: per-ray atomics into
+    // the ray's replicate.
+    auto account = [&](int32_t j, uint64_t b, uint64_t sp, uint64_t r) {
+        if (replicates == 1) {
+            ctr[0] += b;
+            ctr[1] += sp;
+            ctr[2] += r;
+        } else {
+            unsigned long long* c = A.counters + (int64_t)j * kCounterWords;
+            if (b) atomicAdd(&c[0], (unsigned long long)b);
+            if (sp) atomicAdd(&c[1], (unsigned long long)sp);
+            if (r) atomicAdd(&c[2], (unsigned long long)r);
+        }
     };
     if (A.group > 1)
         drain_groups(queue, len, queue_head, A.group, scene_of, args_of, account);
     else
         drain_queue(queue, len, queue_head, A.coop, scene_of, args_of, account);
-    flush_counters(ctr, A.counters);
+    if (replicates == 1) flush_counters(ctr, A.counters);
 }
 
 // LDS bytes of a scene staged by k_drain<1>.
@@ -81,15 +108,22 @@ __global__ void k_zero(unsigned long long* p, int64_t n) {
         p[i] = 0ull;
 }
 
-__global__ void __launch_bounds__(256) k_reduce(int64_t n, int32_t cap, double w, const uint32_t* info,
-                                                const double* contrib, Partial* partials,
-                                                unsigned long long* hist, unsigned long long* ledger,
-                                                int* max_bounces) {
-    reduce_block(blockIdx.x, n, cap, w, info, contrib, partials, hist, ledger, max_bounces);
+// Reduction block blockIdx.x of replicate blockIdx.y.
+__global__ void __launch_bounds__(256) k_reduce(SolveArgs A, int64_t nblocks, Partial* partials,
+                                                unsigned long long* hist, unsigned long long* ledger) {
+    const int32_t r = (int32_t)blockIdx.y;
+    const SolveArgs B = replicate_args(A, r);
+    const int64_t bins = A.cap + 1;
+    reduce_block(blockIdx.x, A.n, A.cap, A.w, B.info, B.contrib, partials + r * nblocks, hist + r * bins,
+                 ledger + r * bins, reinterpret_cast<int*>(B.counters + 6));
 }
 
-__global__ void __launch_bounds__(256) k_finalize(const Partial* partials, int64_t nblocks, Partial* out) {
-    finalize_block(partials, nblocks, out);
+// Final fixed-order combination of replicate blockIdx.x.
+__global__ void __launch_bounds__(256) k_finalize(const Partial* partials, int64_t nblocks,
+                                                  unsigned long long* counters) {
+    const int32_t r = (int32_t)blockIdx.x;
+    finalize_block(partials + r * nblocks, nblocks,
+                   reinterpret_cast<Partial*>(counters + (int64_t)r * kCounterWords + 8));
 }
 
 __global__ void k_source_rays(int64_t n, double cx, double cy, double cz, double R, ehss_sobol sob, double* u_out,
@@ -247,9 +281,8 @@ namespace {
 
 // Workspace layout in scene->counters: [0..5] kernel counters, [6] max bounces,
 // [8..] final Partial.
-constexpr int kQueueWord = 8 + (int)(sizeof(Partial) / 8);  // queue length, queue head (two u32)
-constexpr int kCounterWords = kQueueWord + 1;
 constexpr int kDefaultVariant = 4;
+constexpr int kMaxReplicates = 4096;
 constexpr size_t kDrainLdsLimit = 144 * 1024;  // of the 160 KB per CU
 
 // Ray index of record slot t (host copy of the device mapping: Gray index from
@@ -273,29 +306,51 @@ uint32_t slot_ray(const uint32_t cols[32], int power, uint32_t t) {
     return g;
 }
 
-// Enqueues one solve on the scene's stream, with the copies of its totals,
-// histogram and ledger into the scene's pinned host buffers; solve_collect
-// waits for them.  One solve per scene may be pending.
-int solve_enqueue(ehss_scene* sc, const ehss_sobol* sob, int power, int cap, int precision, unsigned flags) {
-    EHSS_CHECK(sc && sob, "null scene/sobol");
+// Enqueues `reps` solves of 2^power rays (one scramble each) as one launch on the
+// scene's stream, with the copies of their totals, histograms and ledgers into
+// the scene's pinned host buffer; solve_collect waits for them.  One launch per
+// scene may be pending.
+int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, int cap, int precision,
+                  unsigned flags) {
+    EHSS_CHECK(sc && sobs, "null scene/sobol");
     if (sc->pending) return set_error(EHSS_ERR_BUSY, "a solve is already pending on this scene");
+    EHSS_CHECK(reps >= 1 && reps <= kMaxReplicates, "replicate count %d out of range 1..%d", reps, kMaxReplicates);
     EHSS_CHECK(power >= 0 && power <= 30, "power %d out of range 0..30", power);
+    EHSS_CHECK((int64_t)reps << power <= (1LL << 30), "%d replicates of 2^%d rays exceed 2^30 rays per launch", reps,
+               power);
     EHSS_CHECK(cap >= 1 && cap <= (int)kBounceMask - 1, "cap %d out of range 1..%u", cap, kBounceMask - 1);
     EHSS_CHECK(precision == EHSS_PRECISION_FP64 || precision == EHSS_PRECISION_FP32_FIXUP, "unknown precision %d",
                precision);
     hipStream_t s = sc->stream;
     const int64_t n = 1LL << power;
+    const int64_t total = n * reps;
     const int64_t nblocks = (n + kReduceRays - 1) / kReduceRays;
-    EHSS_TRY(sc->ray_info.reserve(sizeof(uint32_t) * n));
-    EHSS_TRY(sc->ray_contrib.reserve(sizeof(double) * n));
-    EHSS_TRY(sc->partials.reserve(sizeof(Partial) * nblocks));
-    EHSS_TRY(sc->hist.reserve(sizeof(unsigned long long) * (cap + 1)));
-    EHSS_TRY(sc->ledger.reserve(sizeof(unsigned long long) * (cap + 1)));
-    EHSS_TRY(sc->counters.reserve(sizeof(unsigned long long) * kCounterWords));
+    const int64_t bins = cap + 1;
+    const int64_t words = (int64_t)kCounterWords * reps + 1;  // + queue length / head
+    EHSS_TRY(sc->ray_info.reserve(sizeof(uint32_t) * total));
+    EHSS_TRY(sc->ray_contrib.reserve(sizeof(double) * total));
+    EHSS_TRY(sc->partials.reserve(sizeof(Partial) * nblocks * reps));
+    EHSS_TRY(sc->hist.reserve(sizeof(unsigned long long) * bins * reps));
+    EHSS_TRY(sc->ledger.reserve(sizeof(unsigned long long) * bins * reps));
+    EHSS_TRY(sc->counters.reserve(sizeof(unsigned long long) * words));
+    EHSS_TRY(sc->tables.reserve((sizeof(ehss_sobol) + sizeof(RayMap)) * reps));
+    EHSS_TRY(sc->host_table.reserve((sizeof(ehss_sobol) + sizeof(RayMap)) * reps));
     unsigned long long* ctr = sc->counters.as<unsigned long long>();
-    EHSS_TRY(launch_zero(ctr, kCounterWords, s));
-    EHSS_TRY(launch_zero(sc->hist.as<unsigned long long>(), cap + 1, s));
-    EHSS_TRY(launch_zero(sc->ledger.as<unsigned long long>(), cap + 1, s));
+    EHSS_TRY(launch_zero(ctr, words, s));
+    EHSS_TRY(launch_zero(sc->hist.as<unsigned long long>(), bins * reps, s));
+    EHSS_TRY(launch_zero(sc->ledger.as<unsigned long long>(), bins * reps, s));
+
+    // Scrambles and ray maps, staged in pinned memory (not reused while pending).
+    ehss_sobol* h_sob = sc->host_table.as<ehss_sobol>();
+    RayMap* h_map = reinterpret_cast<RayMap*>(h_sob + reps);
+    for (int r = 0; r < reps; ++r) {
+        h_sob[r] = sobs[r];
+        make_ray_map(sobs[r], power, h_map[r].col);
+    }
+    EHSS_HIP(hipMemcpyAsync(sc->tables.p, h_sob, (sizeof(ehss_sobol) + sizeof(RayMap)) * reps,
+                            hipMemcpyHostToDevice, s));
+    const ehss_sobol* d_sob = sc->tables.as<ehss_sobol>();
+    const RayMap* d_map = reinterpret_cast<const RayMap*>(d_sob + reps);
 
     SolveArgs A;
     A.n = n;
@@ -310,34 +365,32 @@ int solve_enqueue(ehss_scene* sc, const ehss_sobol* sob, int power, int cap, int
     A.contrib = sc->ray_contrib.as<double>();
     A.counters = ctr;
     A.coop = coop_from_flags(flags);
-    A.group = group_from_flags(flags, n);
+    A.group = group_from_flags(flags, total);
     A.times = nullptr;
     if (flags & EHSS_FLAG_TIMESTAMPS) {
-        EHSS_TRY(sc->ray_times.reserve(sizeof(long long) * n));
+        EHSS_TRY(sc->ray_times.reserve(sizeof(long long) * total));
         A.times = sc->ray_times.as<long long>();
     }
     EHSS_CHECK(A.coop <= kWave, "tail threshold %d out of range 0..64", A.coop);
     const bool pa = (flags & EHSS_FLAG_PA) != 0;
-    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    const dim3 grid((unsigned)((n + 255) / 256), (unsigned)reps), block(256);
     int variant = (int)((flags >> EHSS_VARIANT_SHIFT) & 0xFu);
     if (precision == EHSS_PRECISION_FP64) variant = 0;
     else if (variant == 0) variant = kDefaultVariant;
     EHSS_CHECK(variant >= 0 && variant <= 4, "unknown kernel variant %d", variant);
     RayState* qbuf = nullptr;
-    uint32_t* qcount = reinterpret_cast<uint32_t*>(ctr + kQueueWord);
+    uint32_t* qcount = reinterpret_cast<uint32_t*>(ctr + (int64_t)kCounterWords * reps);
     if (variant == 4) {
-        EHSS_TRY(sc->queue.reserve(sizeof(RayState) * n));
+        EHSS_TRY(sc->queue.reserve(sizeof(RayState) * total));
         qbuf = sc->queue.as<RayState>();
     }
-    RayMap rmap;
-    make_ray_map(*sob, power, rmap.col);
     EHSS_HIP(hipEventRecord(sc->ev0, s));
-#define EHSS_LAUNCH(M)                                                                          \
-    do {                                                                                        \
-        if (pa)                                                                                 \
-            hipLaunchKernelGGL((k_solve<M, 1>), grid, block, 0, s, sc->view, A, *sob, rmap, qbuf, qcount); \
-        else                                                                                    \
-            hipLaunchKernelGGL((k_solve<M, 0>), grid, block, 0, s, sc->view, A, *sob, rmap, qbuf, qcount); \
+#define EHSS_LAUNCH(M)                                                                                   \
+    do {                                                                                                 \
+        if (pa)                                                                                          \
+            hipLaunchKernelGGL((k_solve<M, 1>), grid, block, 0, s, sc->view, A, d_sob, d_map, qbuf, qcount); \
+        else                                                                                             \
+            hipLaunchKernelGGL((k_solve<M, 0>), grid, block, 0, s, sc->view, A, d_sob, d_map, qbuf, qcount); \
     } while (0)
     switch (variant) {
         case 0: EHSS_LAUNCH(0); break;
@@ -349,39 +402,40 @@ int solve_enqueue(ehss_scene* sc, const ehss_sobol* sob, int power, int cap, int
 #undef EHSS_LAUNCH
     if (variant == 4) {
         const size_t lds = drain_lds_bytes(sc->view);
+        const dim3 dgrid((unsigned)drain_blocks(total));
         if (lds <= kDrainLdsLimit) {
-            static bool attr = false;
-            if (!attr) {
-                EHSS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_drain<1>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDrainLdsLimit));
-                attr = true;
-            }
-            hipLaunchKernelGGL(k_drain<1>, dim3((unsigned)drain_blocks(n)), dim3(256), lds, s, sc->view, A, qbuf,
-                               qcount, qcount + 1);
+            static std::once_flag attr_once;
+            hipError_t attr = hipSuccess;
+            std::call_once(attr_once, [&] {
+                attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_drain<1>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDrainLdsLimit);
+            });
+            EHSS_HIP(attr);
+            hipLaunchKernelGGL(k_drain<1>, dgrid, dim3(256), lds, s, sc->view, A, reps, qbuf, qcount, qcount + 1);
         } else {
-            hipLaunchKernelGGL(k_drain<0>, dim3((unsigned)drain_blocks(n)), dim3(256), 0, s, sc->view, A, qbuf,
-                               qcount, qcount + 1);
+            hipLaunchKernelGGL(k_drain<0>, dgrid, dim3(256), 0, s, sc->view, A, reps, qbuf, qcount, qcount + 1);
         }
     }
     EHSS_HIP(hipGetLastError());
     EHSS_HIP(hipEventRecord(sc->ev1, s));
-    hipLaunchKernelGGL(k_reduce, dim3((unsigned)nblocks), dim3(256), 0, s, n, cap, A.w, A.info, A.contrib,
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)nblocks, (unsigned)reps), dim3(256), 0, s, A, nblocks,
                        sc->partials.as<Partial>(), sc->hist.as<unsigned long long>(),
-                       sc->ledger.as<unsigned long long>(), reinterpret_cast<int*>(ctr + 6));
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, sc->partials.as<Partial>(), nblocks,
-                       reinterpret_cast<Partial*>(ctr + 8));
+                       sc->ledger.as<unsigned long long>());
+    hipLaunchKernelGGL(k_finalize, dim3((unsigned)reps), dim3(256), 0, s, sc->partials.as<Partial>(), nblocks, ctr);
     EHSS_HIP(hipGetLastError());
-    EHSS_TRY(sc->host_words.reserve(sizeof(unsigned long long) * (kCounterWords + 2 * (cap + 1))));
+    const int64_t host_words = (int64_t)kCounterWords * reps + 2 * bins * reps;
+    EHSS_TRY(sc->host_words.reserve(sizeof(unsigned long long) * host_words));
     unsigned long long* hw = sc->host_words.as<unsigned long long>();
-    EHSS_HIP(hipMemcpyAsync(hw, ctr, sizeof(unsigned long long) * kCounterWords, hipMemcpyDeviceToHost, s));
-    EHSS_HIP(hipMemcpyAsync(hw + kCounterWords, sc->hist.p, sizeof(unsigned long long) * (cap + 1),
+    EHSS_HIP(hipMemcpyAsync(hw, ctr, sizeof(unsigned long long) * kCounterWords * reps, hipMemcpyDeviceToHost, s));
+    EHSS_HIP(hipMemcpyAsync(hw + (int64_t)kCounterWords * reps, sc->hist.p, sizeof(unsigned long long) * bins * reps,
                             hipMemcpyDeviceToHost, s));
-    EHSS_HIP(hipMemcpyAsync(hw + kCounterWords + (cap + 1), sc->ledger.p, sizeof(unsigned long long) * (cap + 1),
-                            hipMemcpyDeviceToHost, s));
+    EHSS_HIP(hipMemcpyAsync(hw + (int64_t)kCounterWords * reps + bins * reps, sc->ledger.p,
+                            sizeof(unsigned long long) * bins * reps, hipMemcpyDeviceToHost, s));
     EHSS_HIP(hipEventRecord(sc->ev_done, s));
     sc->pending = true;
-    memcpy(sc->p_map, rmap.col, sizeof sc->p_map);
+    memcpy(sc->p_map, h_map[0].col, sizeof sc->p_map);
     sc->p_power = power;
+    sc->p_reps = reps;
     sc->p_rays = n;
     sc->p_cap = cap;
     sc->p_flags = flags;
@@ -402,8 +456,8 @@ int solve_query(ehss_scene* sc, int* done) {
     return EHSS_OK;
 }
 
-// Waits for the pending solve and fills totals (and, if not NULL, the cap + 1
-// histogram and ledger bins).
+// Waits for the pending launch and fills one ehss_totals per replicate (and, if
+// not NULL, cap + 1 histogram and ledger bins per replicate, replicate-major).
 int solve_collect(ehss_scene* sc, ehss_totals* totals, int64_t* histogram, double* ledger_out) {
     EHSS_CHECK(sc && totals, "null scene/totals");
     if (!sc->pending) return set_error(EHSS_ERR_INVALID, "no solve pending on this scene");
@@ -412,40 +466,44 @@ int solve_collect(ehss_scene* sc, ehss_totals* totals, int64_t* histogram, doubl
     float ms = 0.f;
     EHSS_HIP(hipEventElapsedTime(&ms, sc->ev0, sc->ev1));
     const unsigned long long* hw = sc->host_words.as<unsigned long long>();
-    const int cap = sc->p_cap;
+    const int reps = sc->p_reps;
+    const int64_t bins = sc->p_cap + 1;
     const int64_t n = sc->p_rays;
     const double w = sc->p_weight;
     const bool pa = (sc->p_flags & EHSS_FLAG_PA) != 0;
-    Partial fin;
-    memcpy(&fin, hw + 8, sizeof fin);
-    ehss_totals& T = *totals;
-    memset(&T, 0, sizeof T);
-    T.weight = w;
-    T.omega = fin.s + fin.c;
-    T.omega_err = fin.c;
-    T.rays = n;
-    T.escaped = fin.escaped;
-    T.unresolved = fin.unresolved;
-    T.pa_hits = pa ? fin.pa_hits : 0;
-    T.ehss_hits = fin.ehss_hits;
-    T.grazing_rays = fin.grazing;
-    T.pa = pa ? w * (double)fin.pa_hits : 0.;
-    T.pa_from_ehss = w * (double)fin.ehss_hits;
-    T.residual_mass = w * (double)fin.unresolved;
-    T.tail_bound = 2 * T.residual_mass;
-    T.box_tests = (int64_t)hw[0];
-    T.sphere_tests = (int64_t)hw[1];
-    T.fp64_retests = (int64_t)(hw[2] + hw[5]);
-    T.pa_box_tests = (int64_t)hw[3];
-    T.pa_sphere_tests = (int64_t)hw[4];
-    T.max_bounces = (int64_t)(int)hw[6];
-    T.kernel_ms = ms;
-    const unsigned long long* h_hist = hw + kCounterWords;
-    const unsigned long long* h_led = h_hist + (cap + 1);
-    if (histogram)
-        for (int k = 0; k <= cap; ++k) histogram[k] = (int64_t)h_hist[k];
-    if (ledger_out)
-        for (int k = 0; k <= cap; ++k) ledger_out[k] = (double)h_led[k] / kLedgerScale * w;
+    for (int r = 0; r < reps; ++r) {
+        const unsigned long long* c = hw + (int64_t)kCounterWords * r;
+        Partial fin;
+        memcpy(&fin, c + 8, sizeof fin);
+        ehss_totals& T = totals[r];
+        memset(&T, 0, sizeof T);
+        T.weight = w;
+        T.omega = fin.s + fin.c;
+        T.omega_err = fin.c;
+        T.rays = n;
+        T.escaped = fin.escaped;
+        T.unresolved = fin.unresolved;
+        T.pa_hits = pa ? fin.pa_hits : 0;
+        T.ehss_hits = fin.ehss_hits;
+        T.grazing_rays = fin.grazing;
+        T.pa = pa ? w * (double)fin.pa_hits : 0.;
+        T.pa_from_ehss = w * (double)fin.ehss_hits;
+        T.residual_mass = w * (double)fin.unresolved;
+        T.tail_bound = 2 * T.residual_mass;
+        T.box_tests = (int64_t)c[0];
+        T.sphere_tests = (int64_t)c[1];
+        T.fp64_retests = (int64_t)(c[2] + c[5]);
+        T.pa_box_tests = (int64_t)c[3];
+        T.pa_sphere_tests = (int64_t)c[4];
+        T.max_bounces = (int64_t)(int)c[6];
+        T.kernel_ms = ms;  // the whole launch
+        const unsigned long long* h_hist = hw + (int64_t)kCounterWords * reps + bins * r;
+        const unsigned long long* h_led = hw + (int64_t)kCounterWords * reps + bins * reps + bins * r;
+        if (histogram)
+            for (int64_t k = 0; k < bins; ++k) histogram[bins * r + k] = (int64_t)h_hist[k];
+        if (ledger_out)
+            for (int64_t k = 0; k < bins; ++k) ledger_out[bins * r + k] = (double)h_led[k] / kLedgerScale * w;
+    }
     sc->last_rays = (sc->p_flags & EHSS_FLAG_KEEP_RAYS) ? n : 0;
     sc->last_times = (sc->p_flags & EHSS_FLAG_TIMESTAMPS) ? n : 0;
     sc->last_weight = w;
@@ -468,12 +526,24 @@ int ehss_ray_map(const ehss_sobol* sobol, int power, uint32_t* cols) {
 int ehss_solve(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, int precision, unsigned flags,
                ehss_totals* totals, int64_t* histogram, double* ledger) {
     EHSS_CHECK(totals, "null totals");
-    EHSS_TRY(solve_enqueue(scene, sobol, power, cap, precision, flags));
+    EHSS_TRY(solve_enqueue(scene, 1, sobol, power, cap, precision, flags));
     return solve_collect(scene, totals, histogram, ledger);
 }
 
+int ehss_solve_many(ehss_scene* scene, int replicates, const ehss_sobol* sobols, int power, int cap, int precision,
+                    unsigned flags, ehss_totals* totals, int64_t* histograms, double* ledgers) {
+    EHSS_CHECK(totals, "null totals");
+    EHSS_TRY(solve_enqueue(scene, replicates, sobols, power, cap, precision, flags));
+    return solve_collect(scene, totals, histograms, ledgers);
+}
+
 int ehss_solve_async(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, int precision, unsigned flags) {
-    return solve_enqueue(scene, sobol, power, cap, precision, flags);
+    return solve_enqueue(scene, 1, sobol, power, cap, precision, flags);
+}
+
+int ehss_solve_many_async(ehss_scene* scene, int replicates, const ehss_sobol* sobols, int power, int cap,
+                          int precision, unsigned flags) {
+    return solve_enqueue(scene, replicates, sobols, power, cap, precision, flags);
 }
 
 int ehss_solve_query(ehss_scene* scene, int* done) { return solve_query(scene, done); }

@@ -243,31 +243,66 @@ class GPUScene:
             out["ledger"] = ledger
         return out
 
-    def solve_async(self, sobol: L.Sobol, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
+    def solve_many(self, sobols, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
+                   histogram: bool = False, variant: int = 0, coop: int | None = None, group: int | None = None,
+                   keep_rays: bool = False) -> list[dict]:
+        """len(sobols) independent solves in one launch; one result dict per
+        scramble, each equal to solve() with that scramble."""
+        reps = len(sobols)
+        table = (L.Sobol * reps)(*sobols)
+        totals = (L.Totals * reps)()
+        hist = np.zeros(reps * (cap + 1), np.int64) if histogram else None
+        ledger = np.zeros(reps * (cap + 1), np.float64) if histogram else None
+        flags = (L.FLAG_PA if pa else 0) | (L.FLAG_KEEP_RAYS if keep_rays else 0) | \
+            (int(variant) << L.VARIANT_SHIFT) | coop_flags(coop, group)
+        L.call("ehss_solve_many", self.handle, reps, table, int(power), int(cap), precision_code(precision), flags,
+               totals, L.ptr(hist), L.ptr(ledger))
+        return self._split(totals, reps, cap, hist, ledger)
+
+    @staticmethod
+    def _split(totals, reps, cap, hist, ledger) -> list[dict]:
+        out = []
+        for r in range(reps):
+            d = totals[r].as_dict()
+            if hist is not None:
+                d["histogram"] = hist[r * (cap + 1):(r + 1) * (cap + 1)].copy()
+                d["ledger"] = ledger[r * (cap + 1):(r + 1) * (cap + 1)].copy()
+            out.append(d)
+        return out
+
+    def solve_async(self, sobol, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
                     variant: int = 0, coop: int | None = None, group: int | None = None) -> None:
-        """Enqueue solve() on the scene's stream and return at once; collect
-        with solve_wait().  Solves of different scenes overlap on the device."""
+        """Enqueue solve() -- or, given a list of scrambles, solve_many() -- on
+        the scene's stream and return at once; collect with solve_wait().
+        Launches on different scenes overlap on the device."""
         flags = (L.FLAG_PA if pa else 0) | (int(variant) << L.VARIANT_SHIFT) | coop_flags(coop, group)
-        L.call("ehss_solve_async", self.handle, ct.byref(sobol), int(power), int(cap), precision_code(precision),
-               flags)
-        self._pending_cap = int(cap)
+        if isinstance(sobol, (list, tuple)):
+            reps = len(sobol)
+            table = (L.Sobol * reps)(*sobol)
+            L.call("ehss_solve_many_async", self.handle, reps, table, int(power), int(cap),
+                   precision_code(precision), flags)
+        else:
+            reps = 0
+            L.call("ehss_solve_async", self.handle, ct.byref(sobol), int(power), int(cap),
+                   precision_code(precision), flags)
+        self._pending = (int(cap), reps)
 
     def solve_done(self) -> bool:
         flag = ct.c_int(0)
         L.call("ehss_solve_query", self.handle, ct.byref(flag))
         return bool(flag.value)
 
-    def solve_wait(self, histogram: bool = False) -> dict:
-        cap = getattr(self, "_pending_cap", 0)
-        totals = L.Totals()
-        hist = np.zeros(cap + 1, np.int64) if histogram else None
-        ledger = np.zeros(cap + 1, np.float64) if histogram else None
-        L.call("ehss_solve_wait", self.handle, ct.byref(totals), L.ptr(hist), L.ptr(ledger))
-        out = totals.as_dict()
-        if histogram:
-            out["histogram"] = hist
-            out["ledger"] = ledger
-        return out
+    def solve_wait(self, histogram: bool = False):
+        """Result of the pending launch: a dict for solve_async(sobol), a list of
+        dicts for solve_async([sobols])."""
+        cap, reps = getattr(self, "_pending", (0, 0))
+        n = max(reps, 1)
+        totals = (L.Totals * n)()
+        hist = np.zeros(n * (cap + 1), np.int64) if histogram else None
+        ledger = np.zeros(n * (cap + 1), np.float64) if histogram else None
+        L.call("ehss_solve_wait", self.handle, totals, L.ptr(hist), L.ptr(ledger))
+        out = self._split(totals, n, cap, hist, ledger)
+        return out if reps else out[0]
 
     def last_rays(self, n: int) -> dict:
         flags = np.empty(n, np.uint32)

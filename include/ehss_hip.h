@@ -181,12 +181,25 @@ int ehss_ray_map(const ehss_sobol* sobol, int power, uint32_t* cols);
 int ehss_solve(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, int precision, unsigned flags,
                ehss_totals* totals, int64_t* histogram, double* ledger);
 
-/* The same solve split for overlapped use (no reference counterpart; used by the
- * request loop): ehss_solve_async enqueues it on the scene's stream and returns;
- * ehss_solve_query sets *done without blocking; ehss_solve_wait blocks and
- * returns what ehss_solve returns.  One solve may be pending per scene
- * (EHSS_ERR_BUSY otherwise); solves of different scenes run concurrently. */
+/* `replicates` independent solves of the scene in one launch (one scramble each,
+ * the replicates of ccs_cli.py:186-227): totals gets one entry per replicate,
+ * histograms / ledgers (may be NULL) cap + 1 bins per replicate, replicate-major.
+ * Each replicate's totals are bit-identical to ehss_solve with its scramble.
+ * kernel_ms is the device time of the whole launch.  1 <= replicates <= 4096
+ * and replicates x 2^power <= 2^30.  Per-ray records (EHSS_FLAG_KEEP_RAYS) are
+ * those of the first replicate. */
+int ehss_solve_many(ehss_scene* scene, int replicates, const ehss_sobol* sobols, int power, int cap, int precision,
+                    unsigned flags, ehss_totals* totals, int64_t* histograms, double* ledgers);
+
+/* The same solves split for overlapped use (no reference counterpart; used by the
+ * request loop): the _async calls enqueue on the scene's stream and return;
+ * ehss_solve_query sets *done without blocking; ehss_solve_wait blocks and fills
+ * what ehss_solve / ehss_solve_many fill (one totals entry per replicate of the
+ * pending launch).  One launch may be pending per scene (EHSS_ERR_BUSY
+ * otherwise); launches on different scenes run concurrently. */
 int ehss_solve_async(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, int precision, unsigned flags);
+int ehss_solve_many_async(ehss_scene* scene, int replicates, const ehss_sobol* sobols, int power, int cap,
+                          int precision, unsigned flags);
 int ehss_solve_query(ehss_scene* scene, int* done);
 int ehss_solve_wait(ehss_scene* scene, ehss_totals* totals, int64_t* histogram, double* ledger);
 

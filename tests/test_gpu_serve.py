@@ -74,3 +74,42 @@ def test_results_are_the_same_alone_and_in_the_mix():
         (res,) = alone.run([serve.Request(req.rid, req.centers, req.radii, 0.0, req.seed)])
         assert res.omegas == mixed[req.rid].omegas, req.rid
         assert res.ccs == mixed[req.rid].ccs
+
+
+@pytest.mark.parametrize("precision,variant", [("fp64", 0), ("fp32-fixup", 2), ("fp32-fixup", 3),
+                                               ("fp32-fixup", 4)])
+def test_replicates_in_one_launch_equal_separate_solves(precision, variant):
+    for name, (c, r) in (("globule650", scenes.globule(650, 1)), ("c60", scenes.SCENES["c60"]()),
+                         ("single", (np.zeros((1, 3)), np.array([3.1])))):
+        with GPUScene(c, r) as s:
+            for power, cap in ((12, 64), (5, 16)):
+                sobols = [scramble(300 + k) for k in range(5)]
+                say("replicates", name, precision, variant, power)
+                many = s.solve_many(sobols, power, cap, precision=precision, histogram=True, variant=variant,
+                                    keep_rays=True)
+                first = s.last_rays(1 << power)
+                for k, sob in enumerate(sobols):
+                    one = s.solve(sob, power, cap, precision=precision, histogram=True, variant=variant,
+                                  keep_rays=True)
+                    for key in KEYS:
+                        assert many[k][key] == one[key], (name, k, key)
+                    np.testing.assert_array_equal(many[k]["histogram"], one["histogram"])
+                    np.testing.assert_array_equal(many[k]["ledger"], one["ledger"])
+                    if k == 0:
+                        rays = s.last_rays(1 << power)
+                        np.testing.assert_array_equal(first["contributions"], rays["contributions"])
+                        np.testing.assert_array_equal(first["bounces"], rays["bounces"])
+
+
+def test_replicate_launch_async_and_limits():
+    c, r = scenes.SCENES["c60"]()
+    with GPUScene(c, r) as s:
+        sobols = [scramble(k) for k in range(7)]
+        s.solve_async(sobols, 11, 64)
+        got = s.solve_wait()
+        assert len(got) == 7
+        assert [g["omega"] for g in got] == [s.solve(x, 11, 64)["omega"] for x in sobols]
+        with pytest.raises(L.EHSSError, match="replicate count"):
+            s.solve_many([], 10, 64)
+        with pytest.raises(L.EHSSError, match="2\\^30"):
+            s.solve_many(sobols[:4], 29, 64)
