@@ -82,10 +82,11 @@ def ccs_rate_sequential(scene, sobols, power, precision):
     return len(sobols) / (time.perf_counter() - t)
 
 
-def ccs_rate_overlapped(scenes, sobols, power, precision):
-    """CCS estimates with one solve in flight per scene copy (each on its own
-    stream), refilled as they finish."""
-    it = iter(sobols)
+def ccs_rate_overlapped(scenes, sobols, power, precision, per_launch=1):
+    """CCS estimates with one launch of `per_launch` replicates in flight per
+    scene copy (each on its own stream), refilled as they finish."""
+    chunks = [sobols[k:k + per_launch] for k in range(0, len(sobols), per_launch)]
+    it = iter(chunks)
     live = []
     count = 0
     t = time.perf_counter()
@@ -98,14 +99,22 @@ def ccs_rate_overlapped(scenes, sobols, power, precision):
     while live:
         for s in list(live):
             if s.solve_done():
-                s.solve_wait()
-                count += 1
+                count += len(s.solve_wait())
                 nxt = next(it, None)
                 if nxt is None:
                     live.remove(s)
                 else:
                     s.solve_async(nxt, power, CAP, precision=precision, pa=False)
         time.sleep(2e-5)
+    return count / (time.perf_counter() - t)
+
+
+def ccs_rate_replicated(scene, sobols, power, precision, per_launch):
+    """CCS estimates as replicates of one launch (ehss_solve_many)."""
+    t = time.perf_counter()
+    count = 0
+    for k in range(0, len(sobols), per_launch):
+        count += len(scene.solve_many(sobols[k:k + per_launch], power, CAP, precision=precision, pa=False))
     return count / (time.perf_counter() - t)
 
 
@@ -230,12 +239,14 @@ def main():
         plain_rate = 20 * n / (time.perf_counter() - t1)
         say("CCS at 1% RMSE")
         power_ccs, rel = rays_per_ccs(scene, scramble, replicate_seed)
-        ccs_sobols = [scramble(replicate_seed(7, k)) for k in range(400)]
-        seq_fast = ccs_rate_sequential(scene, ccs_sobols, power_ccs, "fp32-fixup")
+        ccs_sobols = [scramble(replicate_seed(7, k)) for k in range(4096)]
+        seq_fast = ccs_rate_sequential(scene, ccs_sobols[:400], power_ccs, "fp32-fixup")
         seq_plain = ccs_rate_sequential(scene, ccs_sobols[:100], power_ccs, "fp64")
-        copies = [GPUScene(centers, radii) for _ in range(8)]
-        ovl_fast = ccs_rate_overlapped(copies, ccs_sobols, power_ccs, "fp32-fixup")
-        ovl_plain = ccs_rate_overlapped(copies, ccs_sobols[:200], power_ccs, "fp64")
+        rep_fast = ccs_rate_replicated(scene, ccs_sobols, power_ccs, "fp32-fixup", 512)
+        rep_plain = ccs_rate_replicated(scene, ccs_sobols[:1024], power_ccs, "fp64", 512)
+        copies = [GPUScene(centers, radii) for _ in range(4)]
+        ovl_fast = ccs_rate_overlapped(copies, ccs_sobols, power_ccs, "fp32-fixup", 256)
+        ovl_plain = ccs_rate_overlapped(copies, ccs_sobols[:1024], power_ccs, "fp64", 256)
         for c in copies:
             c.close()
         extras["globule650"] = dict(
@@ -243,11 +254,15 @@ def main():
             omega_A2=res["omega"], pa_A2=res["pa"], fp64_retests_per_ray=res["fp64_retests"] / n,
             grazing_rays=res["grazing_rays"], unresolved_rays=res["unresolved"], tail_bound_A2=res["tail_bound"],
             omega_fp32_fixup_minus_fp64=scene.solve(sobols[0], POWER, CAP, pa=True)["omega"] - plain["omega"])
+        best_fast = max(rep_fast, ovl_fast)
         extras["ccs_1pct"] = dict(
             rays_per_ccs=1 << power_ccs, rel_sd_at_that_count=rel,
-            ccs_per_s_fp32_fixup_single_stream=seq_fast, ccs_per_s_plain_fp64_single_stream=seq_plain,
-            ccs_per_s_fp32_fixup_overlapped_8=ovl_fast, ccs_per_s_plain_fp64_overlapped_8=ovl_plain,
-            served_fp32_fixup_vs_plain_fp64=ovl_fast / seq_plain)
+            ccs_per_s_plain_fp64_single_solves=seq_plain, ccs_per_s_fp32_fixup_single_solves=seq_fast,
+            ccs_per_s_plain_fp64_512_per_launch=rep_plain, ccs_per_s_fp32_fixup_512_per_launch=rep_fast,
+            ccs_per_s_plain_fp64_4_streams_x_256=ovl_plain, ccs_per_s_fp32_fixup_4_streams_x_256=ovl_fast,
+            batched_fp32_fixup_vs_plain_fp64_kernel=best_fast / seq_plain,
+            batched_fp32_fixup_vs_batched_plain_fp64=best_fast / max(rep_plain, ovl_plain),
+            rays_per_s_at_ccs_size=best_fast * (1 << power_ccs))
         scene.close()
 
         say("ellipsoid2000, 2^22 rays, cap 128")

@@ -13,7 +13,10 @@ Each request is a structure (centres, radii).  The loop
 * chooses the ray count per request adaptively: independent scrambled-Sobol
   replicates of 2^power rays (seeds replicate_seed(seed, k), the reference's
   per-replicate rule) until the relative standard error of the mean CCS is at
-  most target_rel (and at least min_reps replicates), capped at max_reps;
+  most target_rel (and at least min_reps replicates), capped at max_reps.  The
+  first launch of a request runs min_reps replicates together; each later one
+  runs as many as the spread so far predicts are still needed, so a request
+  takes one or two launches rather than one per replicate;
 * records arrival, admission and completion times, from which p50/p99 latency
   and CCS/s follow.
 
@@ -36,17 +39,22 @@ import numpy as np
 # spheres 32, leaf boxes <= 32.
 SCENE_BYTES_PER_ATOM = 24 + 8 + 32 + 4 + 4 + 4 + 128 + 128 + 4 + 64 + 16 + 32 + 32
 SCENE_BYTES_FIXED = 64 * 2 + 4 * 8 * 64 + 4096  # PA root slack, frontier, small buffers
-# Solve workspace per ray: record word 4, contribution 8, queue entry 112.
+# Solve workspace per ray: record word 4, contribution 8, queue entry 112; per
+# replicate: scramble + ray map 660, counters 128, histogram + ledger bins 16.
 SOLVE_BYTES_PER_RAY = 4 + 8 + 112
+SOLVE_BYTES_PER_REPLICATE = 660 + 128
 SOLVE_BYTES_FIXED = 64 * 1024
 
 
-def footprint_bytes(n_atoms: int, power: int, cap: int) -> int:
-    """Upper bound on the device memory one admitted request holds."""
+def footprint_bytes(n_atoms: int, power: int, cap: int, replicates: int = 1) -> int:
+    """Upper bound on the device memory a scene holds with launches of up to
+    `replicates` solves of 2^power rays."""
     rays = 1 << int(power)
-    reduce_blocks = rays // 1024 + 1
-    return (SCENE_BYTES_PER_ATOM * int(n_atoms) + SCENE_BYTES_FIXED + SOLVE_BYTES_PER_RAY * rays
-            + 64 * reduce_blocks + 16 * (int(cap) + 1) + SOLVE_BYTES_FIXED)
+    reduce_blocks = rays // 2048 + 1
+    reps = int(replicates)
+    return (SCENE_BYTES_PER_ATOM * int(n_atoms) + SCENE_BYTES_FIXED
+            + reps * (SOLVE_BYTES_PER_RAY * rays + 64 * reduce_blocks + 16 * (int(cap) + 1)
+                      + SOLVE_BYTES_PER_REPLICATE) + SOLVE_BYTES_FIXED)
 
 
 @dataclass
@@ -68,6 +76,7 @@ class Result:
     pa: list = field(default_factory=list)
     unresolved: int = 0
     rays: int = 0
+    launches: int = 0
     arrival: float = 0.0
     admitted: float = math.nan
     finished: float = math.nan
@@ -121,12 +130,26 @@ class Server:
         self.peak_inflight = 0
 
     def footprint(self, req: Request) -> int:
-        return footprint_bytes(len(req.radii), self.power, self.cap)
+        return footprint_bytes(len(req.radii), self.power, self.cap, self.max_reps)
+
+    def _next_count(self, res: Result) -> int:
+        """Replicates of the next launch: min_reps first, then the predicted
+        remainder ((sd / (target * mean))^2 - k, 20 % margin), at least one."""
+        k = len(res.omegas)
+        if k == 0:
+            return self.min_reps
+        v = np.asarray(res.omegas, dtype=np.float64)
+        need = k + 1
+        if k >= 2 and v.mean() != 0:
+            need = math.ceil(1.2 * (v.std(ddof=1) / (self.target_rel * abs(v.mean()))) ** 2)
+        return int(max(1, min(need - k, self.max_reps - k)))
 
     def _launch(self, live: _Live) -> None:
         k = len(live.res.omegas)
-        live.scene.solve_async(self.scramble(self.replicate_seed(live.req.seed, k)), self.power, self.cap,
-                               precision=self.precision, pa=self.pa)
+        count = self._next_count(live.res)
+        sobols = [self.scramble(self.replicate_seed(live.req.seed, k + j)) for j in range(count)]
+        live.scene.solve_async(sobols, self.power, self.cap, precision=self.precision, pa=self.pa)
+        live.res.launches += 1
 
     def _converged(self, res: Result) -> bool:
         k = len(res.omegas)
@@ -179,12 +202,12 @@ class Server:
                 if not item.scene.solve_done():
                     continue
                 progressed = True
-                out = item.scene.solve_wait()
                 res = item.res
-                res.omegas.append(out["omega"])
-                res.pa.append(out["pa"])
-                res.unresolved += int(out["unresolved"])
-                res.rays += int(out["rays"])
+                for out in item.scene.solve_wait():
+                    res.omegas.append(out["omega"])
+                    res.pa.append(out["pa"])
+                    res.unresolved += int(out["unresolved"])
+                    res.rays += int(out["rays"])
                 if self._converged(res):
                     res.ccs = float(np.mean(res.omegas))
                     res.rel_se = _rel_se(res.omegas)

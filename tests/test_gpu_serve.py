@@ -47,6 +47,8 @@ def test_footprint_bounds_device_memory():
             for power, cap in ((10, 64), (14, 128)):
                 s.solve(scramble(1), power, cap)
                 assert s.device_bytes() <= serve.footprint_bytes(n, power, cap), (n, power, cap)
+            s.solve_many([scramble(k) for k in range(24)], 12, 64)
+            assert s.device_bytes() <= serve.footprint_bytes(n, 14, 128, 24)
 
 
 def _bases():

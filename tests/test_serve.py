@@ -32,10 +32,11 @@ class FakeScene:
         self.busy_until, self.seed, self.closed = None, None, False
         FakeScene.live += 1
 
-    def solve_async(self, sobol, power, cap, precision="fp32-fixup", pa=True):
-        assert self.busy_until is None, "one pending solve per scene"
-        self.seed, self.rays = sobol, 1 << power
-        self.busy_until = self.clock() + 1e-6 * self.n
+    def solve_async(self, sobols, power, cap, precision="fp32-fixup", pa=True):
+        assert self.busy_until is None, "one pending launch per scene"
+        assert isinstance(sobols, list) and sobols
+        self.seeds, self.rays = list(sobols), 1 << power
+        self.busy_until = self.clock() + 1e-6 * self.n * (1 + 0.1 * len(sobols))
 
     def solve_done(self):
         return self.busy_until is not None and self.clock() >= self.busy_until
@@ -43,8 +44,11 @@ class FakeScene:
     def solve_wait(self):
         assert self.solve_done()
         self.busy_until = None
-        z = np.random.default_rng(self.seed).normal()
-        return dict(omega=self.base * (1 + 0.03 * z), pa=self.base, unresolved=0, rays=self.rays)
+        out = []
+        for seed in self.seeds:
+            z = np.random.default_rng(seed).normal()
+            out.append(dict(omega=self.base * (1 + 0.03 * z), pa=self.base, unresolved=0, rays=self.rays))
+        return out
 
     def close(self):
         assert not self.closed
@@ -77,6 +81,7 @@ def test_every_request_served_to_target_and_results_match_alone():
         assert r.error is None
         assert r.rel_se <= 0.01 and len(r.omegas) >= 4
         assert r.finished >= r.admitted >= r.arrival
+        assert r.launches <= 4  # replicates are launched together, not one by one
     for req in reqs[:6]:
         c = Clock()
         alone = make_server(c, max_inflight=8, power=10, target_rel=0.01, min_reps=4, max_reps=200)
@@ -86,7 +91,7 @@ def test_every_request_served_to_target_and_results_match_alone():
 
 def test_admission_respects_the_device_budget():
     reqs = stream(count=30, rate=1e6)  # all arrive at once
-    fps = sorted(serve.footprint_bytes(len(r.radii), 10, 64) for r in reqs)
+    fps = sorted(serve.footprint_bytes(len(r.radii), 10, 64, 64) for r in reqs)
     budget = fps[-1] + fps[-2]  # room for the two largest at most
     clock = Clock()
     srv = make_server(clock, budget=budget, max_inflight=16, power=10)
@@ -100,7 +105,7 @@ def test_oversized_request_is_rejected_and_the_rest_served():
     reqs = stream(count=10)
     big = serve.Request(99, np.zeros((50000, 3)), np.ones(50000), arrival=0.0, seed=1)
     clock = Clock()
-    budget = max(serve.footprint_bytes(len(r.radii), 10, 64) for r in reqs) * 2
+    budget = max(serve.footprint_bytes(len(r.radii), 10, 64, 64) for r in reqs) * 2
     srv = make_server(clock, budget=budget, power=10)
     res = {r.rid: r for r in srv.run(reqs + [big], clock=clock, sleep=clock.sleep)}
     assert res[99].error and "budget" in res[99].error
