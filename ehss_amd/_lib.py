@@ -103,6 +103,7 @@ _SIGS = {
     "ehss_scene_last_rays": (_INT, [_P, _I64, _P, _P, _P]),
     "ehss_scene_last_ray_times": (_INT, [_P, _I64, _P, _P]),
     "ehss_ray_map": (_INT, [ct.POINTER(Sobol), _INT, _P]),
+    "ehss_cross_map": (_INT, [_P, _INT, _INT, _P]),
     "ehss_source_rays": (_INT, [_P, ct.POINTER(Sobol), _INT, _P, _P, _P]),
     "ehss_batch_create": (_INT, [_INT, _P, _P, _P, _P, _INT, ct.c_uint, _INT, ct.POINTER(_P)]),
     "ehss_batch_set_sobol": (_INT, [_P, _INT, ct.POINTER(Sobol)]),

@@ -90,6 +90,50 @@ __device__ __forceinline__ uint32_t sob_sv(const ehss_sobol& s, int d, int j) { 
 __device__ __forceinline__ uint32_t sob_shift(const ehss_sobol& s, int d) { return s.shift[d]; }
 __device__ __forceinline__ uint32_t sob_sv(CSobol s, int d, int j) { return s[d * 32 + j]; }
 __device__ __forceinline__ uint32_t sob_shift(CSobol s, int d) { return s[128 + d]; }
+// Per-lane scrambles (vector loads).
+__device__ __forceinline__ uint32_t sob_sv(const ehss_sobol* s, int d, int j) { return s->sv[d][j]; }
+__device__ __forceinline__ uint32_t sob_shift(const ehss_sobol* s, int d) { return s->shift[d]; }
+
+// Cross-replicate schedule of a multi-replicate launch.  With the spatial rows of
+// a partition shared by every replicate (their rank does not depend on the
+// scramble), the point of replicate r whose partition cell is c has Gray index
+// g = T2_r^-1 (c ^ cell_shift_r); its record slot is the slot the replicate's own
+// wave-coherent map gives it, t = T1_r g.  A wave takes one cell c for 64
+// replicates, so its rays share a cell 64 times smaller than a wave of one
+// replicate's net would.
+struct CrossMap {
+    uint32_t inv2[32];   // columns of T2^-1
+    uint32_t rows1[32];  // rows of T1 (slot of Gray index g: bit k = parity(rows1[k] & g))
+    uint32_t cell_shift;
+    uint32_t pad[3];
+};
+
+// The same schedule as linear maps of the cell index c, in tables laid out
+// replicate-fastest so that a wave (64 replicates) reads each entry with one
+// coalesced load: the 30-bit Sobol words of the point are
+// q_d = off[d][r] ^ XOR_{bit k of c} col[k][d][r], and the slot bits (before the
+// wave/lane rotation) are u = uoff[r] ^ XOR_{bit k of c} ucol[k][r].
+struct CrossTables {
+    const uint32_t* col;   // [m][4][rp]
+    const uint32_t* off;   // [4][rp]
+    const uint32_t* ucol;  // [m][rp]
+    const uint32_t* uoff;  // [rp]
+    int32_t rp;            // replicates padded to whole waves
+    int32_t pad;
+};
+
+// A precomputed point (30-bit words per dimension), used in place of a scramble.
+struct PointQ {
+    uint32_t q[4];
+};
+
+__device__ __forceinline__ void sobol_point_gray(const PointQ& p, uint32_t, double u[4]) {
+    const double scale = 9.313225746154785e-10;  // 2^-30
+    u[0] = (double)p.q[0] * scale;
+    u[1] = (double)p.q[1] * scale;
+    u[2] = (double)p.q[2] * scale;
+    u[3] = (double)p.q[3] * scale;
+}
 
 // Point with Gray-code index g: q = shift ^ XOR_{bit j of g} sv[j].
 template <class Sob>
@@ -322,15 +366,15 @@ constexpr int kPacketQueries = 2;
 //     and finished by the persistent per-lane drain kernel.
 // Variants 1-4 return bit-identical per-ray results (the filter only skips spheres
 // whose exact FP64 outcome is already decided).
-template <int kMode, int kPA, class Sob, class Map>
-__device__ __forceinline__ void solve_ray(const SceneView& S, const SolveArgs& A, Sob sob, Map map, uint32_t t,
-                                          uint64_t* ctr, RayState* queue = nullptr, uint32_t* queue_count = nullptr,
-                                          int32_t job = 0) {
-    const bool valid = t < (uint64_t)A.n;
+// The ray with Gray index g of the net, recorded at slot t (valid: the lane
+// has a ray; every lane of the wave must call this).
+template <int kMode, int kPA, class Sob>
+__device__ __forceinline__ void solve_ray_at(const SceneView& S, const SolveArgs& A, Sob sob, bool valid, uint32_t g,
+                                             uint32_t t, uint64_t* ctr, RayState* queue, uint32_t* queue_count,
+                                             int32_t job) {
     uint64_t boxes = 0, spheres = 0, retests = 0, pboxes = 0, ptests = 0, pretests = 0;
     RayCtx R;
-    const uint32_t g = valid ? ray_gray(map, t, A.power) : 0u;
-    R.i = t;  // record slot: thread order, so a wave's record writes are contiguous
+    R.i = t;
     R.o = V3{0., 0., 0.};
     R.d = V3{0., 0., 1.};
     if (valid) {
@@ -437,6 +481,17 @@ __device__ __forceinline__ void solve_ray(const SceneView& S, const SolveArgs& A
     ctr[3] += pboxes;
     ctr[4] += ptests;
     ctr[5] += pretests;
+}
+
+// Thread t of a solve traces the ray the wave-coherent map assigns to it and
+// records it at slot t, so a wave's record writes are contiguous.
+template <int kMode, int kPA, class Sob, class Map>
+__device__ __forceinline__ void solve_ray(const SceneView& S, const SolveArgs& A, Sob sob, Map map, uint32_t t,
+                                          uint64_t* ctr, RayState* queue = nullptr, uint32_t* queue_count = nullptr,
+                                          int32_t job = 0) {
+    const bool valid = t < (uint64_t)A.n;
+    const uint32_t g = valid ? ray_gray(map, t, A.power) : 0u;
+    solve_ray_at<kMode, kPA>(S, A, sob, valid, g, t, ctr, queue, queue_count, job);
 }
 
 // Persistent per-lane drain of queued rays.  Idle lanes refill from the queue

@@ -40,6 +40,44 @@ __global__ void __launch_bounds__(256) k_solve(SceneView S, SolveArgs A, const e
     flush_counters(ctr, B.counters);
 }
 
+// Cross-replicate schedule (see CrossMap / CrossTables): wave w takes the
+// partition cells c = kCrossCells (w / groups) + j, j < kCrossCells, for
+// replicates 64 (w % groups) + lane.  Each ray is recorded at the slot its
+// replicate's own map gives it, so every replicate's records, and therefore its
+// totals, are those of a separate solve.  Work counters go to the ray's
+// replicate, once per lane.
+constexpr int kCrossCells = 1;
+
+template <int kMode, int kPA>
+__global__ void __launch_bounds__(256) k_solve_x(SceneView S, SolveArgs A, CrossTables X, int32_t reps,
+                                                 int32_t groups, RayState* queue, uint32_t* queue_count) {
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const uint32_t c0 = (uint32_t)(wave / groups) * kCrossCells;
+    const int32_t r = (int32_t)(wave % groups) * 64 + lane;
+    const int m = A.power;
+    const int32_t rr = r < reps ? r : 0;
+    const SolveArgs B = replicate_args(A, rr);
+    uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
+    for (int j = 0; j < kCrossCells; ++j) {
+        const uint32_t c = __builtin_amdgcn_readfirstlane(c0 + (uint32_t)j);
+        const bool valid = r < reps && (int64_t)c < A.n;
+        PointQ p;
+        uint32_t u = X.uoff[r];
+        for (int d = 0; d < 4; ++d) p.q[d] = X.off[d * X.rp + r];
+        for (int k = 0; k < m; ++k) {
+            if (!((c >> k) & 1u)) continue;  // wave-uniform
+            u ^= X.ucol[(int64_t)k * X.rp + r];
+            for (int d = 0; d < 4; ++d) p.q[d] ^= X.col[((int64_t)k * 4 + d) * X.rp + r];
+        }
+        const uint32_t t = m >= 6 ? (((u & ((1u << (m - 6)) - 1u)) << 6) | (u >> (m - 6))) : u;
+        solve_ray_at<kMode, kPA>(S, B, p, valid, 0u, t, ctr, queue, queue_count, rr);
+    }
+    if (r < reps)
+        for (int k = 0; k < 6; ++k)
+            if (ctr[k]) atomicAdd(&B.counters[k], (unsigned long long)ctr[k]);
+}
+
 // Persistent drain of the rays the two-phase variant queued (every replicate's
 // rays share one queue; a ray's job is its replicate): one lane per ray
 // (per-lane tree traversal, cooperative tail) or A.group lanes per ray.  With
@@ -157,25 +195,29 @@ __global__ void k_source_rays(int64_t n, double cx, double cy, double cz, double
 // as balanced as possible (angles first) among partitions whose stacked rows are
 // linearly independent; without one, the identity map (lanes = low index bits)
 // is used.
-void make_ray_map(const ehss_sobol& sob, int m, uint32_t cols[32]) {
+// GF(2) map of a net of 2^m points.  T's first rows are the spatial rows -- the
+// top a_d output bits of dimension d, for the first partition (most balanced,
+// angular dimensions favoured) with sum a_d = spatial whose rows are linearly
+// independent -- completed to a basis by unit rows.  Writes the columns of T^-1
+// (cols), the rows of T (rows, may be NULL) and the partition; false if no
+// partition of that size has independent rows.  Whether the rows are
+// independent, and which unit rows complete them, depend only on their row
+// space, which the LMS scramble does not change (its top-left blocks are unit
+// lower triangular): every scramble of the same net size gets the same
+// partition.
+bool build_map(const ehss_sobol& sob, int m, int spatial, uint32_t cols[32], uint32_t* rows,
+               std::array<int, 4>* part) {
     for (int k = 0; k < 32; ++k) cols[k] = 0;
-    if (m < 6) {
-        for (int k = 0; k < m; ++k) cols[k] = 1u << k;
-        return;
-    }
-    const int cells = m - 6;
     auto row = [&](int d, int p) {
         uint32_t r = 0;
         for (int j = 0; j < m; ++j) r |= ((sob.sv[d][j] >> (29 - p)) & 1u) << j;
         return r;
     };
-    // Candidate partitions, most balanced first; ties favour the angular
-    // coordinates (dimension 1 = azimuth, 3 = impact phase).
     std::vector<std::array<int, 4>> parts;
-    for (int a0 = 0; a0 <= cells; ++a0)
-        for (int a1 = 0; a0 + a1 <= cells; ++a1)
-            for (int a2 = 0; a0 + a1 + a2 <= cells; ++a2) {
-                const int a3 = cells - a0 - a1 - a2;
+    for (int a0 = 0; a0 <= spatial; ++a0)
+        for (int a1 = 0; a0 + a1 <= spatial; ++a1)
+            for (int a2 = 0; a0 + a1 + a2 <= spatial; ++a2) {
+                const int a3 = spatial - a0 - a1 - a2;
                 if (a0 > 30 || a1 > 30 || a2 > 30 || a3 > 30) continue;
                 parts.push_back({a0, a1, a2, a3});
             }
@@ -186,7 +228,7 @@ void make_ray_map(const ehss_sobol& sob, int m, uint32_t cols[32]) {
     std::stable_sort(parts.begin(), parts.end(),
                      [&](const std::array<int, 4>& x, const std::array<int, 4>& y) { return cost(x) < cost(y); });
     for (const auto& a : parts) {
-        // T rows: cell rows first, then unit rows completing the basis.
+        // T rows: spatial rows first, then unit rows completing the basis.
         uint32_t T[32];
         int nr = 0;
         for (int d = 0; d < 4; ++d)
@@ -246,11 +288,109 @@ void make_ray_map(const ehss_sobol& sob, int m, uint32_t cols[32]) {
             for (int c = 0; c < m; ++c) col |= ((B[c] >> k) & 1u) << c;
             cols[k] = col;
         }
-        return;
+        if (rows)
+            for (int k = 0; k < 32; ++k) rows[k] = k < m ? T[k] : 0u;
+        if (part) *part = a;
+        return true;
     }
-    for (int k = 0; k < m; ++k) cols[k] = k < cells ? 1u << (k + 6) : 1u << (k - cells);
+    return false;
 }
 
+// Wave-coherent map of one net: a wave is one cell of m - 6 spatial bits and its
+// lanes are the 64 points inside (identity-like fallback without such a cell).
+void make_ray_map(const ehss_sobol& sob, int m, uint32_t cols[32], uint32_t* rows) {
+    for (int k = 0; k < 32; ++k) cols[k] = 0;
+    if (rows)
+        for (int k = 0; k < 32; ++k) rows[k] = 0;
+    if (m < 6) {
+        for (int k = 0; k < m; ++k) {
+            cols[k] = 1u << k;
+            if (rows) rows[k] = 1u << k;
+        }
+        return;
+    }
+    if (build_map(sob, m, m - 6, cols, rows, nullptr)) return;
+    const int cells = m - 6;
+    for (int k = 0; k < m; ++k) {
+        cols[k] = k < cells ? 1u << (k + 6) : 1u << (k - cells);
+        if (rows) rows[k] = k < cells ? 1u << (k + 6) : 1u << (k - cells);  // a permutation: T = (T^-1)^T
+    }
+}
+
+void make_ray_map(const ehss_sobol& sob, int m, uint32_t cols[32]) { make_ray_map(sob, m, cols, nullptr); }
+
+// Replicate-fastest tables of the cross schedule (CrossTables) for replicates
+// padded to rp: words [m][4][rp] col, [4][rp] off, [m][rp] ucol, [rp] uoff.
+size_t cross_table_words(int m, int rp) { return (size_t)rp * (4 * m + 4 + m + 1); }
+
+void make_cross_tables(const ehss_sobol* sobs, const CrossMap* maps, int reps, int rp, int m, uint32_t* w) {
+    memset(w, 0, sizeof(uint32_t) * cross_table_words(m, rp));
+    uint32_t* col = w;
+    uint32_t* off = col + (size_t)m * 4 * rp;
+    uint32_t* ucol = off + (size_t)4 * rp;
+    uint32_t* uoff = ucol + (size_t)m * rp;
+    for (int r = 0; r < reps; ++r) {
+        const CrossMap& X = maps[r];
+        const ehss_sobol& sb = sobs[r];
+        for (int k = 0; k < m; ++k) {
+            const uint32_t g = X.inv2[k];  // Gray index of cell bit k
+            for (int d = 0; d < 4; ++d) {
+                uint32_t q = 0;
+                for (int j = 0; j < 30; ++j)
+                    if ((g >> j) & 1u) q ^= sb.sv[d][j];
+                col[((size_t)k * 4 + d) * rp + r] = q;
+            }
+            uint32_t u = 0;
+            for (int i = 0; i < m; ++i) u |= (uint32_t)(__builtin_popcount(X.rows1[i] & g) & 1) << i;
+            ucol[(size_t)k * rp + r] = u;
+        }
+        for (int d = 0; d < 4; ++d) {
+            uint32_t q = sb.shift[d];
+            for (int k = 0; k < m; ++k)
+                if ((X.cell_shift >> k) & 1u) q ^= col[((size_t)k * 4 + d) * rp + r];
+            off[(size_t)d * rp + r] = q;
+        }
+        uint32_t u = 0;
+        for (int k = 0; k < m; ++k)
+            if ((X.cell_shift >> k) & 1u) u ^= ucol[(size_t)k * rp + r];
+        uoff[r] = u;
+    }
+}
+
+// Cross-replicate maps for `reps` scrambles of 2^m points: the finest shared
+// spatial partition (m, m - 1, ... spatial bits) whose rows are independent.
+// False when none of at least m - 6 bits exists or the replicates disagree.
+bool make_cross_maps(const ehss_sobol* sobs, int reps, int m, CrossMap* out) {
+    for (int spatial = m; spatial >= std::max(1, m - 6); --spatial) {
+        std::array<int, 4> part0{};
+        bool all = true;
+        for (int r = 0; r < reps && all; ++r) {
+            CrossMap& X = out[r];
+            memset(&X, 0, sizeof X);
+            std::array<int, 4> part{};
+            uint32_t rows2[32];
+            if (!build_map(sobs[r], m, spatial, X.inv2, rows2, &part)) {
+                all = false;
+                break;
+            }
+            if (r == 0) part0 = part;
+            if (part != part0) {
+                all = false;
+                break;
+            }
+            uint32_t cols1[32];
+            make_ray_map(sobs[r], m, cols1, X.rows1);
+            if (m < 6)  // the single-solve slot is the inverse Gray code of g
+                for (int i = 0; i < 32; ++i) X.rows1[i] = i < m ? ((1u << m) - 1u) & ~((1u << i) - 1u) : 0u;
+            // Digital-shift bits of the spatial rows, in T2's row order.
+            int k = 0;
+            for (int d = 0; d < 4; ++d)
+                for (int p = 0; p < part[d]; ++p, ++k) X.cell_shift |= ((sobs[r].shift[d] >> (29 - p)) & 1u) << k;
+        }
+        if (all) return true;
+    }
+    return false;
+}
 // Persistent drain grid: enough waves to keep every SIMD busy, never more than
 // the rays that could be queued.
 int64_t drain_blocks(int64_t n) {
@@ -283,6 +423,7 @@ namespace {
 // [8..] final Partial.
 constexpr int kDefaultVariant = 4;
 constexpr int kMaxReplicates = 4096;
+constexpr int kCrossMinReps = 16;  // replicates per launch from which they are scheduled across lanes
 constexpr size_t kDrainLdsLimit = 144 * 1024;  // of the 160 KB per CU
 
 // Ray index of record slot t (host copy of the device mapping: Gray index from
@@ -333,8 +474,12 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     EHSS_TRY(sc->hist.reserve(sizeof(unsigned long long) * bins * reps));
     EHSS_TRY(sc->ledger.reserve(sizeof(unsigned long long) * bins * reps));
     EHSS_TRY(sc->counters.reserve(sizeof(unsigned long long) * words));
-    EHSS_TRY(sc->tables.reserve((sizeof(ehss_sobol) + sizeof(RayMap)) * reps));
-    EHSS_TRY(sc->host_table.reserve((sizeof(ehss_sobol) + sizeof(RayMap)) * reps));
+    const int32_t groups = (reps + 63) / 64;
+    const int32_t rp = groups * 64;
+    const size_t table_bytes = (sizeof(ehss_sobol) + sizeof(RayMap) + sizeof(CrossMap)) * reps +
+                               sizeof(uint32_t) * cross_table_words(power, rp);
+    EHSS_TRY(sc->tables.reserve(table_bytes));
+    EHSS_TRY(sc->host_table.reserve(table_bytes));
     unsigned long long* ctr = sc->counters.as<unsigned long long>();
     EHSS_TRY(launch_zero(ctr, words, s));
     EHSS_TRY(launch_zero(sc->hist.as<unsigned long long>(), bins * reps, s));
@@ -343,14 +488,26 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     // Scrambles and ray maps, staged in pinned memory (not reused while pending).
     ehss_sobol* h_sob = sc->host_table.as<ehss_sobol>();
     RayMap* h_map = reinterpret_cast<RayMap*>(h_sob + reps);
+    CrossMap* h_cross = reinterpret_cast<CrossMap*>(h_map + reps);
     for (int r = 0; r < reps; ++r) {
         h_sob[r] = sobs[r];
         make_ray_map(sobs[r], power, h_map[r].col);
     }
-    EHSS_HIP(hipMemcpyAsync(sc->tables.p, h_sob, (sizeof(ehss_sobol) + sizeof(RayMap)) * reps,
-                            hipMemcpyHostToDevice, s));
+    // Many replicates: schedule them across lanes when a shared partition exists.
+    const bool cross = reps >= kCrossMinReps && power >= 1 && make_cross_maps(sobs, reps, power, h_cross);
+    uint32_t* h_words = reinterpret_cast<uint32_t*>(h_cross + reps);
+    if (cross) make_cross_tables(sobs, h_cross, reps, rp, power, h_words);
+    EHSS_HIP(hipMemcpyAsync(sc->tables.p, h_sob, table_bytes, hipMemcpyHostToDevice, s));
     const ehss_sobol* d_sob = sc->tables.as<ehss_sobol>();
     const RayMap* d_map = reinterpret_cast<const RayMap*>(d_sob + reps);
+    const uint32_t* d_words = reinterpret_cast<const uint32_t*>(reinterpret_cast<const CrossMap*>(d_map + reps) + reps);
+    CrossTables XT;
+    XT.rp = rp;
+    XT.pad = 0;
+    XT.col = d_words;
+    XT.off = XT.col + (size_t)power * 4 * rp;
+    XT.ucol = XT.off + (size_t)4 * rp;
+    XT.uoff = XT.ucol + (size_t)power * rp;
 
     SolveArgs A;
     A.n = n;
@@ -385,12 +542,20 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
         qbuf = sc->queue.as<RayState>();
     }
     EHSS_HIP(hipEventRecord(sc->ev0, s));
-#define EHSS_LAUNCH(M)                                                                                   \
-    do {                                                                                                 \
-        if (pa)                                                                                          \
-            hipLaunchKernelGGL((k_solve<M, 1>), grid, block, 0, s, sc->view, A, d_sob, d_map, qbuf, qcount); \
-        else                                                                                             \
-            hipLaunchKernelGGL((k_solve<M, 0>), grid, block, 0, s, sc->view, A, d_sob, d_map, qbuf, qcount); \
+    const int64_t cell_waves = (n + kCrossCells - 1) / kCrossCells;
+    const dim3 xgrid((unsigned)((cell_waves * groups * 64 + 255) / 256));
+#define EHSS_LAUNCH(M)                                                                                       \
+    do {                                                                                                     \
+        if (cross && pa)                                                                                     \
+            hipLaunchKernelGGL((k_solve_x<M, 1>), xgrid, block, 0, s, sc->view, A, XT, reps, groups,             \
+                               qbuf, qcount);                                                                \
+        else if (cross)                                                                                      \
+            hipLaunchKernelGGL((k_solve_x<M, 0>), xgrid, block, 0, s, sc->view, A, XT, reps, groups,             \
+                               qbuf, qcount);                                                                \
+        else if (pa)                                                                                         \
+            hipLaunchKernelGGL((k_solve<M, 1>), grid, block, 0, s, sc->view, A, d_sob, d_map, qbuf, qcount);     \
+        else                                                                                                 \
+            hipLaunchKernelGGL((k_solve<M, 0>), grid, block, 0, s, sc->view, A, d_sob, d_map, qbuf, qcount);     \
     } while (0)
     switch (variant) {
         case 0: EHSS_LAUNCH(0); break;
@@ -515,6 +680,17 @@ int solve_collect(ehss_scene* sc, ehss_totals* totals, int64_t* histogram, doubl
 }  // namespace
 
 extern "C" {
+
+int ehss_cross_map(const ehss_sobol* sobols, int reps, int power, uint32_t* out) {
+    EHSS_CHECK(sobols && out, "null sobols/out");
+    EHSS_CHECK(reps >= 1 && reps <= kMaxReplicates, "replicate count %d out of range 1..%d", reps, kMaxReplicates);
+    EHSS_CHECK(power >= 1 && power <= 30, "power %d out of range 1..30", power);
+    std::vector<CrossMap> maps((size_t)reps);
+    if (!make_cross_maps(sobols, reps, power, maps.data()))
+        return set_error(EHSS_ERR_INVALID, "no spatial partition shared by the replicates");
+    memcpy(out, maps.data(), sizeof(CrossMap) * reps);
+    return EHSS_OK;
+}
 
 int ehss_ray_map(const ehss_sobol* sobol, int power, uint32_t* cols) {
     EHSS_CHECK(sobol && cols, "null sobol/cols");

@@ -42,3 +42,51 @@ def test_ray_map_is_a_bijection_with_coherent_waves(oracle, power, seed):
     # sum over dimensions of the fixed bit counts is power - 6: the cell volume
     assert np.prod(np.maximum(widths, 1e-12)) <= 2.0 ** -(power - 6)
     assert widths.max() <= 2.0 ** -((power - 6) // 4 - 1)
+
+
+def _points(oracle, seed, g):
+    sv, shift = oracle.sobol_scramble(seed)
+    q = np.tile(shift.astype(np.uint64), (len(g), 1))
+    for j in range(30):
+        sel = ((g >> np.uint64(j)) & np.uint64(1)).astype(bool)
+        q[sel] ^= sv[:, j].astype(np.uint64)
+    return q
+
+
+@pytest.mark.parametrize("power", [3, 7, 11, 14])
+def test_cross_replicate_schedule(oracle, power):
+    """Every replicate's (cell -> Gray index) is a bijection, its slot is the one
+    its own map gives, and the 64 replicates of a wave share one spatial cell."""
+    seeds = [3 + 7 * k for k in range(20)]
+    sobs = (L.Sobol * len(seeds))(*[scramble(s) for s in seeds])
+    out = np.zeros((len(seeds), 68), np.uint32)
+    L.call("ehss_cross_map", sobs, len(seeds), power, L.ptr(out))
+    n = 1 << power
+    c = np.arange(n, dtype=np.uint64)
+    cells = []
+    for r, seed in enumerate(seeds):
+        inv2, rows1, shift = out[r, :32], out[r, 32:64], np.uint64(out[r, 64])
+        u = c ^ shift
+        g = np.zeros(n, np.uint64)
+        for k in range(power):
+            g ^= np.where((u >> np.uint64(k)) & np.uint64(1), np.uint64(inv2[k]), np.uint64(0))
+        assert np.array_equal(np.sort(g), c)
+        # slot: u1 = T1 g, t = rotate(u1); the replicate's own map sends t back to g
+        u1 = np.zeros(n, np.uint64)
+        for k in range(power):
+            par = np.array([bin(int(x)).count("1") & 1 for x in (g & np.uint64(rows1[k]))], np.uint64)
+            u1 |= par << np.uint64(k)
+        if power < 6:
+            t = g.copy()
+            for sh in (1, 2, 4, 8, 16):
+                t ^= t >> np.uint64(sh)
+        else:
+            lo = np.uint64((1 << (power - 6)) - 1)
+            t = ((u1 & lo) << np.uint64(6)) | (u1 >> np.uint64(power - 6))
+        own = rays_of(scramble(seed), power)
+        assert np.array_equal(own[t.astype(np.int64)], g)
+        cells.append(_points(oracle, seed, g))
+    # replicates at the same cell index: their points agree on a common set of top bits
+    q = np.stack(cells)  # (reps, n, 4)
+    spread = [np.ptp(q[:, :, d].astype(np.float64) / 2 ** 30, axis=0).max() for d in range(4)]
+    assert np.prod(np.maximum(spread, 1e-12)) <= 2.0 ** -(power - 6) + 1e-12
