@@ -77,6 +77,18 @@ class Totals(ct.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+# NumPy view of an array of Totals (every field is 8 bytes: double or int64).
+TOTALS_DTYPE = np.dtype([(name, "<f8" if typ is ct.c_double else "<i8") for name, typ in Totals._fields_])
+assert TOTALS_DTYPE.itemsize == ct.sizeof(Totals)
+
+
+def totals_list(arr, count: int) -> list[dict]:
+    """Dicts of the first `count` entries of a ctypes Totals array."""
+    view = np.frombuffer(arr, dtype=TOTALS_DTYPE, count=count)
+    names = TOTALS_DTYPE.names
+    return [dict(zip(names, row)) for row in view.tolist()]
+
+
 # Signatures: name -> (restype, argtypes)
 _SIGS = {
     "ehss_abi_version": (_INT, []),

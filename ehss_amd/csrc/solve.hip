@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <array>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -195,6 +196,74 @@ __global__ void k_source_rays(int64_t n, double cx, double cy, double cz, double
 // as balanced as possible (angles first) among partitions whose stacked rows are
 // linearly independent; without one, the identity map (lanes = low index bits)
 // is used.
+bool map_for_partition(const ehss_sobol& sob, int m, const std::array<int, 4>& a, uint32_t cols[32],
+                       uint32_t* rows) {
+    for (int k = 0; k < 32; ++k) cols[k] = 0;
+    auto row = [&](int d, int p) {
+        uint32_t r = 0;
+        for (int j = 0; j < m; ++j) r |= ((sob.sv[d][j] >> (29 - p)) & 1u) << j;
+        return r;
+    };
+    // T rows: spatial rows first, then unit rows completing the basis.
+    uint32_t T[32];
+    int nr = 0;
+    for (int d = 0; d < 4; ++d)
+        for (int p = 0; p < a[d]; ++p) T[nr++] = row(d, p);
+    // Gaussian elimination on a copy to test rank and find pivot columns.
+    uint32_t E[32];
+    for (int k = 0; k < nr; ++k) E[k] = T[k];
+    uint32_t pivots = 0;
+    int rank = 0;
+    for (int c = 0; c < m && rank < nr; ++c) {
+        int sel = -1;
+        for (int k = rank; k < nr; ++k)
+            if ((E[k] >> c) & 1u) {
+                sel = k;
+                break;
+            }
+        if (sel < 0) continue;
+        std::swap(E[rank], E[sel]);
+        for (int k = 0; k < nr; ++k)
+            if (k != rank && ((E[k] >> c) & 1u)) E[k] ^= E[rank];
+        pivots |= 1u << c;
+        ++rank;
+    }
+    if (rank < nr) return false;
+    for (int c = 0; c < m; ++c)
+        if (!((pivots >> c) & 1u)) T[nr++] = 1u << c;  // completes to full rank
+    // Invert T (m x m over GF(2)): [T | I] -> [I | T^-1]; rows as bit masks.
+    uint32_t A[32], B[32];
+    for (int k = 0; k < m; ++k) {
+        A[k] = T[k];
+        B[k] = 1u << k;
+    }
+    for (int c = 0; c < m; ++c) {
+        int sel = -1;
+        for (int k = c; k < m; ++k)
+            if ((A[k] >> c) & 1u) {
+                sel = k;
+                break;
+            }
+        if (sel < 0) return false;
+        std::swap(A[c], A[sel]);
+        std::swap(B[c], B[sel]);
+        for (int k = 0; k < m; ++k)
+            if (k != c && ((A[k] >> c) & 1u)) {
+                A[k] ^= A[c];
+                B[k] ^= B[c];
+            }
+    }
+    // Row c of T^-1 is B[c]; column k of T^-1 collects bit k of every row.
+    for (int k = 0; k < m; ++k) {
+        uint32_t col = 0;
+        for (int c = 0; c < m; ++c) col |= ((B[c] >> k) & 1u) << c;
+        cols[k] = col;
+    }
+    if (rows)
+        for (int k = 0; k < 32; ++k) rows[k] = k < m ? T[k] : 0u;
+    return true;
+}
+
 // GF(2) map of a net of 2^m points.  T's first rows are the spatial rows -- the
 // top a_d output bits of dimension d, for the first partition (most balanced,
 // angular dimensions favoured) with sum a_d = spatial whose rows are linearly
@@ -205,14 +274,28 @@ __global__ void k_source_rays(int64_t n, double cx, double cy, double cz, double
 // space, which the LMS scramble does not change (its top-left blocks are unit
 // lower triangular): every scramble of the same net size gets the same
 // partition.
+// T for one partition; false if its spatial rows are dependent.
+bool map_for_partition(const ehss_sobol& sob, int m, const std::array<int, 4>& a, uint32_t cols[32],
+                       uint32_t* rows);
+
+// Partition chosen for (m, spatial), found once on the first scramble and reused:
+// it does not depend on the scramble.  {-1,...} when none exists.
+std::mutex g_part_mutex;
+std::map<std::pair<int, int>, std::array<int, 4>> g_part_cache;
+
 bool build_map(const ehss_sobol& sob, int m, int spatial, uint32_t cols[32], uint32_t* rows,
                std::array<int, 4>* part) {
-    for (int k = 0; k < 32; ++k) cols[k] = 0;
-    auto row = [&](int d, int p) {
-        uint32_t r = 0;
-        for (int j = 0; j < m; ++j) r |= ((sob.sv[d][j] >> (29 - p)) & 1u) << j;
-        return r;
-    };
+    {
+        std::lock_guard<std::mutex> lock(g_part_mutex);
+        auto it = g_part_cache.find({m, spatial});
+        if (it != g_part_cache.end()) {
+            if (it->second[0] < 0) return false;
+            if (map_for_partition(sob, m, it->second, cols, rows)) {
+                if (part) *part = it->second;
+                return true;
+            }
+        }
+    }
     std::vector<std::array<int, 4>> parts;
     for (int a0 = 0; a0 <= spatial; ++a0)
         for (int a1 = 0; a0 + a1 <= spatial; ++a1)
@@ -228,70 +311,15 @@ bool build_map(const ehss_sobol& sob, int m, int spatial, uint32_t cols[32], uin
     std::stable_sort(parts.begin(), parts.end(),
                      [&](const std::array<int, 4>& x, const std::array<int, 4>& y) { return cost(x) < cost(y); });
     for (const auto& a : parts) {
-        // T rows: spatial rows first, then unit rows completing the basis.
-        uint32_t T[32];
-        int nr = 0;
-        for (int d = 0; d < 4; ++d)
-            for (int p = 0; p < a[d]; ++p) T[nr++] = row(d, p);
-        // Gaussian elimination on a copy to test rank and find pivot columns.
-        uint32_t E[32];
-        for (int k = 0; k < nr; ++k) E[k] = T[k];
-        uint32_t pivots = 0;
-        int rank = 0;
-        for (int c = 0; c < m && rank < nr; ++c) {
-            int sel = -1;
-            for (int k = rank; k < nr; ++k)
-                if ((E[k] >> c) & 1u) {
-                    sel = k;
-                    break;
-                }
-            if (sel < 0) continue;
-            std::swap(E[rank], E[sel]);
-            for (int k = 0; k < nr; ++k)
-                if (k != rank && ((E[k] >> c) & 1u)) E[k] ^= E[rank];
-            pivots |= 1u << c;
-            ++rank;
-        }
-        if (rank < nr) continue;
-        for (int c = 0; c < m; ++c)
-            if (!((pivots >> c) & 1u)) T[nr++] = 1u << c;  // completes to full rank
-        // Invert T (m x m over GF(2)): [T | I] -> [I | T^-1]; rows as bit masks.
-        uint32_t A[32], B[32];
-        for (int k = 0; k < m; ++k) {
-            A[k] = T[k];
-            B[k] = 1u << k;
-        }
-        bool ok = true;
-        for (int c = 0; c < m && ok; ++c) {
-            int sel = -1;
-            for (int k = c; k < m; ++k)
-                if ((A[k] >> c) & 1u) {
-                    sel = k;
-                    break;
-                }
-            if (sel < 0) {
-                ok = false;
-                break;
-            }
-            std::swap(A[c], A[sel]);
-            std::swap(B[c], B[sel]);
-            for (int k = 0; k < m; ++k)
-                if (k != c && ((A[k] >> c) & 1u)) {
-                    A[k] ^= A[c];
-                    B[k] ^= B[c];
-                }
-        }
-        if (!ok) continue;
-        // Row c of T^-1 is B[c]; column k of T^-1 collects bit k of every row.
-        for (int k = 0; k < m; ++k) {
-            uint32_t col = 0;
-            for (int c = 0; c < m; ++c) col |= ((B[c] >> k) & 1u) << c;
-            cols[k] = col;
-        }
-        if (rows)
-            for (int k = 0; k < 32; ++k) rows[k] = k < m ? T[k] : 0u;
+        if (!map_for_partition(sob, m, a, cols, rows)) continue;
         if (part) *part = a;
+        std::lock_guard<std::mutex> lock(g_part_mutex);
+        g_part_cache[{m, spatial}] = a;
         return true;
+    }
+    {
+        std::lock_guard<std::mutex> lock(g_part_mutex);
+        g_part_cache[{m, spatial}] = {-1, -1, -1, -1};
     }
     return false;
 }
@@ -336,8 +364,7 @@ void make_cross_tables(const ehss_sobol* sobs, const CrossMap* maps, int reps, i
             const uint32_t g = X.inv2[k];  // Gray index of cell bit k
             for (int d = 0; d < 4; ++d) {
                 uint32_t q = 0;
-                for (int j = 0; j < 30; ++j)
-                    if ((g >> j) & 1u) q ^= sb.sv[d][j];
+                for (uint32_t bits = g; bits; bits &= bits - 1u) q ^= sb.sv[d][__builtin_ctz(bits)];
                 col[((size_t)k * 4 + d) * rp + r] = q;
             }
             uint32_t u = 0;

@@ -261,13 +261,11 @@ class GPUScene:
 
     @staticmethod
     def _split(totals, reps, cap, hist, ledger) -> list[dict]:
-        out = []
-        for r in range(reps):
-            d = totals[r].as_dict()
-            if hist is not None:
+        out = L.totals_list(totals, reps)
+        if hist is not None:
+            for r, d in enumerate(out):
                 d["histogram"] = hist[r * (cap + 1):(r + 1) * (cap + 1)].copy()
                 d["ledger"] = ledger[r * (cap + 1):(r + 1) * (cap + 1)].copy()
-            out.append(d)
         return out
 
     def solve_async(self, sobol, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
