@@ -116,3 +116,17 @@ def test_replicate_launch_async_and_limits():
             s.solve_many([], 10, 64)
         with pytest.raises(L.EHSSError, match="2\\^30"):
             s.solve_many(sobols[:4], 29, 64)
+
+
+def test_publication_protocol_on_device():
+    from ehss_amd import protocol
+    c, r = scenes.globule(650, 1)
+    with GPUScene(c, r) as s:
+        res = protocol.publication_estimate(s, 4242, 12, caps=(1, 2, 4, 64), tail_fraction=1e-3)
+        assert [a.cap for a in res.attempts][0] == 1 and len(res.attempts) >= 2
+        for att in res.attempts:  # each attempt is the mean of the three blocks solved alone
+            alone = [s.solve(scramble(x), 12, att.cap)["omega"] for x in protocol.block_seeds(4242)]
+            assert att.block_omegas == alone
+            assert att.omega == pytest.approx(np.mean(alone), rel=1e-15)
+        tails = [a.tail_bound for a in res.attempts]
+        assert all(b <= a for a, b in zip(tails, tails[1:]))
