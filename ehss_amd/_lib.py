@@ -117,6 +117,8 @@ _SIGS = {
     "ehss_ray_map": (_INT, [ct.POINTER(Sobol), _INT, _P]),
     "ehss_cross_map": (_INT, [_P, _INT, _INT, _P]),
     "ehss_source_rays": (_INT, [_P, ct.POINTER(Sobol), _INT, _P, _P, _P]),
+    "ehss_pa_grid": (_INT, [_I64, _P, _P, _INT, _P, _INT, _P, _P]),
+    "ehss_pa_grid_count": (_INT, [_I64, _P, _P, _P, ct.c_double, _INT, _P]),
     "ehss_batch_create": (_INT, [_INT, _P, _P, _P, _P, _INT, ct.c_uint, _INT, ct.POINTER(_P)]),
     "ehss_batch_set_sobol": (_INT, [_P, _INT, ct.POINTER(Sobol)]),
     "ehss_batch_launch": (_INT, [_P]),

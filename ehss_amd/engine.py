@@ -73,6 +73,20 @@ def scramble(seed: int) -> L.Sobol:
     return s
 
 
+def sobol_points(seed: int, dim: int, m: int) -> np.ndarray:
+    """First 2^m points of scipy.stats.qmc.Sobol(dim, scramble=True,
+    seed).random_base2(m) (Gray-code order, _qmc.py:1866-1879), on the host."""
+    sv, shift = scramble_matrices(seed, dim)
+    n = 1 << int(m)
+    i = np.arange(n, dtype=np.uint64)
+    g = i ^ (i >> np.uint64(1))
+    q = np.tile(np.asarray(shift, dtype=np.uint64), (n, 1))
+    for j in range(int(m)):
+        sel = ((g >> np.uint64(j)) & np.uint64(1)).astype(bool)
+        q[sel] ^= np.asarray(sv[:, j], dtype=np.uint64)
+    return q.astype(np.float64) * 2.0 ** -BITS
+
+
 def replicate_seed(seed: int, repeat: int) -> int:
     """ccs_cli.py:190: (seed + 104729 * repeat) mod 2^32."""
     return (int(seed) + 104729 * int(repeat)) % (2 ** 32)

@@ -234,6 +234,24 @@ int ehss_source_rays(ehss_scene* scene, const ehss_sobol* sobol, int power, doub
                      double* directions);
 
 /* ---------------------------------------------------------------------------
+ * Finite-grid PA: the CLI's adaptive method (ccs_cli.py:142-160; the union
+ * count of pa_sparse_blocks.py:125-174 / pa_dyadic_bitmask.py:86-97).
+ * ------------------------------------------------------------------------- */
+/* For each of n_dirs directions: frames[6 d .. 6 d + 5] = the unit vectors u, v
+ * of the projection plane (pa_walsh_geometry.py:173-176, computed by the
+ * caller); centered = atom centres minus their mean.  The atoms are projected on
+ * the device, the grid half-width is max(|x| + r, |y| + r)(1 + 1e-10), and
+ * counts[d] gets the number of grid x grid midpoint pixels covered by the union
+ * of the projected disks, spacing[d] (may be NULL) the pixel size: PA of the
+ * direction = counts[d] spacing[d]^2.  grid: power of two in 8..8192. */
+int ehss_pa_grid(int64_t n_atoms, const double* centered, const double* radii, int n_dirs, const double* frames,
+                 int grid, int64_t* counts, double* spacing);
+/* The covered-pixel count for caller-projected disks (xy: n x 2) on the grid
+ * with lower corner lower[2] and pixel size spacing. */
+int ehss_pa_grid_count(int64_t n_atoms, const double* xy, const double* radii, const double* lower, double spacing,
+                       int grid, int64_t* count);
+
+/* ---------------------------------------------------------------------------
  * Batched solves: many (scene, seed, 2^power rays, cap) jobs in one persistent
  * launch with a ray-packet work queue; each job is reduced in its own fixed
  * order, so its totals equal those of ehss_solve on the same job.  Scenes must

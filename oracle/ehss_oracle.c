@@ -601,3 +601,106 @@ void oracle_solve_cpu(int64_t n_rays, const uint32_t* sv, const uint32_t* shift,
 }
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+/* ---------------------------------------------------------------------------
+ * Finite-grid PA (the CLI's adaptive PA): union of projected disks on an n x n
+ * midpoint grid.
+ * ------------------------------------------------------------------------- */
+
+/* pa_dyadic_bitmask.py:55-66 (row_span): pixels [lo, hi) of row y whose midpoint
+ * satisfies (lower0 + (x + .5) spacing - cx)^2 <= r^2 - dy^2, with the same
+ * rounding of the first guess and the same refinement loops. */
+static void row_span(double cx, double cy, double r, const double* lower, double spacing, int64_t y, int64_t n,
+                     int64_t* lo_out, int64_t* hi_out) {
+    double dy = lower[1] + ((double)y + .5) * spacing - cy;
+    double rem = r * r - dy * dy;
+    *lo_out = *hi_out = 0;
+    if (rem < 0) return;
+    double half = sqrt(rem);
+    int64_t lo = (int64_t)ceil((cx - half - lower[0]) / spacing - .5);
+    int64_t hi = (int64_t)floor((cx + half - lower[0]) / spacing - .5) + 1;
+    lo = lo < n ? lo : n;
+    lo = lo > 0 ? lo : 0;
+    hi = hi < n ? hi : n;
+    hi = hi > 0 ? hi : 0;
+#define SQ(v) ((v) * (v))
+    while (lo > 0 && SQ(lower[0] + ((double)lo - .5) * spacing - cx) <= rem) --lo;
+    while (lo < hi && SQ(lower[0] + ((double)lo + .5) * spacing - cx) > rem) ++lo;
+    while (hi < n && SQ(lower[0] + ((double)hi + .5) * spacing - cx) <= rem) ++hi;
+    while (hi > lo && SQ(lower[0] + ((double)hi - .5) * spacing - cx) > rem) --hi;
+#undef SQ
+    *lo_out = lo;
+    *hi_out = hi;
+}
+
+/* pa_dyadic_bitmask.py:68-72 (row_bounds): candidate rows [lo, hi), one row of
+ * slack on each side of the rounded disk extent. */
+static void row_bounds(double cy, double r, const double* lower, double spacing, int64_t n, int64_t* lo_out,
+                       int64_t* hi_out) {
+    int64_t lo = (int64_t)ceil((cy - r - lower[1]) / spacing - .5) - 1;
+    int64_t hi = (int64_t)floor((cy + r - lower[1]) / spacing - .5) + 2;
+    lo = lo < n ? lo : n;
+    lo = lo > 0 ? lo : 0;
+    hi = hi < n ? hi : n;
+    hi = hi > 0 ? hi : 0;
+    *lo_out = lo;
+    *hi_out = hi;
+}
+
+void oracle_row_span(double cx, double cy, double r, const double* lower, double spacing, int64_t y, int64_t n,
+                     int64_t* out) {
+    row_span(cx, cy, r, lower, spacing, y, n, out, out + 1);
+}
+
+/* pa_dyadic_bitmask.py:86-97 (count_disks): number of grid pixels covered by
+ * the union of the disks (the value block_count(block_disks(...)) returns,
+ * pa_sparse_blocks.py:166-174).  rows_out (may be NULL) gets the row spans
+ * evaluated. */
+int64_t oracle_grid_count(int64_t na, const double* xy, const double* radii, const double* lower, double spacing,
+                          int64_t n, int64_t* rows_out) {
+    int64_t* delta = (int64_t*)calloc((size_t)(n * (n + 1)), sizeof(int64_t));
+    int64_t rows = 0;
+    for (int64_t a = 0; a < na; ++a) {
+        int64_t first, stop;
+        row_bounds(xy[2 * a + 1], radii[a], lower, spacing, n, &first, &stop);
+        for (int64_t y = first; y < stop; ++y) {
+            int64_t lo, hi;
+            row_span(xy[2 * a], xy[2 * a + 1], radii[a], lower, spacing, y, n, &lo, &hi);
+            ++rows;
+            if (hi > lo) {
+                delta[y * (n + 1) + lo] += 1;
+                delta[y * (n + 1) + hi] -= 1;
+            }
+        }
+    }
+    int64_t count = 0;
+    for (int64_t y = 0; y < n; ++y) {
+        int64_t active = 0;
+        for (int64_t x = 0; x < n; ++x) {
+            active += delta[y * (n + 1) + x];
+            count += active > 0;
+        }
+    }
+    free(delta);
+    if (rows_out) *rows_out = rows;
+    return count;
+}
+
+/* Brute force over every pixel and every disk with the row_span inequality (an
+ * independent twin of oracle_grid_count for small grids). */
+int64_t oracle_grid_brute(int64_t na, const double* xy, const double* radii, const double* lower, double spacing,
+                          int64_t n) {
+    int64_t count = 0;
+    for (int64_t y = 0; y < n; ++y)
+        for (int64_t x = 0; x < n; ++x)
+            for (int64_t a = 0; a < na; ++a) {
+                double dy = lower[1] + ((double)y + .5) * spacing - xy[2 * a + 1];
+                double rem = radii[a] * radii[a] - dy * dy;
+                double dx = lower[0] + ((double)x + .5) * spacing - xy[2 * a];
+                if (rem >= 0 && dx * dx <= rem) {
+                    ++count;
+                    break;
+                }
+            }
+    return count;
+}

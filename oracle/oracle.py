@@ -83,6 +83,12 @@ def lib() -> ct.CDLL:
         L.oracle_solve_cpu.argtypes = ([_I64, _P, _P, _P, ct.c_double, ct.c_double, _INT, _INT, _INT, _I64]
                                        + [_P] * 8 + [_I64] + [_P] * 6 + [_P])
         L.oracle_max_threads.restype = _INT
+        L.oracle_row_span.restype = None
+        L.oracle_row_span.argtypes = [ct.c_double, ct.c_double, ct.c_double, _P, ct.c_double, _I64, _I64, _P]
+        L.oracle_grid_count.restype = _I64
+        L.oracle_grid_count.argtypes = [_I64, _P, _P, _P, ct.c_double, _I64, _P]
+        L.oracle_grid_brute.restype = _I64
+        L.oracle_grid_brute.argtypes = [_I64, _P, _P, _P, ct.c_double, _I64]
         _lib = L
     return _lib
 
@@ -454,3 +460,62 @@ def ref_lib():
     L.ehss_trace.restype = None
     L.ehss_trace.argtypes = [_I64, _I64, _INT, _INT, _INT] + [_P] * 17
     return L
+
+
+# ---------------------------------------------------------------------------
+# Finite-grid ("adaptive") PA of the CLI (ccs_cli.py:142-160)
+# ---------------------------------------------------------------------------
+def project_grid(centers, radii, direction, n):
+    """pa_walsh_geometry.py:171-180: projection frame, centred disk centres,
+    enclosing square (lower corner, pixel spacing)."""
+    direction = np.asarray(direction, float)
+    direction = direction / np.linalg.norm(direction)
+    axis = np.array([0., 0., 1.]) if abs(direction[2]) < .9 else np.array([1., 0., 0.])
+    u = np.cross(axis, direction)
+    u /= np.linalg.norm(u)
+    v = np.cross(direction, u)
+    centered = np.asarray(centers, float) - np.asarray(centers, float).mean(axis=0)
+    xy = np.column_stack((centered @ u, centered @ v))
+    extent = float(np.max(np.abs(xy) + np.asarray(radii, float)[:, None])) * (1. + 1e-10)
+    lower = np.array([-extent, -extent])
+    spacing = 2 * extent / n
+    return xy, lower, spacing, u, v, direction
+
+
+def grid_count(xy, radii, lower, spacing, n):
+    """Covered-pixel count of the disk union (pa_dyadic_bitmask.py:86-97)."""
+    xy = _f64(xy)
+    r = _f64(radii)
+    lo = _f64(lower)
+    rows = ct.c_int64(0)
+    c = lib().oracle_grid_count(len(r), _ptr(xy), _ptr(r), _ptr(lo), float(spacing), int(n), ct.byref(rows))
+    return int(c), int(rows.value)
+
+
+def grid_brute(xy, radii, lower, spacing, n):
+    xy, r, lo = _f64(xy), _f64(radii), _f64(lower)
+    return int(lib().oracle_grid_brute(len(r), _ptr(xy), _ptr(r), _ptr(lo), float(spacing), int(n)))
+
+
+def row_span(cx, cy, r, lower, spacing, y, n):
+    out = np.zeros(2, np.int64)
+    lo = _f64(lower)
+    lib().oracle_row_span(float(cx), float(cy), float(r), _ptr(lo), float(spacing), int(y), int(n), _ptr(out))
+    return int(out[0]), int(out[1])
+
+
+def adaptive_directions(directions, seed):
+    """ccs_cli.py:148: unit_sphere of the first `directions` points of a 2-D
+    scrambled Sobol net."""
+    m = int(directions).bit_length() - 1
+    sv, shift = sobol_scramble(seed, dim=2)
+    return unit_sphere(sobol_points(sv, shift, m))
+
+
+def adaptive_pa(centers, radii, directions, grid_size, seed):
+    """ccs_cli.py:142-160 with the union count of oracle_grid_count."""
+    values = []
+    for direction in adaptive_directions(directions, seed):
+        xy, lower, spacing, *_ = project_grid(centers, radii, direction, grid_size)
+        values.append(float(grid_count(xy, radii, lower, spacing, grid_size)[0]) * spacing ** 2)
+    return float(np.mean(values)), values

@@ -134,3 +134,18 @@ def test_execute_writes_outputs_and_refuses_overwrite(tmp_path):
     with pytest.raises(ValueError, match="already exists"):
         execute(args)
     assert (args.output / "results.json").read_bytes() == before
+
+
+@pytest.mark.gpu
+def test_default_adaptive_pa_single_sphere(tmp_path):
+    """tests/test_ccs_cli.py:59-70 with the default --pa-method adaptive."""
+    from ehss_amd.ccs_cli import solve_frame
+    from ehss_amd.state import Spheres
+    args = options(tmp_path)
+    assert args.pa_method == "adaptive"
+    s = Spheres(np.zeros((1, 3)), np.array([3.1]))
+    a = solve_frame(s, args, False)
+    truth = np.pi * 3.1 ** 2
+    assert a["summary"]["pa_angstrom2"]["mean"] == pytest.approx(truth, rel=0.01)
+    assert a["summary"]["ehss_angstrom2"]["mean"] == pytest.approx(truth, rel=2e-6)
+    assert a["summary"] == solve_frame(s, args, False)["summary"]
