@@ -30,6 +30,7 @@ FLAG_TIMESTAMPS = 8
 VARIANT_SHIFT = 8
 COOP_SHIFT = 12
 GROUP_SHIFT = 19
+STAGE1_SHIFT = 22
 
 RAY_ESCAPED = 0x1
 RAY_UNRESOLVED = 0x2

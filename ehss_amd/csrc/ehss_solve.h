@@ -308,13 +308,21 @@ inline int coop_from_flags(unsigned flags) {
     return f == 0 ? kDefaultCoop : f - 1;
 }
 
-// Lanes per ray of the queue drain from flags (bits 19..21: log2 + 1).  The
-// default gives every queued ray of a small solve the whole wave, and two rays
-// per wave from 2^18 rays up, where the queue is long enough to fill the chip.
+// Drain shape from flags: lanes per ray of the first drain (bits 19..21,
+// log2 + 1) and the queries a ray gets there before it moves to the whole-wave
+// second drain (bits 22..25, q + 1; 0 queries = one drain).  Defaults (field
+// 0): from 2^18 rays, where the queue fills the chip, 4 lanes per ray for 2
+// queries, then the second drain; below, one drain with a wave per ray.
 inline int group_from_flags(unsigned flags, int64_t rays) {
     const int f = (int)((flags >> EHSS_GROUP_SHIFT) & 0x7u);
-    if (f == 0) return rays >= (1 << 18) ? 32 : 64;
+    if (f == 0) return rays >= (1 << 18) ? 4 : 64;
     return 1 << (f - 1);
+}
+
+inline int stage1_queries(unsigned flags, int64_t rays) {
+    const int f = (int)((flags >> EHSS_STAGE1_SHIFT) & 0xFu);
+    if (f == 0) return rays >= (1 << 18) ? 2 : 0;
+    return f - 1;
 }
 
 // Value of lane `src` (wave-uniform) on every lane.
@@ -565,9 +573,16 @@ __device__ __forceinline__ uint64_t group_sum(uint64_t v, int G) {
 // group takes the next queued ray (one atomic per wave per refill round), so a
 // long path holds only its own group.  `account` receives each finished ray's
 // work (summed over the group, on the group's first lane).
+//
+// With max_queries > 0 a ray that is still bouncing after that many queries in
+// this drain is appended to `spill` (count in *spill_count) instead, for a
+// second drain with more lanes per ray: the first drain keeps its lanes on the
+// many short paths, the second gives the few long ones whole waves.
 template <class SceneOf, class ArgsOf, class Account>
 __device__ __forceinline__ void drain_groups(const RayState* queue, uint32_t queue_len, uint32_t* queue_head,
-                                             int G, SceneOf scene_of, ArgsOf args_of, Account account) {
+                                             int G, SceneOf scene_of, ArgsOf args_of, Account account,
+                                             int max_queries = 0, RayState* spill = nullptr,
+                                             uint32_t* spill_count = nullptr) {
     const int lane = threadIdx.x & (kWave - 1);
     const int sub = lane & (G - 1);
     const int leader = lane - sub;
@@ -576,6 +591,7 @@ __device__ __forceinline__ void drain_groups(const RayState* queue, uint32_t que
     int32_t job = 0;
     bool have = false;       // the same on every lane of a group
     bool exhausted = false;  // wave-uniform: the queue head has passed the end
+    int queries = 0;         // queries of the group's current ray in this drain
     uint64_t boxes = 0, spheres = 0, retests = 0;  // this lane's share of the current ray
     for (;;) {
         const uint64_t idle = __ballot(!have && sub == 0);  // one bit per idle group, at its first lane
@@ -590,6 +606,7 @@ __device__ __forceinline__ void drain_groups(const RayState* queue, uint32_t que
             if (!have && slot < queue_len) {
                 job = ray_load(queue + slot, scene_of(ray_job(queue + slot)), R);
                 have = true;
+                queries = 0;
             }
         }
         if (__ballot(have) == 0) break;
@@ -599,7 +616,18 @@ __device__ __forceinline__ void drain_groups(const RayState* queue, uint32_t que
             nearest_group(S, R.o, R.d, R.last, R.tol, G, sub, q, boxes, spheres, retests);
             const SolveArgs& A = args_of(job);
             ray_process(S, A.cap, R, q);
-            if (R.done) {
+            if (!R.done && ++queries == max_queries) {
+                // Long path: hand it to the second drain.
+                boxes = group_sum(boxes, G);
+                spheres = group_sum(spheres, G);
+                retests = group_sum(retests, G);
+                if (sub == 0) {
+                    account(job, boxes, spheres, retests);
+                    ray_save(R, job, spill + atomicAdd(spill_count, 1u));
+                }
+                boxes = spheres = retests = 0;
+                have = false;
+            } else if (R.done) {
                 boxes = group_sum(boxes, G);
                 spheres = group_sum(spheres, G);
                 retests = group_sum(retests, G);

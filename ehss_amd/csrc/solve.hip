@@ -87,7 +87,8 @@ __global__ void __launch_bounds__(256) k_solve_x(SceneView S, SolveArgs A, Cross
 // drain is an LDS access instead of a dependent L2 round trip.
 template <int kLds>
 __global__ void __launch_bounds__(256) k_drain(SceneView S, SolveArgs A, int32_t replicates, const RayState* queue,
-                                               const uint32_t* queue_count, uint32_t* queue_head) {
+                                               const uint32_t* queue_count, uint32_t* queue_head, int32_t group,
+                                               int32_t max_queries, RayState* spill, uint32_t* spill_count) {
     extern __shared__ __align__(16) unsigned char lds[];
     uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
     const uint32_t len = __builtin_amdgcn_readfirstlane(*queue_count);
@@ -129,8 +130,8 @@ __global__ void __launch_bounds__(256) k_drain(SceneView S, SolveArgs A, int32_t
             if (r) atomicAdd(&c[2], (unsigned long long)r);
         }
     };
-    if (A.group > 1)
-        drain_groups(queue, len, queue_head, A.group, scene_of, args_of, account);
+    if (group > 1)
+        drain_groups(queue, len, queue_head, group, scene_of, args_of, account, max_queries, spill, spill_count);
     else
         drain_queue(queue, len, queue_head, A.coop, scene_of, args_of, account);
     if (replicates == 1) flush_counters(ctr, A.counters);
@@ -494,7 +495,7 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     const int64_t total = n * reps;
     const int64_t nblocks = (n + kReduceRays - 1) / kReduceRays;
     const int64_t bins = cap + 1;
-    const int64_t words = (int64_t)kCounterWords * reps + 1;  // + queue length / head
+    const int64_t words = (int64_t)kCounterWords * reps + 2;  // + queue and spill length / head
     EHSS_TRY(sc->ray_info.reserve(sizeof(uint32_t) * total));
     EHSS_TRY(sc->ray_contrib.reserve(sizeof(double) * total));
     EHSS_TRY(sc->partials.reserve(sizeof(Partial) * nblocks * reps));
@@ -564,9 +565,16 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     EHSS_CHECK(variant >= 0 && variant <= 4, "unknown kernel variant %d", variant);
     RayState* qbuf = nullptr;
     uint32_t* qcount = reinterpret_cast<uint32_t*>(ctr + (int64_t)kCounterWords * reps);
+    RayState* spill = nullptr;
+    uint32_t* scount = qcount + 2;
+    const int stage1 = stage1_queries(flags, total);
     if (variant == 4) {
         EHSS_TRY(sc->queue.reserve(sizeof(RayState) * total));
         qbuf = sc->queue.as<RayState>();
+        if (stage1 > 0 && A.group > 1) {
+            EHSS_TRY(sc->queue2.reserve(sizeof(RayState) * total));
+            spill = sc->queue2.as<RayState>();
+        }
     }
     EHSS_HIP(hipEventRecord(sc->ev0, s));
     const int64_t cell_waves = (n + kCrossCells - 1) / kCrossCells;
@@ -603,10 +611,21 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDrainLdsLimit);
             });
             EHSS_HIP(attr);
-            hipLaunchKernelGGL(k_drain<1>, dgrid, dim3(256), lds, s, sc->view, A, reps, qbuf, qcount, qcount + 1);
-        } else {
-            hipLaunchKernelGGL(k_drain<0>, dgrid, dim3(256), 0, s, sc->view, A, reps, qbuf, qcount, qcount + 1);
         }
+        // First drain: A.group lanes per ray; with a spill buffer, rays still
+        // bouncing after stage1 queries move to a second drain with whole
+        // waves per ray.
+        auto drain = [&](const RayState* in, const uint32_t* count, uint32_t* head, int group, int max_q,
+                         RayState* out, uint32_t* out_count) {
+            if (lds <= kDrainLdsLimit)
+                hipLaunchKernelGGL(k_drain<1>, dgrid, dim3(256), lds, s, sc->view, A, reps, in, count, head, group,
+                                   max_q, out, out_count);
+            else
+                hipLaunchKernelGGL(k_drain<0>, dgrid, dim3(256), 0, s, sc->view, A, reps, in, count, head, group,
+                                   max_q, out, out_count);
+        };
+        drain(qbuf, qcount, qcount + 1, A.group, spill ? stage1 : 0, spill, scount);
+        if (spill) drain(spill, scount, scount + 1, kWave, 0, nullptr, nullptr);
     }
     EHSS_HIP(hipGetLastError());
     EHSS_HIP(hipEventRecord(sc->ev1, s));

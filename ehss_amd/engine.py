@@ -110,10 +110,15 @@ def precision_code(precision) -> int:
     return int(precision)
 
 
-def coop_flags(coop: int | None, group: int | None = None) -> int:
-    """Flag bits of a tail threshold and of the drain's lanes per ray (None:
-    library defaults)."""
+def coop_flags(coop: int | None, group: int | None = None, stage1: int | None = None) -> int:
+    """Flag bits of a tail threshold, of the drain's lanes per ray and of the
+    queries a ray gets in the first drain (None: library defaults; stage1 0: one
+    drain)."""
     flags = 0
+    if stage1 is not None:
+        if not 0 <= int(stage1) <= 14:
+            raise ValueError("stage1 must be in 0..14")
+        flags |= (int(stage1) + 1) << L.STAGE1_SHIFT
     if coop is not None:
         if not 0 <= int(coop) <= 64:
             raise ValueError("coop must be in 0..64")
@@ -236,7 +241,7 @@ class GPUScene:
     # -- fused device solve ---------------------------------------------------------
     def solve(self, sobol: L.Sobol, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
               keep_rays: bool = False, histogram: bool = False, variant: int = 0, coop: int | None = None,
-              timestamps: bool = False, group: int | None = None) -> dict:
+              timestamps: bool = False, group: int | None = None, stage1: int | None = None) -> dict:
         """One fused device solve of 2^power rays.  `variant` selects the FP32
         filter kernel (0 default, 1 per-lane, 2 packet first query, 3 packet
         throughout, 4 packet first queries then queue + persistent drain);
@@ -248,7 +253,7 @@ class GPUScene:
         hist = np.zeros(cap + 1, np.int64) if histogram else None
         ledger = np.zeros(cap + 1, np.float64) if histogram else None
         flags = (L.FLAG_PA if pa else 0) | (L.FLAG_KEEP_RAYS if keep_rays else 0) | (int(variant) << L.VARIANT_SHIFT)
-        flags |= coop_flags(coop, group) | (L.FLAG_TIMESTAMPS if timestamps else 0)
+        flags |= coop_flags(coop, group, stage1) | (L.FLAG_TIMESTAMPS if timestamps else 0)
         L.call("ehss_solve", self.handle, ct.byref(sobol), int(power), int(cap), precision_code(precision),
                flags, ct.byref(totals), L.ptr(hist), L.ptr(ledger))
         out = totals.as_dict()
@@ -259,7 +264,7 @@ class GPUScene:
 
     def solve_many(self, sobols, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
                    histogram: bool = False, variant: int = 0, coop: int | None = None, group: int | None = None,
-                   keep_rays: bool = False) -> list[dict]:
+                   keep_rays: bool = False, stage1: int | None = None) -> list[dict]:
         """len(sobols) independent solves in one launch; one result dict per
         scramble, each equal to solve() with that scramble."""
         reps = len(sobols)
@@ -268,7 +273,7 @@ class GPUScene:
         hist = np.zeros(reps * (cap + 1), np.int64) if histogram else None
         ledger = np.zeros(reps * (cap + 1), np.float64) if histogram else None
         flags = (L.FLAG_PA if pa else 0) | (L.FLAG_KEEP_RAYS if keep_rays else 0) | \
-            (int(variant) << L.VARIANT_SHIFT) | coop_flags(coop, group)
+            (int(variant) << L.VARIANT_SHIFT) | coop_flags(coop, group, stage1)
         L.call("ehss_solve_many", self.handle, reps, table, int(power), int(cap), precision_code(precision), flags,
                totals, L.ptr(hist), L.ptr(ledger))
         return self._split(totals, reps, cap, hist, ledger)

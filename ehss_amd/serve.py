@@ -39,9 +39,9 @@ import numpy as np
 # spheres 32, leaf boxes <= 32.
 SCENE_BYTES_PER_ATOM = 24 + 8 + 32 + 4 + 4 + 4 + 128 + 128 + 4 + 64 + 16 + 32 + 32
 SCENE_BYTES_FIXED = 64 * 2 + 4 * 8 * 64 + 4096  # PA root slack, frontier, small buffers
-# Solve workspace per ray: record word 4, contribution 8, queue entry 112; per
+# Solve workspace per ray: record word 4, contribution 8, two queue entries 112; per
 # replicate: scramble + ray map 660, counters 128, histogram + ledger bins 16.
-SOLVE_BYTES_PER_RAY = 4 + 8 + 112
+SOLVE_BYTES_PER_RAY = 4 + 8 + 2 * 112
 SOLVE_BYTES_PER_REPLICATE = 660 + 128
 SOLVE_BYTES_FIXED = 64 * 1024
 

@@ -68,6 +68,11 @@ extern "C" {
  * per lane; with more, each query is split over the ray's lanes. */
 #define EHSS_GROUP_SHIFT 19
 #define EHSS_GROUP_LOG2(k) (((unsigned)(k) + 1u) << EHSS_GROUP_SHIFT)
+/* Variant 4: queries a queued ray gets in the drain before it moves to a second
+ * drain with a whole wave per ray, as q + 1 in bits 22..25 (0 = default,
+ * EHSS_STAGE1(0) = a single drain).  q <= 14. */
+#define EHSS_STAGE1_SHIFT 22
+#define EHSS_STAGE1(q) (((unsigned)(q) + 1u) << EHSS_STAGE1_SHIFT)
 
 /* Library / device. */
 int ehss_abi_version(void);

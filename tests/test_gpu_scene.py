@@ -121,22 +121,25 @@ def test_plain_and_fp32_fixup_kernels_are_bit_identical(oracle, name):
         # coop None: default tail threshold; 0: tail mode off; 64: every wave
         # switches to cooperative queries as soon as it may.  group: lanes per
         # ray in variant 4's drain (1: per-lane traversal).
-        kernels = [("fp64", 0, None, None)] + [("fp32-fixup", v, c, None) for v in (1, 2, 3, 4)
-                                               for c in (None, 0, 64)]
-        kernels += [("fp32-fixup", 4, c, g) for c in (0, 64) for g in (1, 2, 16, 64)]
-        for prec, variant, coop, group in kernels:
-            say("kernels", name, prec, variant, coop, group)
+        # stage1: queries in the first drain before a ray moves to the whole-wave
+        # second drain (0: one drain).
+        kernels = [("fp64", 0, None, None, None)] + [("fp32-fixup", v, c, None, None) for v in (1, 2, 3, 4)
+                                                     for c in (None, 0, 64)]
+        kernels += [("fp32-fixup", 4, c, g, 0) for c in (0, 64) for g in (1, 2, 16, 64)]
+        kernels += [("fp32-fixup", 4, None, g, q) for g, q in ((4, 1), (8, 2), (8, 4), (32, 14))]
+        for prec, variant, coop, group, stage1 in kernels:
+            say("kernels", name, prec, variant, coop, group, stage1)
             res = sc.solve(scramble(7), 16, 64, precision=prec, pa=True, keep_rays=True, histogram=True,
-                           variant=variant, coop=coop, group=group)
-            runs[(prec, variant, coop, group)] = (res, sc.last_rays(1 << 16))
-        a = runs[("fp64", 0, None, None)]
+                           variant=variant, coop=coop, group=group, stage1=stage1)
+            runs[(prec, variant, coop, group, stage1)] = (res, sc.last_rays(1 << 16))
+        a = runs[("fp64", 0, None, None, None)]
         for key, b in runs.items():
             assert a[0]["omega"] == b[0]["omega"] and a[0]["pa"] == b[0]["pa"], key
             np.testing.assert_array_equal(a[1]["contributions"], b[1]["contributions"], err_msg=str(key))
             phys = ~np.uint32(16)  # every flag bit except the kernel-specific grazing diagnostic
             np.testing.assert_array_equal(a[1]["flags"] & phys, b[1]["flags"] & phys, err_msg=str(key))
             np.testing.assert_array_equal(a[0]["histogram"], b[0]["histogram"], err_msg=str(key))
-        b = runs[("fp32-fixup", 2, None, None)]
+        b = runs[("fp32-fixup", 2, None, None, None)]
         # determinism: repeat
         again = sc.solve(scramble(7), 16, 64, precision="fp32-fixup", pa=True)
         assert again["omega"] == b[0]["omega"]
