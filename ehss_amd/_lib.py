@@ -27,6 +27,7 @@ FLAG_PA = 1
 FLAG_KEEP_RAYS = 2
 FLAG_GRAPH = 4
 FLAG_TIMESTAMPS = 8
+FLAG_LDS_PACKETS = 16
 VARIANT_SHIFT = 8
 COOP_SHIFT = 12
 GROUP_SHIFT = 19

@@ -249,18 +249,61 @@ __device__ __forceinline__ double4 load_sphere64(const double4* base, int32_t k)
     return make_double4(f[0], f[1], f[2], f[3]);
 }
 
+// kLds: the view's nodes32, offsets, ids and sph32 point into LDS (a block's
+// staged copy); every lane then reads the same LDS address (a broadcast)
+// instead of issuing a scalar load.
+typedef const __attribute__((address_space(3))) float* LFloat;
+typedef const __attribute__((address_space(3))) int32_t* LInt32;
+
+// The links are made wave-uniform (readfirstlane) so that the traversal's
+// control flow stays scalar, as with constant-address-space loads.
+__device__ __forceinline__ Node32 load_node_lds(const Node32* base, int32_t node) {
+    const LFloat f = (LFloat)(base + node);
+    const LInt32 w = (LInt32)(base + node);
+    Node32 n;
+    n.lo[0] = f[0];
+    n.lo[1] = f[1];
+    n.lo[2] = f[2];
+    n.hi[0] = f[3];
+    n.hi[1] = f[4];
+    n.hi[2] = f[5];
+    n.end = __builtin_amdgcn_readfirstlane(w[6]);
+    n.leaf = __builtin_amdgcn_readfirstlane(w[7]);
+    return n;
+}
+
+__device__ __forceinline__ float4 load_sphere32_lds(const float4* base, int32_t k) {
+    const LFloat f = (LFloat)(base + k);
+    return make_float4(f[0], f[1], f[2], f[3]);
+}
+
+template <int kLds = 0>
 __device__ __forceinline__ void nearest_packet(const SceneView& S, const V3& o, const V3& d, int32_t last,
                                                double tol, bool live, Query& q, uint64_t& boxes,
                                                uint64_t& spheres, uint64_t& retests, PAState* pa = nullptr) {
     query_init(q);
     const Ray32 r = ray32(S, o, d);
-    const CInt32 offsets = (CInt32)S.offsets;
-    const CInt32 ids = (CInt32)S.ids;
+    auto node_at = [&](int32_t i) -> Node32 {
+        if constexpr (kLds) return load_node_lds(S.nodes32, i);
+        else return load_node(S.nodes32, i);
+    };
+    auto offset_at = [&](int32_t g) -> int32_t {
+        if constexpr (kLds) return __builtin_amdgcn_readfirstlane(((LInt32)S.offsets)[g]);
+        else return ((CInt32)S.offsets)[g];
+    };
+    auto id_at = [&](int32_t k) -> int32_t {
+        if constexpr (kLds) return __builtin_amdgcn_readfirstlane(((LInt32)S.ids)[k]);
+        else return ((CInt32)S.ids)[k];
+    };
+    auto sphere_at = [&](int32_t k) -> float4 {
+        if constexpr (kLds) return load_sphere32_lds(S.sph32, k);
+        else return load_sphere32(S.sph32, k);
+    };
     float bestf = INFINITY;
     int32_t node = 0;
     const int32_t nn = S.n_nodes;
     while (node < nn) {
-        const Node32 n = load_node(S.nodes32, node);
+        const Node32 n = node_at(node);
         boxes += live ? 1 : 0;
         const bool hit = live && box32(r, n, bestf);
         if (__ballot(hit) == 0) {
@@ -269,10 +312,10 @@ __device__ __forceinline__ void nearest_packet(const SceneView& S, const V3& o, 
         }
         const int32_t g = n.leaf;
         if (g >= 0) {
-            const int32_t k1 = offsets[g + 1];
-            for (int32_t k = offsets[g]; k < k1; ++k) {
-                const int32_t wanted = ids[k];
-                const float4 s = load_sphere32(S.sph32, k);
+            const int32_t k1 = offset_at(g + 1);
+            for (int32_t k = offset_at(g); k < k1; ++k) {
+                const int32_t wanted = id_at(k);
+                const float4 s = sphere_at(k);
                 if (!hit || wanted == last) continue;
                 ++spheres;
                 if (!needs_exact(S, r, s, q.best, tol, pa)) continue;

@@ -376,7 +376,7 @@ constexpr int kPacketQueries = 2;
 // whose exact FP64 outcome is already decided).
 // The ray with Gray index g of the net, recorded at slot t (valid: the lane
 // has a ray; every lane of the wave must call this).
-template <int kMode, int kPA, class Sob>
+template <int kMode, int kPA, int kLds = 0, class Sob>
 __device__ __forceinline__ void solve_ray_at(const SceneView& S, const SolveArgs& A, Sob sob, bool valid, uint32_t g,
                                              uint32_t t, uint64_t* ctr, RayState* queue, uint32_t* queue_count,
                                              int32_t job) {
@@ -418,7 +418,7 @@ __device__ __forceinline__ void solve_ray_at(const SceneView& S, const SolveArgs
         if (kMode == 1) {
             if (valid) nearest_fast(S, R.o, R.d, -1, R.tol, q, boxes, spheres, retests, pap);
         } else {
-            nearest_packet(S, R.o, R.d, -1, R.tol, valid, q, boxes, spheres, retests, pap);
+            nearest_packet<kLds>(S, R.o, R.d, -1, R.tol, valid, q, boxes, spheres, retests, pap);
         }
         if (valid) {
             if (kPA) {
@@ -450,7 +450,7 @@ __device__ __forceinline__ void solve_ray_at(const SceneView& S, const SolveArgs
                     finish_coop(R, 0, scene_of, args_of, account);
                     break;
                 }
-                nearest_packet(S, R.o, R.d, R.last, R.tol, live, q, boxes, spheres, retests);
+                nearest_packet<kLds>(S, R.o, R.d, R.last, R.tol, live, q, boxes, spheres, retests);
                 if (live) ray_process(S, A.cap, R, q);
             }
             if (kMode == 4) {
@@ -493,13 +493,45 @@ __device__ __forceinline__ void solve_ray_at(const SceneView& S, const SolveArgs
 
 // Thread t of a solve traces the ray the wave-coherent map assigns to it and
 // records it at slot t, so a wave's record writes are contiguous.
-template <int kMode, int kPA, class Sob, class Map>
+template <int kMode, int kPA, int kLds = 0, class Sob, class Map>
 __device__ __forceinline__ void solve_ray(const SceneView& S, const SolveArgs& A, Sob sob, Map map, uint32_t t,
                                           uint64_t* ctr, RayState* queue = nullptr, uint32_t* queue_count = nullptr,
                                           int32_t job = 0) {
     const bool valid = t < (uint64_t)A.n;
     const uint32_t g = valid ? ray_gray(map, t, A.power) : 0u;
-    solve_ray_at<kMode, kPA>(S, A, sob, valid, g, t, ctr, queue, queue_count, job);
+    solve_ray_at<kMode, kPA, kLds>(S, A, sob, valid, g, t, ctr, queue, queue_count, job);
+}
+
+// Copies the scene arrays the FP32 filter reads (tree, leaf boxes, leaf
+// offsets, slot ids, FP32 spheres) into the block's LDS and returns the view
+// pointing there.  All threads of the block call it.
+__device__ __forceinline__ SceneView stage_scene(const SceneView& S, unsigned char* lds) {
+    Node32* nodes = reinterpret_cast<Node32*>(lds);
+    Node32* leaves = nodes + S.n_nodes;
+    float4* sph = reinterpret_cast<float4*>(leaves + S.n_groups);
+    int32_t* offsets = reinterpret_cast<int32_t*>(sph + S.n_atoms);
+    int32_t* ids = offsets + (S.n_groups + 1);
+    for (int32_t k = threadIdx.x; k < S.n_nodes; k += blockDim.x) nodes[k] = S.nodes32[k];
+    for (int32_t k = threadIdx.x; k < S.n_groups; k += blockDim.x) leaves[k] = S.leaves32[k];
+    for (int32_t k = threadIdx.x; k < S.n_atoms; k += blockDim.x) {
+        sph[k] = S.sph32[k];
+        ids[k] = S.ids[k];
+    }
+    for (int32_t k = threadIdx.x; k <= S.n_groups; k += blockDim.x) offsets[k] = S.offsets[k];
+    __syncthreads();
+    SceneView L = S;
+    L.nodes32 = nodes;
+    L.leaves32 = leaves;
+    L.sph32 = sph;
+    L.offsets = offsets;
+    L.ids = ids;
+    return L;
+}
+
+// LDS bytes stage_scene uses for a scene.
+__host__ __device__ inline size_t staged_scene_bytes(const SceneView& v) {
+    return sizeof(Node32) * (v.n_nodes + v.n_groups) + sizeof(float4) * v.n_atoms +
+           sizeof(int32_t) * (v.n_groups + 1) + sizeof(int32_t) * v.n_atoms;
 }
 
 // Persistent per-lane drain of queued rays.  Idle lanes refill from the queue

@@ -30,14 +30,17 @@ __device__ __forceinline__ SolveArgs replicate_args(const SolveArgs& A, int32_t 
 
 // One thread per (ray slot, replicate): blockIdx.y is the replicate, whose
 // scrambled net and ray map are read (scalar loads) from device tables.
-template <int kMode, int kPA>
+// kLds: the block first stages the scene's filter arrays in LDS (stage_scene).
+template <int kMode, int kPA, int kLds>
 __global__ void __launch_bounds__(256) k_solve(SceneView S, SolveArgs A, const ehss_sobol* sobs, const RayMap* maps,
                                                RayState* queue, uint32_t* queue_count) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const SceneView V = kLds ? stage_scene(S, lds) : S;
     const int32_t r = (int32_t)blockIdx.y;
     const SolveArgs B = replicate_args(A, r);
     uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
-    solve_ray<kMode, kPA>(S, B, (CSobol)(sobs + r), (CMap)(maps + r), blockIdx.x * blockDim.x + threadIdx.x, ctr,
-                          queue, queue_count, r);
+    solve_ray<kMode, kPA, kLds>(V, B, (CSobol)(sobs + r), (CMap)(maps + r), blockIdx.x * blockDim.x + threadIdx.x,
+                                ctr, queue, queue_count, r);
     flush_counters(ctr, B.counters);
 }
 
@@ -49,9 +52,11 @@ __global__ void __launch_bounds__(256) k_solve(SceneView S, SolveArgs A, const e
 // replicate, once per lane.
 constexpr int kCrossCells = 1;
 
-template <int kMode, int kPA>
-__global__ void __launch_bounds__(256) k_solve_x(SceneView S, SolveArgs A, CrossTables X, int32_t reps,
+template <int kMode, int kPA, int kLds>
+__global__ void __launch_bounds__(256) k_solve_x(SceneView S0, SolveArgs A, CrossTables X, int32_t reps,
                                                  int32_t groups, RayState* queue, uint32_t* queue_count) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const SceneView S = kLds ? stage_scene(S0, lds) : S0;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const uint32_t c0 = (uint32_t)(wave / groups) * kCrossCells;
@@ -72,7 +77,7 @@ __global__ void __launch_bounds__(256) k_solve_x(SceneView S, SolveArgs A, Cross
             for (int d = 0; d < 4; ++d) p.q[d] ^= X.col[((int64_t)k * 4 + d) * X.rp + r];
         }
         const uint32_t t = m >= 6 ? (((u & ((1u << (m - 6)) - 1u)) << 6) | (u >> (m - 6))) : u;
-        solve_ray_at<kMode, kPA>(S, B, p, valid, 0u, t, ctr, queue, queue_count, rr);
+        solve_ray_at<kMode, kPA, kLds>(S, B, p, valid, 0u, t, ctr, queue, queue_count, rr);
     }
     if (r < reps)
         for (int k = 0; k < 6; ++k)
@@ -93,27 +98,7 @@ __global__ void __launch_bounds__(256) k_drain(SceneView S, SolveArgs A, int32_t
     uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
     const uint32_t len = __builtin_amdgcn_readfirstlane(*queue_count);
     if (len == 0) return;
-    SceneView L = S;
-    if (kLds) {
-        Node32* nodes = reinterpret_cast<Node32*>(lds);
-        Node32* leaves = nodes + S.n_nodes;
-        float4* sph = reinterpret_cast<float4*>(leaves + S.n_groups);
-        int32_t* offsets = reinterpret_cast<int32_t*>(sph + S.n_atoms);
-        int32_t* ids = offsets + (S.n_groups + 1);
-        for (int32_t k = threadIdx.x; k < S.n_nodes; k += blockDim.x) nodes[k] = S.nodes32[k];
-        for (int32_t k = threadIdx.x; k < S.n_groups; k += blockDim.x) leaves[k] = S.leaves32[k];
-        for (int32_t k = threadIdx.x; k < S.n_atoms; k += blockDim.x) {
-            sph[k] = S.sph32[k];
-            ids[k] = S.ids[k];
-        }
-        for (int32_t k = threadIdx.x; k <= S.n_groups; k += blockDim.x) offsets[k] = S.offsets[k];
-        __syncthreads();
-        L.nodes32 = nodes;
-        L.leaves32 = leaves;
-        L.sph32 = sph;
-        L.offsets = offsets;
-        L.ids = ids;
-    }
+    const SceneView L = kLds ? stage_scene(S, lds) : S;
     auto scene_of = [&](int32_t) -> const SceneView& { return L; };
     auto args_of = [&](int32_t j) -> SolveArgs { return replicate_args(A, j); };
     // One replicate: per-lane sums flushed once; several: per-ray atomics into
@@ -137,11 +122,8 @@ __global__ void __launch_bounds__(256) k_drain(SceneView S, SolveArgs A, int32_t
     if (replicates == 1) flush_counters(ctr, A.counters);
 }
 
-// LDS bytes of a scene staged by k_drain<1>.
-size_t drain_lds_bytes(const SceneView& v) {
-    return sizeof(Node32) * (v.n_nodes + v.n_groups) + sizeof(float4) * v.n_atoms +
-           sizeof(int32_t) * (v.n_groups + 1) + sizeof(int32_t) * v.n_atoms;
-}
+// LDS bytes of a scene staged by the kLds kernels.
+size_t drain_lds_bytes(const SceneView& v) { return staged_scene_bytes(v); }
 
 __global__ void k_zero(unsigned long long* p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -453,6 +435,7 @@ constexpr int kDefaultVariant = 4;
 constexpr int kMaxReplicates = 4096;
 constexpr int kCrossMinReps = 16;  // replicates per launch from which they are scheduled across lanes
 constexpr size_t kDrainLdsLimit = 144 * 1024;  // of the 160 KB per CU
+constexpr size_t kSolveLdsLimit = 48 * 1024;   // packet kernels: keeps several blocks per CU
 
 // Ray index of record slot t (host copy of the device mapping: Gray index from
 // the ray map, then inverse Gray code).
@@ -579,18 +562,28 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     EHSS_HIP(hipEventRecord(sc->ev0, s));
     const int64_t cell_waves = (n + kCrossCells - 1) / kCrossCells;
     const dim3 xgrid((unsigned)((cell_waves * groups * 64 + 255) / 256));
+    // Packet variants stage the scene in LDS on request (EHSS_FLAG_LDS_PACKETS).
+    const size_t stage_bytes = staged_scene_bytes(sc->view);
+    const bool lds_packets = (variant >= 2) && stage_bytes <= kSolveLdsLimit && (flags & EHSS_FLAG_LDS_PACKETS);
+    const size_t solve_lds = lds_packets ? stage_bytes : 0;
+#define EHSS_LAUNCH_L(M, P, K)                                                                               \
+    do {                                                                                                     \
+        if (cross)                                                                                           \
+            hipLaunchKernelGGL((k_solve_x<M, P, K>), xgrid, block, solve_lds, s, sc->view, A, XT, reps, groups,   \
+                               qbuf, qcount);                                                                \
+        else                                                                                                 \
+            hipLaunchKernelGGL((k_solve<M, P, K>), grid, block, solve_lds, s, sc->view, A, d_sob, d_map, qbuf,    \
+                               qcount);                                                                      \
+    } while (0)
 #define EHSS_LAUNCH(M)                                                                                       \
     do {                                                                                                     \
-        if (cross && pa)                                                                                     \
-            hipLaunchKernelGGL((k_solve_x<M, 1>), xgrid, block, 0, s, sc->view, A, XT, reps, groups,             \
-                               qbuf, qcount);                                                                \
-        else if (cross)                                                                                      \
-            hipLaunchKernelGGL((k_solve_x<M, 0>), xgrid, block, 0, s, sc->view, A, XT, reps, groups,             \
-                               qbuf, qcount);                                                                \
-        else if (pa)                                                                                         \
-            hipLaunchKernelGGL((k_solve<M, 1>), grid, block, 0, s, sc->view, A, d_sob, d_map, qbuf, qcount);     \
-        else                                                                                                 \
-            hipLaunchKernelGGL((k_solve<M, 0>), grid, block, 0, s, sc->view, A, d_sob, d_map, qbuf, qcount);     \
+        if (lds_packets) {                                                                                   \
+            if (pa) EHSS_LAUNCH_L(M, 1, 1);                                                                  \
+            else EHSS_LAUNCH_L(M, 0, 1);                                                                     \
+        } else {                                                                                             \
+            if (pa) EHSS_LAUNCH_L(M, 1, 0);                                                                  \
+            else EHSS_LAUNCH_L(M, 0, 0);                                                                     \
+        }                                                                                                    \
     } while (0)
     switch (variant) {
         case 0: EHSS_LAUNCH(0); break;
@@ -600,6 +593,7 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
         default: EHSS_LAUNCH(4); break;
     }
 #undef EHSS_LAUNCH
+#undef EHSS_LAUNCH_L
     if (variant == 4) {
         const size_t lds = drain_lds_bytes(sc->view);
         const dim3 dgrid((unsigned)drain_blocks(total));

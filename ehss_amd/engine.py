@@ -241,7 +241,8 @@ class GPUScene:
     # -- fused device solve ---------------------------------------------------------
     def solve(self, sobol: L.Sobol, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
               keep_rays: bool = False, histogram: bool = False, variant: int = 0, coop: int | None = None,
-              timestamps: bool = False, group: int | None = None, stage1: int | None = None) -> dict:
+              timestamps: bool = False, group: int | None = None, stage1: int | None = None,
+              lds: bool = False) -> dict:
         """One fused device solve of 2^power rays.  `variant` selects the FP32
         filter kernel (0 default, 1 per-lane, 2 packet first query, 3 packet
         throughout, 4 packet first queries then queue + persistent drain);
@@ -254,6 +255,7 @@ class GPUScene:
         ledger = np.zeros(cap + 1, np.float64) if histogram else None
         flags = (L.FLAG_PA if pa else 0) | (L.FLAG_KEEP_RAYS if keep_rays else 0) | (int(variant) << L.VARIANT_SHIFT)
         flags |= coop_flags(coop, group, stage1) | (L.FLAG_TIMESTAMPS if timestamps else 0)
+        flags |= L.FLAG_LDS_PACKETS if lds else 0
         L.call("ehss_solve", self.handle, ct.byref(sobol), int(power), int(cap), precision_code(precision),
                flags, ct.byref(totals), L.ptr(hist), L.ptr(ledger))
         out = totals.as_dict()
@@ -264,7 +266,7 @@ class GPUScene:
 
     def solve_many(self, sobols, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
                    histogram: bool = False, variant: int = 0, coop: int | None = None, group: int | None = None,
-                   keep_rays: bool = False, stage1: int | None = None) -> list[dict]:
+                   keep_rays: bool = False, stage1: int | None = None, lds: bool = False) -> list[dict]:
         """len(sobols) independent solves in one launch; one result dict per
         scramble, each equal to solve() with that scramble."""
         reps = len(sobols)
@@ -273,7 +275,7 @@ class GPUScene:
         hist = np.zeros(reps * (cap + 1), np.int64) if histogram else None
         ledger = np.zeros(reps * (cap + 1), np.float64) if histogram else None
         flags = (L.FLAG_PA if pa else 0) | (L.FLAG_KEEP_RAYS if keep_rays else 0) | \
-            (int(variant) << L.VARIANT_SHIFT) | coop_flags(coop, group, stage1)
+            (int(variant) << L.VARIANT_SHIFT) | coop_flags(coop, group, stage1) | (L.FLAG_LDS_PACKETS if lds else 0)
         L.call("ehss_solve_many", self.handle, reps, table, int(power), int(cap), precision_code(precision), flags,
                totals, L.ptr(hist), L.ptr(ledger))
         return self._split(totals, reps, cap, hist, ledger)

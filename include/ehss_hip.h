@@ -51,6 +51,8 @@ extern "C" {
 #define EHSS_FLAG_KEEP_RAYS 2u /* keep per-ray records for ehss_scene_last_rays */
 #define EHSS_FLAG_GRAPH 4u     /* batch: capture the launch sequence in a hipGraph and replay it */
 #define EHSS_FLAG_TIMESTAMPS 8u /* record each ray's completion time (ehss_scene_last_ray_times) */
+#define EHSS_FLAG_LDS_PACKETS 16u /* packet kernels stage the scene in LDS first (measured: not faster than
+                                     scalar loads through the constant cache; kept for measurement) */
 /* Kernel variant of the FP32 filter path in flag bits 8..11 (0 = default):
  * 1 one ray per lane, 2 wave-uniform first query then per-lane bounces,
  * 3 wave-uniform traversal for every bounce, 4 wave-uniform first queries, then

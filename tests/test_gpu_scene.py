@@ -132,6 +132,10 @@ def test_plain_and_fp32_fixup_kernels_are_bit_identical(oracle, name):
             res = sc.solve(scramble(7), 16, 64, precision=prec, pa=True, keep_rays=True, histogram=True,
                            variant=variant, coop=coop, group=group, stage1=stage1)
             runs[(prec, variant, coop, group, stage1)] = (res, sc.last_rays(1 << 16))
+        for variant in (3, 4):  # packet phase reading an LDS-staged scene
+            say("kernels", name, "lds packets", variant)
+            res = sc.solve(scramble(7), 16, 64, pa=True, keep_rays=True, histogram=True, variant=variant, lds=True)
+            runs[("fp32-fixup", variant, "lds", None, None)] = (res, sc.last_rays(1 << 16))
         a = runs[("fp64", 0, None, None, None)]
         for key, b in runs.items():
             assert a[0]["omega"] == b[0]["omega"] and a[0]["pa"] == b[0]["pa"], key
