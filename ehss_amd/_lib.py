@@ -127,6 +127,12 @@ _SIGS = {
     "ehss_batch_query": (_INT, [_P, ct.POINTER(_INT)]),
     "ehss_batch_wait": (_INT, [_P, _P, _P, _P]),
     "ehss_batch_destroy": (_INT, [_P]),
+    "ehss_plan_create": (_INT, [_INT, _P, _P, _P, _P, _INT, ct.c_uint, ct.POINTER(_P)]),
+    "ehss_plan_set_sobol": (_INT, [_P, _INT, ct.POINTER(Sobol)]),
+    "ehss_plan_launch": (_INT, [_P]),
+    "ehss_plan_query": (_INT, [_P, ct.POINTER(_INT)]),
+    "ehss_plan_wait": (_INT, [_P, _P, _P, _P]),
+    "ehss_plan_destroy": (_INT, [_P]),
 }
 
 _lib = None

@@ -159,6 +159,7 @@ struct RayRecords {
 constexpr uint32_t kBounceMask = 0xFFFFFu;
 
 int64_t drain_blocks(int64_t n);
+int launch_zero(unsigned long long* p, int64_t n, hipStream_t s);
 
 // Wave-coherent ray map columns for a scrambled net of 2^m points (host).
 void make_ray_map(const ehss_sobol& sob, int m, uint32_t cols[32]);

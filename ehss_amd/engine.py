@@ -348,52 +348,44 @@ class GPUScene:
         return dict(u=u, origins=o, directions=d)
 
 
-class GPUBatch:
-    """Many fused solves -- different scenes, scrambles, ray counts and caps -- in
-    one persistent launch (ehss_batch_*).  Each job's totals are bit-identical to
-    the same solve run alone (GPUScene.solve): every ray writes its own record
-    and each job is reduced in its own fixed order.
+class _MultiSolve:
+    """Shared wrapper of the multi-structure entry points (ehss_<prefix>_*):
+    launch() is asynchronous, done() polls, wait() collects per-job results."""
 
-    With graph=True the launch sequence is captured once into a hipGraph and
-    replayed by later launches; set_sobol() changes a job's scramble between
-    launches (graph or not).  launch() is asynchronous; done() polls and wait()
-    collects the per-job results."""
+    _prefix = ""
 
-    def __init__(self, scenes, sobols, powers, caps, precision="fp32-fixup", pa: bool = True, graph: bool = False,
-                 variant: int = 0, coop: int | None = None, group: int | None = None, blocks_per_cu: int = 0):
+    def _create(self, scenes, sobols, powers, caps, flags: int, precision, *extra):
         n = len(scenes)
         if n == 0 or not (len(sobols) == len(powers) == len(caps) == n):
             raise ValueError("one scene, scramble, power and cap per job required")
-        self.scenes = list(scenes)  # keep the scenes alive while the batch exists
+        self.scenes = list(scenes)  # keep the scenes alive while the handle exists
         self.caps = [int(c) for c in caps]
         self.powers = [int(p) for p in powers]
         handles = (ct.c_void_p * n)(*[s.handle for s in self.scenes])
         sob = (L.Sobol * n)(*sobols)
         pw = np.asarray(self.powers, np.int32)
         cp = np.asarray(self.caps, np.int32)
-        flags = (L.FLAG_PA if pa else 0) | (L.FLAG_GRAPH if graph else 0) | (int(variant) << L.VARIANT_SHIFT)
-        flags |= coop_flags(coop, group)
         h = ct.c_void_p()
-        L.call("ehss_batch_create", n, handles, sob, L.ptr(pw), L.ptr(cp), precision_code(precision), flags,
-               int(blocks_per_cu), ct.byref(h))
+        L.call(f"ehss_{self._prefix}_create", n, handles, sob, L.ptr(pw), L.ptr(cp), precision_code(precision),
+               flags, *extra, ct.byref(h))
         self._h = h
         self.n_jobs = n
 
     @property
     def handle(self):
         if not getattr(self, "_h", None):
-            raise RuntimeError("batch is closed")
+            raise RuntimeError(f"{self._prefix} is closed")
         return self._h
 
     def set_sobol(self, job: int, sobol: L.Sobol) -> None:
-        L.call("ehss_batch_set_sobol", self.handle, int(job), ct.byref(sobol))
+        L.call(f"ehss_{self._prefix}_set_sobol", self.handle, int(job), ct.byref(sobol))
 
     def launch(self) -> None:
-        L.call("ehss_batch_launch", self.handle)
+        L.call(f"ehss_{self._prefix}_launch", self.handle)
 
     def done(self) -> bool:
         flag = ct.c_int(0)
-        L.call("ehss_batch_query", self.handle, ct.byref(flag))
+        L.call(f"ehss_{self._prefix}_query", self.handle, ct.byref(flag))
         return bool(flag.value)
 
     def wait(self, histogram: bool = False) -> list[dict]:
@@ -401,7 +393,7 @@ class GPUBatch:
         bins = sum(c + 1 for c in self.caps)
         hist = np.zeros(bins, np.int64) if histogram else None
         ledger = np.zeros(bins, np.float64) if histogram else None
-        L.call("ehss_batch_wait", self.handle, totals, L.ptr(hist), L.ptr(ledger))
+        L.call(f"ehss_{self._prefix}_wait", self.handle, totals, L.ptr(hist), L.ptr(ledger))
         out, off = [], 0
         for j in range(self.n_jobs):
             d = totals[j].as_dict()
@@ -418,7 +410,7 @@ class GPUBatch:
 
     def close(self):
         if getattr(self, "_h", None):
-            L.fn("ehss_batch_destroy")(self._h)
+            L.fn(f"ehss_{self._prefix}_destroy")(self._h)
             self._h = None
 
     def __enter__(self):
@@ -432,3 +424,36 @@ class GPUBatch:
             self.close()
         except Exception:
             pass
+
+
+class GPUPlan(_MultiSolve):
+    """Many fused solves -- different scenes, scrambles, ray counts and caps -- as
+    one prepared launch sequence (ehss_plan_*): a packet grid over (ray block,
+    job), one drain queue shared by every job, per-job fixed-order reductions.
+    Each job's totals are bit-identical to the same solve run alone
+    (GPUScene.solve with the same variant/coop/group/stage1 settings).
+
+    With graph=True the sequence is captured once into a hipGraph and replayed
+    by later launches; set_sobol() changes a job's scramble between launches
+    (graph or not)."""
+
+    _prefix = "plan"
+
+    def __init__(self, scenes, sobols, powers, caps, precision="fp32-fixup", pa: bool = True, graph: bool = False,
+                 variant: int = 0, coop: int | None = None, group: int | None = None, stage1: int | None = None):
+        flags = (L.FLAG_PA if pa else 0) | (L.FLAG_GRAPH if graph else 0) | (int(variant) << L.VARIANT_SHIFT)
+        flags |= coop_flags(coop, group, stage1)
+        self._create(scenes, sobols, powers, caps, flags, precision)
+
+
+class GPUBatch(_MultiSolve):
+    """The persistent-kernel form of GPUPlan (ehss_batch_*); the library
+    currently refuses to create it (see ehss_batch_create)."""
+
+    _prefix = "batch"
+
+    def __init__(self, scenes, sobols, powers, caps, precision="fp32-fixup", pa: bool = True, graph: bool = False,
+                 variant: int = 0, coop: int | None = None, group: int | None = None, blocks_per_cu: int = 0):
+        flags = (L.FLAG_PA if pa else 0) | (L.FLAG_GRAPH if graph else 0) | (int(variant) << L.VARIANT_SHIFT)
+        flags |= coop_flags(coop, group)
+        self._create(scenes, sobols, powers, caps, flags, precision, int(blocks_per_cu))

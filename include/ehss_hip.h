@@ -19,8 +19,9 @@
  *   ehss_solve               one replicate of solve_frame (ccs_cli.py:196-219):
  *                            external_source (ehss_state.py:42-52) -> index.query ->
  *                            engine.trace -> EHSSResult readout (ehss_state.py:84-103).
- *   ehss_batch_*             no single reference counterpart: many solve_frame
- *                            replicates over many structures in one launch.
+ *   ehss_batch_*, ehss_plan_*  no single reference counterpart: the per-frame
+ *                            solve_frame loop (ccs_cli.py:305-307) over many
+ *                            structures as one launch.
  */
 #ifndef EHSS_HIP_H
 #define EHSS_HIP_H
@@ -279,6 +280,37 @@ int ehss_batch_query(ehss_batch* batch, int* done);
  * time of the whole batch. */
 int ehss_batch_wait(ehss_batch* batch, ehss_totals* totals, int64_t* histograms, double* ledgers);
 int ehss_batch_destroy(ehss_batch* batch);
+
+/* ---------------------------------------------------------------------------
+ * Prepared multi-structure plans: many (scene, scramble, 2^power rays, cap)
+ * jobs run as one launch sequence on the plan's own stream -- a 2-D packet grid
+ * (blockIdx.y = job), one shared drain queue for the rays still bouncing after
+ * the packet queries, and per-job fixed-order reductions -- optionally captured
+ * once into a hipGraph (EHSS_FLAG_GRAPH) and replayed.  Each job's totals,
+ * histogram and ledger equal those of ehss_solve on the same scene, scramble,
+ * power, cap, precision and flags.  Replaces, for a set of structures, the
+ * per-frame loop of the CLI (ccs_cli.py:305-307, one solve_frame per structure,
+ * ccs_cli.py:170).  Scenes must outlive the plan and must have no solve pending
+ * while a plan launch runs.
+ * ------------------------------------------------------------------------- */
+typedef struct ehss_plan ehss_plan;
+
+/* flags: EHSS_FLAG_PA, EHSS_FLAG_GRAPH, EHSS_VARIANT(v), EHSS_COOP(n),
+ * EHSS_GROUP_LOG2(k), EHSS_STAGE1(q); powers 0..26 each, at most 2^30 rays and
+ * 65535 jobs per plan. */
+int ehss_plan_create(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols, const int32_t* powers,
+                     const int32_t* caps, int precision, unsigned flags, ehss_plan** out);
+/* Replaces the scramble of one job (used by the next launch, graph or not). */
+int ehss_plan_set_sobol(ehss_plan* plan, int job, const ehss_sobol* sobol);
+/* Asynchronous: enqueues the plan on its stream and returns. */
+int ehss_plan_launch(ehss_plan* plan);
+/* *done = 1 once the last launch has finished (never blocks). */
+int ehss_plan_query(ehss_plan* plan, int* done);
+/* Blocks until done; totals: n_jobs entries; histograms / ledgers (may be NULL):
+ * concatenated (cap_j + 1)-bin arrays in job order.  kernel_ms is the device
+ * time of the whole plan. */
+int ehss_plan_wait(ehss_plan* plan, ehss_totals* totals, int64_t* histograms, double* ledgers);
+int ehss_plan_destroy(ehss_plan* plan);
 
 #ifdef __cplusplus
 }

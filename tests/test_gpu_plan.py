@@ -1,0 +1,139 @@
+"""Prepared multi-structure plans (ehss_plan_*): many structures of uneven size,
+scrambles, ray counts and caps as one launch sequence.  Every job's physical
+totals, histogram and ledger must equal the same solve run alone, bit for bit,
+in every kernel variant and drain schedule, with and without hipGraph replay."""
+import numpy as np
+import pytest
+
+from ehss_amd.engine import GPUPlan, GPUScene, scramble
+from tests import scenes
+from tests.gpu_helpers import say
+
+pytestmark = pytest.mark.gpu
+
+# Totals that are physical results (work counters and the kernel-specific
+# grazing diagnostic excluded).
+PHYSICAL = ("omega", "pa", "pa_from_ehss", "residual_mass", "tail_bound", "weight", "rays", "escaped",
+            "unresolved", "pa_hits", "ehss_hits", "max_bounces")
+
+
+def _jobs():
+    """(scene arrays, power, cap): uneven sizes, one job smaller than a wave."""
+    return [
+        (scenes.globule(650, 1), 14, 64),
+        (scenes.SCENES["c60"](), 12, 64),
+        (scenes.SCENES["f29"](), 10, 32),
+        (scenes.SCENES["cross6"](), 8, 16),
+        ((np.zeros((1, 3)), np.array([3.1])), 5, 8),
+        (scenes.globule(300, 4), 13, 128),
+    ]
+
+
+def _same(alone, planned, key):
+    for k in PHYSICAL:
+        assert alone[k] == planned[k], (key, k, alone[k], planned[k])
+    np.testing.assert_array_equal(alone["histogram"], planned["histogram"], err_msg=str(key))
+    np.testing.assert_array_equal(alone["ledger"], planned["ledger"], err_msg=str(key))
+
+
+CASES = [
+    ("fp32-fixup", 4, None, None),  # library defaults: one drain, whole waves per ray at these sizes
+    ("fp32-fixup", 4, 4, 2),        # two-stage drain: 4 lanes per ray, spill after 2 queries
+    ("fp32-fixup", 4, 1, None),     # per-lane drain with cooperative tail
+    ("fp32-fixup", 3, None, None),
+    ("fp32-fixup", 2, None, None),
+    ("fp32-fixup", 1, None, None),
+    ("fp64", 0, None, None),
+]
+
+
+@pytest.mark.parametrize("precision,variant,group,stage1", CASES)
+def test_plan_equals_solves_alone(precision, variant, group, stage1):
+    jobs = _jobs()
+    sc = [GPUScene(c, r) for (c, r), _, _ in jobs]
+    sobols = [scramble(11 + j) for j in range(len(jobs))]
+    powers = [p for _, p, _ in jobs]
+    caps = [c for _, _, c in jobs]
+    try:
+        say("plan alone", precision, variant, group, stage1)
+        alone = [s.solve(sob, p, c, precision=precision, pa=True, histogram=True, variant=variant)
+                 for s, sob, p, c in zip(sc, sobols, powers, caps)]
+        say("plan", precision, variant, group, stage1)
+        with GPUPlan(sc, sobols, powers, caps, precision=precision, pa=True, variant=variant, group=group,
+                     stage1=stage1) as plan:
+            got = plan.run(histogram=True)
+            again = plan.run(histogram=True)
+        for j, (a, g, h) in enumerate(zip(alone, got, again)):
+            _same(a, g, (precision, variant, group, stage1, j))
+            _same(a, h, (precision, variant, group, stage1, j, "relaunch"))
+            assert g["histogram"].sum() == 1 << powers[j]
+    finally:
+        for s in sc:
+            s.close()
+
+
+def test_plan_default_two_stage_drain_at_size():
+    """2^18 rays and more in one plan select the two-stage drain by default."""
+    jobs = [(scenes.globule(650, 1), 17, 64), (scenes.globule(650, 2), 17, 64), (scenes.SCENES["c60"](), 12, 64)]
+    sc = [GPUScene(c, r) for (c, r), _, _ in jobs]
+    sobols = [scramble(7 + j) for j in range(len(jobs))]
+    powers = [p for _, p, _ in jobs]
+    caps = [c for _, _, c in jobs]
+    try:
+        say("plan two-stage alone")
+        alone = [s.solve(sob, p, c, pa=True, histogram=True) for s, sob, p, c in zip(sc, sobols, powers, caps)]
+        say("plan two-stage")
+        with GPUPlan(sc, sobols, powers, caps, pa=True) as plan:
+            got = plan.run(histogram=True)
+        for j, (a, g) in enumerate(zip(alone, got)):
+            _same(a, g, ("two-stage", j))
+    finally:
+        for s in sc:
+            s.close()
+
+
+def test_plan_graph_replays_follow_new_scrambles():
+    """Capture once, replay with a different scramble per launch; eager solves
+    and host synchronisation between replays; every replay equals the solves
+    alone for its scrambles."""
+    jobs = _jobs()[:5]
+    sc = [GPUScene(c, r) for (c, r), _, _ in jobs]
+    powers = [p for _, p, _ in jobs]
+    caps = [c for _, _, c in jobs]
+    try:
+        with GPUPlan(sc, [scramble(100 + j) for j in range(len(jobs))], powers, caps, graph=True) as plan:
+            for rep in range(4):
+                sobols = [scramble(1000 * rep + j) for j in range(len(jobs))]
+                for j, s in enumerate(sobols):
+                    plan.set_sobol(j, s)
+                say("plan graph replay", rep)
+                plan.launch()
+                got = plan.wait(histogram=True)
+                alone = [s.solve(sob, p, c, pa=True, histogram=True)
+                         for s, sob, p, c in zip(sc, sobols, powers, caps)]
+                for j, (a, g) in enumerate(zip(alone, got)):
+                    _same(a, g, ("graph", rep, j))
+    finally:
+        for s in sc:
+            s.close()
+
+
+def test_plan_async_query_and_argument_checks():
+    c, r = scenes.SCENES["c60"]()
+    with GPUScene(c, r) as s:
+        say("plan async")
+        with GPUPlan([s, s], [scramble(1), scramble(2)], [12, 12], [64, 64]) as plan:
+            plan.launch()
+            res = plan.wait()
+            assert plan.done()
+            assert res[0]["omega"] != res[1]["omega"]
+            assert res[0]["omega"] == s.solve(scramble(1), 12, 64)["omega"]
+            assert res[1]["omega"] == s.solve(scramble(2), 12, 64)["omega"]
+        with pytest.raises(Exception, match="power"):
+            GPUPlan([s], [scramble(1)], [27], [64])
+        with pytest.raises(Exception, match="cap"):
+            GPUPlan([s], [scramble(1)], [10], [0])
+        with pytest.raises(ValueError):
+            GPUPlan([s], [scramble(1)], [10, 11], [64])
+        with pytest.raises(Exception, match="2\\^30"):
+            GPUPlan([s] * 17, [scramble(1)] * 17, [26] * 17, [64] * 17)
