@@ -12,7 +12,8 @@ Side measurements (not part of `value`): the plain FP64 kernel on the same
 workload, CCS/s at 1 % relative RMSE (single stream, and with overlapped
 solves), a Poisson serving stream (p50/p99 latency), the 2000-atom ellipsoid at
 2^22 rays / cap 128 (FP32 error, re-test fraction, unresolved bound), the C60
-case, and the reference CPU kernel on the host cores.
+case, 18 structures of 200-5000 atoms at 2^20 rays each as one autotuned plan
+replayed from a hipGraph, and the reference CPU kernel on the host cores.
 
 Prints one JSON line.  --traffic-probe runs only a fixed copy of known size and
 a few solves (the command profiled for HBM traffic).
@@ -116,6 +117,63 @@ def ccs_rate_replicated(scene, sobols, power, precision, per_launch):
     for k in range(0, len(sobols), per_launch):
         count += len(scene.solve_many(sobols[k:k + per_launch], power, CAP, precision=precision, pa=False))
     return count / (time.perf_counter() - t)
+
+
+def batched18(molecules, scramble, replicate_seed, power=20, cap=64, launches=10):
+    """18 globules (log-uniform 200..5000 atoms, seeds 3..20), 2^power rays each.
+    Times: the 18 solves one after another (default settings), the plain FP64
+    kernel as one plan, the default plan, and the autotuned plan replayed from a
+    hipGraph with new scrambles every launch."""
+    import tempfile
+
+    from ehss_amd import autotune
+    from ehss_amd.engine import GPUPlan, GPUScene
+
+    structs = molecules.batch_set()
+    jobs = len(structs)
+    rays = jobs << power
+
+    def scrambles(k):
+        return [scramble(replicate_seed(3 + j, k)) for j in range(jobs)]
+
+    def plan_rate(plan, count):
+        plan.run()
+        t0 = time.perf_counter()
+        for k in range(count):
+            for j, sob in enumerate(scrambles(k + 1)):
+                plan.set_sobol(j, sob)
+            plan.run()
+        return count * rays / (time.perf_counter() - t0)
+
+    with tempfile.TemporaryDirectory() as tmp:
+        tuned, scs, kw = autotune.tune_plan(structs, power, cap, scrambles(0), repeats=2, passes=2,
+                                            cache_path=Path(tmp) / "tune.json")
+    try:
+        t0 = time.perf_counter()
+        for k in range(2):
+            for s, sob in zip(scs, scrambles(k)):
+                s.solve(sob, power, cap, pa=True)
+        seq = 2 * rays / (time.perf_counter() - t0)
+        with GPUPlan(scs, scrambles(0), [power] * jobs, [cap] * jobs, precision="fp64", graph=True) as plan:
+            plain = plan_rate(plan, 2)
+        with GPUPlan(scs, scrambles(0), [power] * jobs, [cap] * jobs, graph=True) as plan:
+            default = plan_rate(plan, launches)
+        with GPUPlan(scs, scrambles(0), [power] * jobs, [cap] * jobs, graph=True, **kw) as plan:
+            best = plan_rate(plan, launches)
+            res = plan.run()
+        return dict(
+            structures=jobs, atoms=[len(r) for _, r in structs], rays_per_structure=1 << power, cap=cap,
+            rays_per_launch=rays, tuned_config=tuned.config,
+            tuner_trials=[dict(config=c, seconds=round(t, 5)) for c, t in tuned.trials],
+            tuner_speedup_vs_default=tuned.speedup,
+            rays_per_s_sequential_solves=seq, rays_per_s_plain_fp64_plan=plain,
+            rays_per_s_default_plan_graph=default, rays_per_s_tuned_plan_graph=best,
+            tuned_plan_vs_plain_fp64_plan=best / plain, tuned_plan_vs_sequential=best / seq,
+            ms_per_launch=1e3 * rays / best, omega_A2=[r["omega"] for r in res],
+            unresolved_rays=int(sum(r["unresolved"] for r in res)))
+    finally:
+        for s in scs:
+            s.close()
 
 
 def cpu_baseline(centers, radii, sample_power=21):
@@ -288,6 +346,9 @@ def main():
         extras["c60"] = dict(omega_mean=float(np.mean([v["omega"] for v in vals])),
                              pa_mean=float(np.mean([v["pa"] for v in vals])),
                              ms_per_repeat=float(np.mean([v["kernel_ms"] for v in vals])))
+
+        say("C4: 18 structures x 2^20 rays, autotuned plan under hipGraph")
+        extras["batched18"] = batched18(molecules, scramble, replicate_seed)
 
         say("serving stream")
         rng = np.random.default_rng(2024)

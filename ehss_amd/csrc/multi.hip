@@ -35,17 +35,20 @@ struct PlanJob {
 
 namespace {
 
-template <int kMode, int kPA>
-__global__ void __launch_bounds__(256) k_plan_solve(const PlanJob* __restrict__ jobs, RayState* queue, uint32_t* queue_count) {
+// kLds: the block first stages its job's filter arrays in LDS (stage_scene).
+template <int kMode, int kPA, int kLds>
+__global__ void __launch_bounds__(256) k_plan_solve(const PlanJob* __restrict__ jobs, RayState* queue,
+                                                    uint32_t* queue_count) {
+    extern __shared__ __align__(16) unsigned char lds[];
     const int32_t j = (int32_t)blockIdx.y;
     const PlanJob& J = jobs[j];
     const int64_t n = J.args.n;
     if ((int64_t)blockIdx.x * blockDim.x >= n) return;  // whole block: past this job's rays
-    const SceneView S = J.view;
+    const SceneView S = kLds ? stage_scene(J.view, lds) : J.view;
     const SolveArgs A = J.args;
     uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
-    solve_ray<kMode, kPA>(S, A, (CSobol)&J.sob, (CMap)&J.map, blockIdx.x * blockDim.x + threadIdx.x, ctr, queue,
-                          queue_count, j);
+    solve_ray<kMode, kPA, kLds>(S, A, (CSobol)&J.sob, (CMap)&J.map, blockIdx.x * blockDim.x + threadIdx.x, ctr,
+                                queue, queue_count, j);
     flush_counters(ctr, A.counters);
 }
 
@@ -95,6 +98,7 @@ struct ehss_plan {
     std::vector<ehss_scene*> scenes;
     DevBuf jobs_dev, info, contrib, partials, words, bins, queue, queue2;
     int64_t rays = 0, max_blocks = 0, max_red = 0, n_bins = 0, n_words = 0;
+    size_t lds_bytes = 0;  // dynamic LDS of the packet kernel (0: scenes read from HBM)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipGraph_t graph = nullptr;
@@ -104,6 +108,7 @@ struct ehss_plan {
 namespace {
 
 constexpr int kPlanWords = kCounterWords;  // per job; then 2 queue words
+constexpr size_t kPlanLdsLimit = 48 * 1024;  // packet kernels: keeps several blocks per CU
 
 void destroy_plan(ehss_plan* p) {
     if (!p) return;
@@ -129,20 +134,27 @@ int enqueue_plan(ehss_plan* p) {
     RayState* qbuf = p->variant == 4 ? p->queue.as<RayState>() : nullptr;
     RayState* spill = p->variant == 4 && p->stage1 > 0 && p->group > 1 ? p->queue2.as<RayState>() : nullptr;
     const dim3 grid((unsigned)p->max_blocks, (unsigned)p->n_jobs), block(256);
-#define EHSS_PLAN(M)                                                                                  \
+    const size_t lds = p->lds_bytes;
+#define EHSS_PLAN_L(M, K)                                                                             \
     do {                                                                                              \
         if (p->pa)                                                                                    \
-            hipLaunchKernelGGL((k_plan_solve<M, 1>), grid, block, 0, s, jd, qbuf, qcount);           \
+            hipLaunchKernelGGL((k_plan_solve<M, 1, K>), grid, block, lds, s, jd, qbuf, qcount);      \
         else                                                                                          \
-            hipLaunchKernelGGL((k_plan_solve<M, 0>), grid, block, 0, s, jd, qbuf, qcount);           \
+            hipLaunchKernelGGL((k_plan_solve<M, 0, K>), grid, block, lds, s, jd, qbuf, qcount);      \
+    } while (0)
+#define EHSS_PLAN(M)                 \
+    do {                             \
+        if (lds) EHSS_PLAN_L(M, 1);  \
+        else EHSS_PLAN_L(M, 0);      \
     } while (0)
     switch (p->variant) {
-        case 0: EHSS_PLAN(0); break;
-        case 1: EHSS_PLAN(1); break;
+        case 0: EHSS_PLAN_L(0, 0); break;
+        case 1: EHSS_PLAN_L(1, 0); break;
         case 2: EHSS_PLAN(2); break;
         case 3: EHSS_PLAN(3); break;
         default: EHSS_PLAN(4); break;
     }
+#undef EHSS_PLAN_L
 #undef EHSS_PLAN
     if (p->variant == 4) {
         const dim3 dgrid((unsigned)drain_blocks(p->rays));
@@ -254,6 +266,13 @@ int create_plan(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols,
         J.ledger = H + bins + bin_off[j];
     }
     EHSS_TRY(p->jobs_dev.alloc(sizeof(PlanJob) * n_jobs));
+    // LDS-staged scenes for the packet kernels, when every job's scene fits
+    // (as for a single solve: the flag has no effect otherwise).
+    if ((flags & EHSS_FLAG_LDS_PACKETS) && variant >= 2) {
+        size_t most = 0;
+        for (int j = 0; j < n_jobs; ++j) most = std::max(most, staged_scene_bytes(scenes[j]->view));
+        if (most <= kPlanLdsLimit) p->lds_bytes = most;
+    }
     *out = p;
     p = nullptr;
     return EHSS_OK;

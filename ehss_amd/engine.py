@@ -440,9 +440,10 @@ class GPUPlan(_MultiSolve):
     _prefix = "plan"
 
     def __init__(self, scenes, sobols, powers, caps, precision="fp32-fixup", pa: bool = True, graph: bool = False,
-                 variant: int = 0, coop: int | None = None, group: int | None = None, stage1: int | None = None):
+                 variant: int = 0, coop: int | None = None, group: int | None = None, stage1: int | None = None,
+                 lds: bool = False):
         flags = (L.FLAG_PA if pa else 0) | (L.FLAG_GRAPH if graph else 0) | (int(variant) << L.VARIANT_SHIFT)
-        flags |= coop_flags(coop, group, stage1)
+        flags |= coop_flags(coop, group, stage1) | (L.FLAG_LDS_PACKETS if lds else 0)
         self._create(scenes, sobols, powers, caps, flags, precision)
 
 

@@ -37,35 +37,37 @@ def _same(alone, planned, key):
 
 
 CASES = [
-    ("fp32-fixup", 4, None, None),  # library defaults: one drain, whole waves per ray at these sizes
-    ("fp32-fixup", 4, 4, 2),        # two-stage drain: 4 lanes per ray, spill after 2 queries
-    ("fp32-fixup", 4, 1, None),     # per-lane drain with cooperative tail
-    ("fp32-fixup", 3, None, None),
-    ("fp32-fixup", 2, None, None),
-    ("fp32-fixup", 1, None, None),
-    ("fp64", 0, None, None),
+    ("fp32-fixup", 4, None, None, False),  # library defaults: one drain, whole waves per ray at these sizes
+    ("fp32-fixup", 4, 4, 2, False),        # two-stage drain: 4 lanes per ray, spill after 2 queries
+    ("fp32-fixup", 4, 1, None, False),     # per-lane drain with cooperative tail
+    ("fp32-fixup", 4, None, None, True),   # packet kernels on LDS-staged scenes
+    ("fp32-fixup", 3, None, None, False),
+    ("fp32-fixup", 2, None, None, True),
+    ("fp32-fixup", 1, None, None, False),
+    ("fp64", 0, None, None, False),
 ]
 
 
-@pytest.mark.parametrize("precision,variant,group,stage1", CASES)
-def test_plan_equals_solves_alone(precision, variant, group, stage1):
+@pytest.mark.parametrize("precision,variant,group,stage1,lds", CASES)
+def test_plan_equals_solves_alone(precision, variant, group, stage1, lds):
     jobs = _jobs()
     sc = [GPUScene(c, r) for (c, r), _, _ in jobs]
     sobols = [scramble(11 + j) for j in range(len(jobs))]
     powers = [p for _, p, _ in jobs]
     caps = [c for _, _, c in jobs]
+    key = (precision, variant, group, stage1, lds)
     try:
-        say("plan alone", precision, variant, group, stage1)
+        say("plan alone", *key)
         alone = [s.solve(sob, p, c, precision=precision, pa=True, histogram=True, variant=variant)
                  for s, sob, p, c in zip(sc, sobols, powers, caps)]
-        say("plan", precision, variant, group, stage1)
+        say("plan", *key)
         with GPUPlan(sc, sobols, powers, caps, precision=precision, pa=True, variant=variant, group=group,
-                     stage1=stage1) as plan:
+                     stage1=stage1, lds=lds) as plan:
             got = plan.run(histogram=True)
             again = plan.run(histogram=True)
         for j, (a, g, h) in enumerate(zip(alone, got, again)):
-            _same(a, g, (precision, variant, group, stage1, j))
-            _same(a, h, (precision, variant, group, stage1, j, "relaunch"))
+            _same(a, g, key + (j,))
+            _same(a, h, key + (j, "relaunch"))
             assert g["histogram"].sum() == 1 << powers[j]
     finally:
         for s in sc:
@@ -137,3 +139,27 @@ def test_plan_async_query_and_argument_checks():
             GPUPlan([s], [scramble(1)], [10, 11], [64])
         with pytest.raises(Exception, match="2\\^30"):
             GPUPlan([s] * 17, [scramble(1)] * 17, [26] * 17, [64] * 17)
+
+
+def test_autotuned_plan_matches_solves_alone(tmp_path):
+    """The tuner times real plans (its fingerprint check asserts that every
+    configuration gives the same totals) and the winner equals the solves alone."""
+    from ehss_amd import autotune
+
+    structs = [scenes.globule(400, 5), scenes.SCENES["c60"](), scenes.globule(900, 6)]
+    sobols = [scramble(50 + j) for j in range(len(structs))]
+    space = {"group": (64, 4), "stage1": (2, 0), "coop": (8,), "lds": (False, True), "leaf_size": (4, 8)}
+    say("autotune plan")
+    res, scs, kw = autotune.tune_plan(structs, 14, 64, sobols, space=space, repeats=1, passes=1,
+                                      cache_path=tmp_path / "tune.json")
+    try:
+        assert res.config["leaf_size"] in (4, 8) and len(res.trials) >= 4
+        with GPUPlan(scs, sobols, [14] * 3, [64] * 3, **kw) as plan:
+            got = plan.run(histogram=True)
+        for j, (s, sob) in enumerate(zip(scs, sobols)):
+            _same(s.solve(sob, 14, 64, pa=True, histogram=True), got[j], ("tuned", j))
+        again, _, _ = autotune.tune_plan(structs, 14, 64, sobols, space=space, cache_path=tmp_path / "tune.json")
+        assert again.cached and again.config == res.config
+    finally:
+        for s in scs:
+            s.close()
