@@ -19,6 +19,27 @@
 #pragma once
 #include "ehss_query.h"
 
+// Minimum waves per SIMD the packet kernels and the queue drains are compiled
+// for (the register budget follows: 512 / waves VGPRs per lane).  Both kernels
+// are latency-bound, so resident waves are what hides their memory round trips.
+// Measured on the 650-atom globule (2^20 rays) and on 18 globules of 200-5000
+// atoms (2^20 rays each): packet kernels at 6 waves (a few spilled registers)
+// beat 4; the single-scene drain, whose LDS-staged scene already limits it to
+// 3 blocks per CU, gains nothing from more; the multi-scene drain, which reads
+// the scenes from HBM, gains 5-8 % at 4.
+#ifndef EHSS_SOLVE_WAVES
+#define EHSS_SOLVE_WAVES 6
+#endif
+#ifndef EHSS_DRAIN_WAVES
+#define EHSS_DRAIN_WAVES 3
+#endif
+#ifndef EHSS_PLAN_DRAIN_WAVES
+#define EHSS_PLAN_DRAIN_WAVES 4
+#endif
+#define EHSS_SOLVE_KERNEL __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EHSS_SOLVE_WAVES)))
+#define EHSS_DRAIN_KERNEL __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EHSS_DRAIN_WAVES)))
+#define EHSS_PLAN_DRAIN_KERNEL __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EHSS_PLAN_DRAIN_WAVES)))
+
 namespace ehss {
 
 struct SolveArgs {

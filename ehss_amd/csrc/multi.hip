@@ -4,18 +4,21 @@
 //
 // Grid layout: the packet kernel runs blocks (x, j) for job j = blockIdx.y and
 // its 256-ray block x; blocks past a job's ray count exit at once.  Every ray is
-// recorded at its job's slot, the rays still bouncing after the packet queries of
-// every job share one drain queue (entries carry their job), and each job is
-// reduced in its own fixed order by blocks (x, j) of the reduction kernel.  A
-// job's totals are therefore bit-identical to ehss_solve on its scene alone.
-// The job table (scene views, arguments, scrambles, ray maps) lives in device
-// memory and is re-uploaded before a launch when a scramble changed, so a
-// captured graph follows new scrambles.
+// recorded at its job's slot; the rays still bouncing after the packet queries
+// go to their job's own queue segment, and the drain kernels run blocks (x, j)
+// over job j's segment, so the waves of a block work on one scene.  (Staging
+// that scene in LDS, as the single-scene drain does, measured slower here:
+// 28.9 ms against 26.4 ms per launch of 18 globules of 200-5000 atoms at 2^20
+// rays each.)  Each job is reduced in its own fixed order by blocks (x, j) of
+// the reduction kernel.  A job's totals are therefore bit-identical to
+// ehss_solve on its scene alone.  The job table (scene views, arguments,
+// scrambles, ray maps, queue segments) lives in device memory and is
+// re-uploaded before a launch when a scramble changed, so a captured graph
+// follows new scrambles.
 #include <math.h>
 #include <string.h>
 
 #include <algorithm>
-#include <mutex>
 #include <vector>
 
 #include "ehss_solve.h"
@@ -31,14 +34,16 @@ struct PlanJob {
     Partial* partials;
     unsigned long long* hist;
     unsigned long long* ledger;
+    RayState* queue;  // rays still bouncing after the packet queries (n slots)
+    RayState* spill;  // rays still bouncing after the first drain stage (n slots, or null)
+    uint32_t* qwords;  // [0] queue length, [1] queue head, [2] spill length, [3] spill head
 };
 
 namespace {
 
 // kLds: the block first stages its job's filter arrays in LDS (stage_scene).
 template <int kMode, int kPA, int kLds>
-__global__ void __launch_bounds__(256) k_plan_solve(const PlanJob* __restrict__ jobs, RayState* queue,
-                                                    uint32_t* queue_count) {
+__global__ void EHSS_SOLVE_KERNEL k_plan_solve(const PlanJob* __restrict__ jobs) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int32_t j = (int32_t)blockIdx.y;
     const PlanJob& J = jobs[j];
@@ -48,29 +53,40 @@ __global__ void __launch_bounds__(256) k_plan_solve(const PlanJob* __restrict__ 
     const SolveArgs A = J.args;
     uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
     solve_ray<kMode, kPA, kLds>(S, A, (CSobol)&J.sob, (CMap)&J.map, blockIdx.x * blockDim.x + threadIdx.x, ctr,
-                                queue, queue_count, j);
+                                J.queue, J.qwords, j);
     flush_counters(ctr, A.counters);
 }
 
-// Drain of the shared queue; a queued ray's job selects its scene and arguments.
-__global__ void __launch_bounds__(256) k_plan_drain(const PlanJob* jobs, const RayState* queue,
-                                                    const uint32_t* queue_count, uint32_t* queue_head, int32_t coop,
-                                                    int32_t group, int32_t max_queries, RayState* spill,
-                                                    uint32_t* spill_count) {
-    const uint32_t len = __builtin_amdgcn_readfirstlane(*queue_count);
+// Drain stage `stage` (0: the packet kernel's queue, optionally spilling rays
+// still bouncing after max_queries into the spill segment; 1: the spill
+// segment) of job blockIdx.y, by blocks x.  A wave whose job's segment is
+// already taken exits at once.
+__global__ void EHSS_PLAN_DRAIN_KERNEL k_plan_drain(const PlanJob* __restrict__ jobs, int32_t stage, int32_t group,
+                                                    int32_t max_queries) {
+    const PlanJob& J = jobs[blockIdx.y];
+    uint32_t* w = J.qwords + 2 * stage;
+    const uint32_t len = __builtin_amdgcn_readfirstlane(w[0]);
     if (len == 0) return;
-    auto scene_of = [&](int32_t j) -> const SceneView& { return jobs[j].view; };
-    auto args_of = [&](int32_t j) -> const SolveArgs& { return jobs[j].args; };
-    auto account = [&](int32_t j, uint64_t b, uint64_t sp, uint64_t r) {
-        unsigned long long* c = jobs[j].args.counters;
-        if (b) atomicAdd(&c[0], (unsigned long long)b);
-        if (sp) atomicAdd(&c[1], (unsigned long long)sp);
-        if (r) atomicAdd(&c[2], (unsigned long long)r);
+    const uint32_t head =
+        __builtin_amdgcn_readfirstlane(__hip_atomic_load(&w[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (head >= len) return;
+    const SceneView S = J.view;
+    const SolveArgs A = J.args;
+    uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
+    auto scene_of = [&](int32_t) -> const SceneView& { return S; };
+    auto args_of = [&](int32_t) -> const SolveArgs& { return A; };
+    auto account = [&](int32_t, uint64_t b, uint64_t sp, uint64_t r) {
+        ctr[0] += b;
+        ctr[1] += sp;
+        ctr[2] += r;
     };
+    const RayState* q = stage == 0 ? J.queue : J.spill;
+    RayState* out = stage == 0 && max_queries > 0 ? J.spill : nullptr;
     if (group > 1)
-        drain_groups(queue, len, queue_head, group, scene_of, args_of, account, max_queries, spill, spill_count);
+        drain_groups(q, len, &w[1], group, scene_of, args_of, account, max_queries, out, out ? J.qwords + 2 : nullptr);
     else
-        drain_queue(queue, len, queue_head, coop, scene_of, args_of, account);
+        drain_queue(q, len, &w[1], A.coop, scene_of, args_of, account);
+    flush_counters(ctr, A.counters);
 }
 
 __global__ void __launch_bounds__(256) k_plan_reduce(const PlanJob* jobs) {
@@ -97,7 +113,7 @@ struct ehss_plan {
     std::vector<PlanJob> jobs;
     std::vector<ehss_scene*> scenes;
     DevBuf jobs_dev, info, contrib, partials, words, bins, queue, queue2;
-    int64_t rays = 0, max_blocks = 0, max_red = 0, n_bins = 0, n_words = 0;
+    int64_t rays = 0, max_blocks = 0, max_red = 0, n_bins = 0, n_words = 0, drain_grid = 0;
     size_t lds_bytes = 0;  // dynamic LDS of the packet kernel (0: scenes read from HBM)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -107,8 +123,8 @@ struct ehss_plan {
 
 namespace {
 
-constexpr int kPlanWords = kCounterWords;  // per job; then 2 queue words
-constexpr size_t kPlanLdsLimit = 48 * 1024;  // packet kernels: keeps several blocks per CU
+constexpr int kPlanWords = kCounterWords + 2;  // per job: counters, then 4 uint32 queue words
+constexpr size_t kPlanLdsLimit = 48 * 1024;     // packet kernels: keeps several blocks per CU
 
 void destroy_plan(ehss_plan* p) {
     if (!p) return;
@@ -122,25 +138,20 @@ void destroy_plan(ehss_plan* p) {
 }
 
 // The launch sequence: zero fills (kernels, so that it can be captured), the
-// packet kernel, the drain(s), the reductions.
+// packet kernel, the drain stage(s), the reductions.
 int enqueue_plan(ehss_plan* p) {
     hipStream_t s = p->stream;
-    unsigned long long* W = p->words.as<unsigned long long>();
-    EHSS_TRY(launch_zero(W, p->n_words, s));
+    EHSS_TRY(launch_zero(p->words.as<unsigned long long>(), p->n_words, s));
     EHSS_TRY(launch_zero(p->bins.as<unsigned long long>(), 2 * p->n_bins, s));
     const PlanJob* jd = p->jobs_dev.as<PlanJob>();
-    uint32_t* qcount = reinterpret_cast<uint32_t*>(W + (int64_t)kPlanWords * p->n_jobs);
-    uint32_t* scount = qcount + 2;
-    RayState* qbuf = p->variant == 4 ? p->queue.as<RayState>() : nullptr;
-    RayState* spill = p->variant == 4 && p->stage1 > 0 && p->group > 1 ? p->queue2.as<RayState>() : nullptr;
     const dim3 grid((unsigned)p->max_blocks, (unsigned)p->n_jobs), block(256);
     const size_t lds = p->lds_bytes;
-#define EHSS_PLAN_L(M, K)                                                                             \
-    do {                                                                                              \
-        if (p->pa)                                                                                    \
-            hipLaunchKernelGGL((k_plan_solve<M, 1, K>), grid, block, lds, s, jd, qbuf, qcount);      \
-        else                                                                                          \
-            hipLaunchKernelGGL((k_plan_solve<M, 0, K>), grid, block, lds, s, jd, qbuf, qcount);      \
+#define EHSS_PLAN_L(M, K)                                                                  \
+    do {                                                                                   \
+        if (p->pa)                                                                         \
+            hipLaunchKernelGGL((k_plan_solve<M, 1, K>), grid, block, lds, s, jd);         \
+        else                                                                               \
+            hipLaunchKernelGGL((k_plan_solve<M, 0, K>), grid, block, lds, s, jd);         \
     } while (0)
 #define EHSS_PLAN(M)                 \
     do {                             \
@@ -157,12 +168,13 @@ int enqueue_plan(ehss_plan* p) {
 #undef EHSS_PLAN_L
 #undef EHSS_PLAN
     if (p->variant == 4) {
-        const dim3 dgrid((unsigned)drain_blocks(p->rays));
-        hipLaunchKernelGGL(k_plan_drain, dgrid, block, 0, s, jd, qbuf, qcount, qcount + 1, p->coop, p->group,
-                           spill ? p->stage1 : 0, spill, scount);
-        if (spill)
-            hipLaunchKernelGGL(k_plan_drain, dgrid, block, 0, s, jd, spill, scount, scount + 1, p->coop, kWave, 0,
-                               (RayState*)nullptr, (uint32_t*)nullptr);
+        const dim3 dgrid((unsigned)p->drain_grid, (unsigned)p->n_jobs);
+        auto drain = [&](int stage, int group, int max_queries) {
+            hipLaunchKernelGGL(k_plan_drain, dgrid, block, 0, s, jd, stage, group, max_queries);
+        };
+        const bool two = p->stage1 > 0 && p->group > 1;
+        drain(0, p->group, two ? p->stage1 : 0);
+        if (two) drain(1, kWave, 0);
     }
     hipLaunchKernelGGL(k_plan_reduce, dim3((unsigned)p->max_red, (unsigned)p->n_jobs), block, 0, s, jd);
     hipLaunchKernelGGL(k_plan_finalize, dim3((unsigned)p->n_jobs), block, 0, s, jd);
@@ -222,7 +234,7 @@ int create_plan(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols,
     }
     EHSS_CHECK(p->max_blocks <= (1LL << 31) - 1, "too many blocks per job");
     p->n_bins = bins;
-    p->n_words = (int64_t)kPlanWords * n_jobs + 2;
+    p->n_words = (int64_t)kPlanWords * n_jobs;
     EHSS_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
     EHSS_HIP(hipEventCreate(&p->ev0));
     EHSS_HIP(hipEventCreate(&p->ev1));
@@ -264,8 +276,17 @@ int create_plan(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols,
         J.partials = p->partials.as<Partial>() + red_off[j];
         J.hist = H + bin_off[j];
         J.ledger = H + bins + bin_off[j];
+        J.qwords = reinterpret_cast<uint32_t*>(W + (int64_t)kPlanWords * j + kCounterWords);
+        if (variant == 4) {
+            J.queue = p->queue.as<RayState>() + ray_off[j];
+            if (p->queue2.p) J.spill = p->queue2.as<RayState>() + ray_off[j];
+        }
     }
     EHSS_TRY(p->jobs_dev.alloc(sizeof(PlanJob) * n_jobs));
+    // Drain blocks per job: as for a single solve of the largest job.
+    int64_t most_rays = 0;
+    for (int j = 0; j < n_jobs; ++j) most_rays = std::max<int64_t>(most_rays, 1LL << powers[j]);
+    p->drain_grid = drain_blocks(most_rays);
     // LDS-staged scenes for the packet kernels, when every job's scene fits
     // (as for a single solve: the flag has no effect otherwise).
     if ((flags & EHSS_FLAG_LDS_PACKETS) && variant >= 2) {

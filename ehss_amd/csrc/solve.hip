@@ -32,7 +32,7 @@ __device__ __forceinline__ SolveArgs replicate_args(const SolveArgs& A, int32_t 
 // scrambled net and ray map are read (scalar loads) from device tables.
 // kLds: the block first stages the scene's filter arrays in LDS (stage_scene).
 template <int kMode, int kPA, int kLds>
-__global__ void __launch_bounds__(256) k_solve(SceneView S, SolveArgs A, const ehss_sobol* sobs, const RayMap* maps,
+__global__ void EHSS_SOLVE_KERNEL k_solve(SceneView S, SolveArgs A, const ehss_sobol* sobs, const RayMap* maps,
                                                RayState* queue, uint32_t* queue_count) {
     extern __shared__ __align__(16) unsigned char lds[];
     const SceneView V = kLds ? stage_scene(S, lds) : S;
@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(256) k_solve(SceneView S, SolveArgs A, const e
 constexpr int kCrossCells = 1;
 
 template <int kMode, int kPA, int kLds>
-__global__ void __launch_bounds__(256) k_solve_x(SceneView S0, SolveArgs A, CrossTables X, int32_t reps,
+__global__ void EHSS_SOLVE_KERNEL k_solve_x(SceneView S0, SolveArgs A, CrossTables X, int32_t reps,
                                                  int32_t groups, RayState* queue, uint32_t* queue_count) {
     extern __shared__ __align__(16) unsigned char lds[];
     const SceneView S = kLds ? stage_scene(S0, lds) : S0;
@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(256) k_solve_x(SceneView S0, SolveArgs A, Cros
 // spheres are first copied into LDS, so every node, leaf and sphere fetch of the
 // drain is an LDS access instead of a dependent L2 round trip.
 template <int kLds>
-__global__ void __launch_bounds__(256) k_drain(SceneView S, SolveArgs A, int32_t replicates, const RayState* queue,
+__global__ void EHSS_DRAIN_KERNEL k_drain(SceneView S, SolveArgs A, int32_t replicates, const RayState* queue,
                                                const uint32_t* queue_count, uint32_t* queue_head, int32_t group,
                                                int32_t max_queries, RayState* spill, uint32_t* spill_count) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -434,7 +434,10 @@ namespace {
 constexpr int kDefaultVariant = 4;
 constexpr int kMaxReplicates = 4096;
 constexpr int kCrossMinReps = 16;  // replicates per launch from which they are scheduled across lanes
-constexpr size_t kDrainLdsLimit = 144 * 1024;  // of the 160 KB per CU
+#ifndef EHSS_DRAIN_LDS_KB
+#define EHSS_DRAIN_LDS_KB 144
+#endif
+constexpr size_t kDrainLdsLimit = (size_t)EHSS_DRAIN_LDS_KB * 1024;  // of the 160 KB per CU
 constexpr size_t kSolveLdsLimit = 48 * 1024;   // packet kernels: keeps several blocks per CU
 
 // Ray index of record slot t (host copy of the device mapping: Gray index from
