@@ -187,7 +187,8 @@ struct ehss_scene {
     ehss::HostBuf host_words;  // counters, histograms and ledgers of the pending launch
     ehss::HostBuf host_table;  // scrambles and ray maps of the pending launch (staging)
     // solve workspace (grow-only)
-    ehss::DevBuf ray_info, ray_contrib, partials, hist, ledger, counters, queue, queue2, ray_times, tables;
+    ehss::DevBuf ray_info, ray_contrib, partials, hist, ledger, counters, queue, queue2, ray_times, tables,
+        drain_stripes;
     int64_t last_rays = 0, last_times = 0;
     uint32_t last_map[32] = {0};  // ray map of the last solve (slot -> ray index)
     int last_power = 0;

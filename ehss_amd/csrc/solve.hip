@@ -90,10 +90,17 @@ __global__ void EHSS_SOLVE_KERNEL k_solve_x(SceneView S0, SolveArgs A, CrossTabl
 // kLds the FP32 filter tree, the leaf boxes, leaf offsets, slot ids and FP32
 // spheres are first copied into LDS, so every node, leaf and sphere fetch of the
 // drain is an LDS access instead of a dependent L2 round trip.
+// Several replicates: the drain's work counters of replicate r go to
+// stripes[(r * kDrainStripes + s) * 4 + k], stripe s picked by the wave, so
+// that the atomics of a launch spread over many addresses; k_finalize adds a
+// replicate's stripes into its counters.
+constexpr int kDrainStripes = 64;
+
 template <int kLds>
 __global__ void EHSS_DRAIN_KERNEL k_drain(SceneView S, SolveArgs A, int32_t replicates, const RayState* queue,
                                                const uint32_t* queue_count, uint32_t* queue_head, int32_t group,
-                                               int32_t max_queries, RayState* spill, uint32_t* spill_count) {
+                                               int32_t max_queries, RayState* spill, uint32_t* spill_count,
+                                               unsigned long long* stripes) {
     extern __shared__ __align__(16) unsigned char lds[];
     uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
     const uint32_t len = __builtin_amdgcn_readfirstlane(*queue_count);
@@ -101,25 +108,51 @@ __global__ void EHSS_DRAIN_KERNEL k_drain(SceneView S, SolveArgs A, int32_t repl
     const SceneView L = kLds ? stage_scene(S, lds) : S;
     auto scene_of = [&](int32_t) -> const SceneView& { return L; };
     auto args_of = [&](int32_t j) -> SolveArgs { return replicate_args(A, j); };
-    // One replicate: per-lane sums flushed once; several: per-ray atomics into
-    // the ray's replicate.
+    // Per-lane sums.  One replicate: flushed once per wave at the end.  Several:
+    // the lane's sums belong to replicate `cur` and are added to its counters
+    // when the lane moves to a ray of another replicate (queued rays of one
+    // replicate mostly come in runs), never one atomic per ray.
+    int32_t cur = -1;
+    const int64_t stripe = (int64_t)((blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) % kDrainStripes);
+    auto flush_lane = [&]() {
+        if (cur < 0) return;
+        unsigned long long* c = stripes + ((int64_t)cur * kDrainStripes + stripe) * 4;
+        for (int k = 0; k < 3; ++k)
+            if (ctr[k]) atomicAdd(&c[k], (unsigned long long)ctr[k]);
+        ctr[0] = ctr[1] = ctr[2] = 0;
+    };
     auto account = [&](int32_t j, uint64_t b, uint64_t sp, uint64_t r) {
-        if (replicates == 1) {
-            ctr[0] += b;
-            ctr[1] += sp;
-            ctr[2] += r;
-        } else {
-            unsigned long long* c = A.counters + (int64_t)j * kCounterWords;
-            if (b) atomicAdd(&c[0], (unsigned long long)b);
-            if (sp) atomicAdd(&c[1], (unsigned long long)sp);
-            if (r) atomicAdd(&c[2], (unsigned long long)r);
+        if (replicates > 1 && j != cur) {
+            flush_lane();
+            cur = j;
         }
+        ctr[0] += b;
+        ctr[1] += sp;
+        ctr[2] += r;
     };
     if (group > 1)
         drain_groups(queue, len, queue_head, group, scene_of, args_of, account, max_queries, spill, spill_count);
     else
         drain_queue(queue, len, queue_head, A.coop, scene_of, args_of, account);
-    if (replicates == 1) flush_counters(ctr, A.counters);
+    if (replicates == 1) {
+        flush_counters(ctr, A.counters);
+        return;
+    }
+    // Remaining per-lane sums: one wave reduction and one atomic per distinct
+    // replicate held by the wave's lanes (usually one or two).
+    const int lane = threadIdx.x & (kWave - 1);
+    for (;;) {
+        const uint64_t pending = __ballot(cur >= 0);
+        if (!pending) break;
+        const int32_t j = __builtin_amdgcn_readlane(cur, __builtin_ctzll(pending));
+        const bool mine = cur == j;
+        unsigned long long* c = stripes + ((int64_t)j * kDrainStripes + stripe) * 4;
+        for (int k = 0; k < 3; ++k) {
+            const int64_t v = wave_sum_i64(mine ? (int64_t)ctr[k] : 0);
+            if (lane == 0 && v) atomicAdd(&c[k], (unsigned long long)v);
+        }
+        if (mine) cur = -1;
+    }
 }
 
 // LDS bytes of a scene staged by the kLds kernels.
@@ -140,12 +173,18 @@ __global__ void __launch_bounds__(256) k_reduce(SolveArgs A, int64_t nblocks, Pa
                  ledger + r * bins, reinterpret_cast<int*>(B.counters + 6));
 }
 
-// Final fixed-order combination of replicate blockIdx.x.
+// Final fixed-order combination of replicate blockIdx.x, and the sum of its
+// drain counter stripes (if any) into its counters.
 __global__ void __launch_bounds__(256) k_finalize(const Partial* partials, int64_t nblocks,
-                                                  unsigned long long* counters) {
+                                                  unsigned long long* counters, const unsigned long long* stripes) {
     const int32_t r = (int32_t)blockIdx.x;
     finalize_block(partials + r * nblocks, nblocks,
                    reinterpret_cast<Partial*>(counters + (int64_t)r * kCounterWords + 8));
+    if (stripes && threadIdx.x < 3) {
+        unsigned long long sum = 0;
+        for (int s = 0; s < kDrainStripes; ++s) sum += stripes[((int64_t)r * kDrainStripes + s) * 4 + threadIdx.x];
+        counters[(int64_t)r * kCounterWords + threadIdx.x] += sum;
+    }
 }
 
 __global__ void k_source_rays(int64_t n, double cx, double cy, double cz, double R, ehss_sobol sob, double* u_out,
@@ -597,9 +636,16 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     }
 #undef EHSS_LAUNCH
 #undef EHSS_LAUNCH_L
+    unsigned long long* stripes = nullptr;
     if (variant == 4) {
         const size_t lds = drain_lds_bytes(sc->view);
         const dim3 dgrid((unsigned)drain_blocks(total));
+        if (reps > 1) {
+            const int64_t words = (int64_t)reps * kDrainStripes * 4;
+            EHSS_TRY(sc->drain_stripes.reserve(sizeof(unsigned long long) * words));
+            stripes = sc->drain_stripes.as<unsigned long long>();
+            EHSS_TRY(launch_zero(stripes, words, s));
+        }
         if (lds <= kDrainLdsLimit) {
             static std::once_flag attr_once;
             hipError_t attr = hipSuccess;
@@ -616,10 +662,10 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
                          RayState* out, uint32_t* out_count) {
             if (lds <= kDrainLdsLimit)
                 hipLaunchKernelGGL(k_drain<1>, dgrid, dim3(256), lds, s, sc->view, A, reps, in, count, head, group,
-                                   max_q, out, out_count);
+                                   max_q, out, out_count, stripes);
             else
                 hipLaunchKernelGGL(k_drain<0>, dgrid, dim3(256), 0, s, sc->view, A, reps, in, count, head, group,
-                                   max_q, out, out_count);
+                                   max_q, out, out_count, stripes);
         };
         drain(qbuf, qcount, qcount + 1, A.group, spill ? stage1 : 0, spill, scount);
         if (spill) drain(spill, scount, scount + 1, kWave, 0, nullptr, nullptr);
@@ -629,7 +675,8 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     hipLaunchKernelGGL(k_reduce, dim3((unsigned)nblocks, (unsigned)reps), dim3(256), 0, s, A, nblocks,
                        sc->partials.as<Partial>(), sc->hist.as<unsigned long long>(),
                        sc->ledger.as<unsigned long long>());
-    hipLaunchKernelGGL(k_finalize, dim3((unsigned)reps), dim3(256), 0, s, sc->partials.as<Partial>(), nblocks, ctr);
+    hipLaunchKernelGGL(k_finalize, dim3((unsigned)reps), dim3(256), 0, s, sc->partials.as<Partial>(), nblocks, ctr,
+                       (const unsigned long long*)stripes);
     EHSS_HIP(hipGetLastError());
     const int64_t host_words = (int64_t)kCounterWords * reps + 2 * bins * reps;
     EHSS_TRY(sc->host_words.reserve(sizeof(unsigned long long) * host_words));

@@ -40,9 +40,10 @@ import numpy as np
 SCENE_BYTES_PER_ATOM = 24 + 8 + 32 + 4 + 4 + 4 + 128 + 128 + 4 + 64 + 16 + 32 + 32
 SCENE_BYTES_FIXED = 64 * 2 + 4 * 8 * 64 + 4096  # PA root slack, frontier, small buffers
 # Solve workspace per ray: record word 4, contribution 8, two queue entries 112; per
-# replicate: scramble + ray map 660, counters 128, histogram + ledger bins 16.
+# replicate: scramble + ray map 660, counters 128, drain counter stripes 2048,
+# histogram + ledger bins 16.
 SOLVE_BYTES_PER_RAY = 4 + 8 + 2 * 112
-SOLVE_BYTES_PER_REPLICATE = 660 + 128
+SOLVE_BYTES_PER_REPLICATE = 660 + 128 + 2048
 SOLVE_BYTES_FIXED = 64 * 1024
 
 
