@@ -6,7 +6,10 @@ molecule (random walk, 1.5 A steps, 0.11 atoms/A^3, seed 1): 2^20 scrambled
 Sobol rays -> impact-disk source -> BVH any-hit PA -> reflect-until-escape trace
 (cap 64) -> FP64 compensated (1 - cos chi) reduction.  The scene and its trees
 are resident in HBM before the timed region; each step uses its own replicate
-seed.  `value` is traced rays per second of the whole step (host wall clock).
+seed.  Steps are issued STEPS_PER_LAUNCH at a time as one multi-replicate launch
+(ehss_solve_many), whose per-step totals are bit-identical to separate solves.
+`value` is traced rays per second of the whole timed region (host wall clock);
+the rate with one step per launch is reported beside it.
 
 Side measurements (not part of `value`): the plain FP64 kernel on the same
 workload, CCS/s at 1 % relative RMSE (single stream, and with overlapped
@@ -37,6 +40,7 @@ METRIC = "EHSS traced rays/s + CCS/s at 1% RMSE on 1 MI355X; p50/p99 serve laten
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 POWER, CAP, SEED = 20, 64, 1
+STEPS_PER_LAUNCH = 16     # 2^20-ray steps per launch: 1.25 G rays/s vs 0.53 one per launch
 COPY_BYTES = 256 << 20    # calibration copy of the traffic probe (bytes read = bytes written)
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_solve_globule650.txt"
 
@@ -221,7 +225,8 @@ def traffic_probe(args):
     centers, radii, _ = molecules.workload("globule650")
     with GPUScene(centers, radii) as scene:
         for k in range(args.steps):
-            scene.solve(scramble(replicate_seed(SEED, k)), POWER, CAP, precision=args.precision, pa=True)
+            scene.solve_many([scramble(replicate_seed(SEED, STEPS_PER_LAUNCH * k + j))
+                              for j in range(STEPS_PER_LAUNCH)], POWER, CAP, precision=args.precision, pa=True)
     say("traffic probe done")
     return 0
 
@@ -255,19 +260,30 @@ def main():
     sobols = [scramble(s) for s in seeds]
     n = 1 << POWER
 
-    # Warmup, then exactly K timed steps; every solve returns after its stream
-    # has drained, so the region is bracketed by device synchronisation.
-    for k in range(args.warmup):
-        scene.solve(sobols[k], POWER, CAP, precision=args.precision, pa=True)
-    kernel_ms = []
+    # Warmup, then exactly K timed steps in launches of STEPS_PER_LAUNCH (the
+    # last one takes the remainder); every launch returns after its stream has
+    # drained, so the region is bracketed by device synchronisation.
+    def run_steps(first, count):
+        ms, rays, out = [], [], None
+        for b in range(first, first + count, STEPS_PER_LAUNCH):
+            batch = sobols[b:min(b + STEPS_PER_LAUNCH, first + count)]
+            out = scene.solve_many(batch, POWER, CAP, precision=args.precision, pa=True)
+            ms.append(out[0]["kernel_ms"])
+            rays.append(len(batch) * n)
+        return ms, rays, out
+
+    run_steps(0, args.warmup)
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        res = scene.solve(sobols[args.warmup + k], POWER, CAP, precision=args.precision, pa=True)
-        kernel_ms.append(res["kernel_ms"])
+    kernel_ms, launch_rays, many = run_steps(args.warmup, args.steps)
     elapsed = time.perf_counter() - t0
+    res = many[-1]
     ms_per_step = 1e3 * elapsed / args.steps
     value = n * args.steps / elapsed
     say(f"{args.steps} steps, {ms_per_step:.3f} ms/step, {value:.4g} rays/s")
+    t1 = time.perf_counter()
+    for k in range(20):
+        scene.solve(sobols[k], POWER, CAP, precision=args.precision, pa=True)
+    one_per_launch = 20 * n / (time.perf_counter() - t1)
 
     # Algorithmic work per ray from the plain FP64 kernel, whose box/sphere
     # counters equal the reference traversal's (pinned by the parity tests).
@@ -275,12 +291,17 @@ def main():
     refl = float(np.dot(np.arange(CAP + 1), plain["histogram"]))
     fpr = flops_per_ray(plain["box_tests"], plain["sphere_tests"], refl, plain["pa_box_tests"],
                         plain["pa_sphere_tests"], n)
-    avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
-    achieved = fpr * n / avg_kernel_s / 1e12
-    kernels = ("k_solve<4,1> + k_drain<1>" if args.precision == "fp32-fixup" else "k_solve<0,1>")
+    # Per launch of STEPS_PER_LAUNCH steps (HIP events on the scene's stream
+    # around the trace kernels; a shorter remainder launch is left out).
+    rays_per_launch = max(launch_rays)
+    full = [ms for ms, r in zip(kernel_ms, launch_rays) if r == rays_per_launch]
+    avg_kernel_s = float(np.mean(full)) / 1e3
+    achieved = fpr * rays_per_launch / avg_kernel_s / 1e12
+    kernels = ("k_solve<4,1,0> + 2 x k_drain<1>" if args.precision == "fp32-fixup" else "k_solve<0,1,0>")
     roofline = dict(bound="valu", achieved=round(achieved, 4), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=round(achieved / FP32_PEAK_TFLOPS, 6), traffic=read_traffic(),
-                    kernel=kernels, kernel_ms=round(float(np.mean(kernel_ms)), 4), flops_per_ray=round(fpr, 2),
+                    kernel=kernels, kernel_ms=round(float(np.mean(full)), 4), rays_per_launch=rays_per_launch,
+                    flops_per_ray=round(fpr, 2),
                     per_ray=dict(box=plain["box_tests"] / n, sphere=plain["sphere_tests"] / n,
                                  reflections=refl / n, pa_box=plain["pa_box_tests"] / n,
                                  pa_sphere=plain["pa_sphere_tests"] / n))
@@ -295,6 +316,11 @@ def main():
         for k in range(20):
             scene.solve(sobols[k], POWER, CAP, precision="fp64", pa=True)
         plain_rate = 20 * n / (time.perf_counter() - t1)
+        t1 = time.perf_counter()
+        for k in range(2):
+            scene.solve_many(sobols[STEPS_PER_LAUNCH * k:STEPS_PER_LAUNCH * (k + 1)], POWER, CAP, precision="fp64",
+                             pa=True)
+        plain_batched = 2 * STEPS_PER_LAUNCH * n / (time.perf_counter() - t1)
         say("CCS at 1% RMSE")
         power_ccs, rel = rays_per_ccs(scene, scramble, replicate_seed)
         ccs_sobols = [scramble(replicate_seed(7, k)) for k in range(4096)]
@@ -308,7 +334,9 @@ def main():
         for c in copies:
             c.close()
         extras["globule650"] = dict(
-            plain_fp64_rays_per_s=plain_rate, speedup_vs_plain_fp64=value / plain_rate,
+            steps_per_launch=STEPS_PER_LAUNCH, rays_per_s_one_step_per_launch=one_per_launch,
+            plain_fp64_rays_per_s=plain_rate, plain_fp64_rays_per_s_same_launches=plain_batched,
+            speedup_vs_plain_fp64=value / plain_rate, speedup_vs_plain_fp64_same_launches=value / plain_batched,
             omega_A2=res["omega"], pa_A2=res["pa"], fp64_retests_per_ray=res["fp64_retests"] / n,
             grazing_rays=res["grazing_rays"], unresolved_rays=res["unresolved"], tail_bound_A2=res["tail_bound"],
             omega_fp32_fixup_minus_fp64=scene.solve(sobols[0], POWER, CAP, pa=True)["omega"] - plain["omega"])
@@ -375,7 +403,7 @@ def main():
                 config=dict(workload="globule650: 650-atom synthetic globular molecule (seed 1), 2^20 scrambled-Sobol "
                                      "rays per step, cap 64, BVH any-hit PA, fused device solve",
                             atoms=len(radii), rays_per_step=n, cap=CAP, precision=args.precision,
-                            parallelism="single GPU"),
+                            steps_per_launch=STEPS_PER_LAUNCH, parallelism="single GPU"),
                 roofline=roofline, cpu_baseline=cpu, device=dev["name"], **extras)
     print(json.dumps(line))
     return 0

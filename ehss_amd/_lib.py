@@ -28,6 +28,7 @@ FLAG_KEEP_RAYS = 2
 FLAG_GRAPH = 4
 FLAG_TIMESTAMPS = 8
 FLAG_LDS_PACKETS = 16
+FLAG_PER_REPLICATE = 32
 VARIANT_SHIFT = 8
 COOP_SHIFT = 12
 GROUP_SHIFT = 19

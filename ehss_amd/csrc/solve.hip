@@ -547,7 +547,12 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
         make_ray_map(sobs[r], power, h_map[r].col);
     }
     // Many replicates: schedule them across lanes when a shared partition exists.
-    const bool cross = reps >= kCrossMinReps && power >= 1 && make_cross_maps(sobs, reps, power, h_cross);
+    // Measured on the 650-atom globule: the cross schedule wins for small nets
+    // (2^11 rays from 16 replicates, 2^14 from 64) and loses for large ones
+    // (2^14 rays x 16: 158 vs 192 M rays/s; 2^20 x 16: 0.40 vs 1.25 G rays/s).
+    const bool cross_shape = (power <= 12 && reps >= kCrossMinReps) || (power <= 16 && reps >= 64);
+    const bool cross = cross_shape && power >= 1 && !(flags & EHSS_FLAG_PER_REPLICATE) &&
+                       make_cross_maps(sobs, reps, power, h_cross);
     uint32_t* h_words = reinterpret_cast<uint32_t*>(h_cross + reps);
     if (cross) make_cross_tables(sobs, h_cross, reps, rp, power, h_words);
     EHSS_HIP(hipMemcpyAsync(sc->tables.p, h_sob, table_bytes, hipMemcpyHostToDevice, s));

@@ -266,9 +266,11 @@ class GPUScene:
 
     def solve_many(self, sobols, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
                    histogram: bool = False, variant: int = 0, coop: int | None = None, group: int | None = None,
-                   keep_rays: bool = False, stage1: int | None = None, lds: bool = False) -> list[dict]:
+                   keep_rays: bool = False, stage1: int | None = None, lds: bool = False,
+                   per_replicate: bool = False) -> list[dict]:
         """len(sobols) independent solves in one launch; one result dict per
-        scramble, each equal to solve() with that scramble."""
+        scramble, each equal to solve() with that scramble.  per_replicate:
+        never schedule the replicates across the lanes of a wave."""
         reps = len(sobols)
         table = (L.Sobol * reps)(*sobols)
         totals = (L.Totals * reps)()
@@ -276,6 +278,7 @@ class GPUScene:
         ledger = np.zeros(reps * (cap + 1), np.float64) if histogram else None
         flags = (L.FLAG_PA if pa else 0) | (L.FLAG_KEEP_RAYS if keep_rays else 0) | \
             (int(variant) << L.VARIANT_SHIFT) | coop_flags(coop, group, stage1) | (L.FLAG_LDS_PACKETS if lds else 0)
+        flags |= L.FLAG_PER_REPLICATE if per_replicate else 0
         L.call("ehss_solve_many", self.handle, reps, table, int(power), int(cap), precision_code(precision), flags,
                totals, L.ptr(hist), L.ptr(ledger))
         return self._split(totals, reps, cap, hist, ledger)

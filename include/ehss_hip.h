@@ -54,6 +54,7 @@ extern "C" {
 #define EHSS_FLAG_TIMESTAMPS 8u /* record each ray's completion time (ehss_scene_last_ray_times) */
 #define EHSS_FLAG_LDS_PACKETS 16u /* packet kernels stage the scene in LDS first (measured: not faster than
                                      scalar loads through the constant cache; kept for measurement) */
+#define EHSS_FLAG_PER_REPLICATE 32u /* solve_many: never schedule replicates across the lanes of a wave */
 /* Kernel variant of the FP32 filter path in flag bits 8..11 (0 = default):
  * 1 one ray per lane, 2 wave-uniform first query then per-lane bounces,
  * 3 wave-uniform traversal for every bounce, 4 wave-uniform first queries, then

@@ -104,6 +104,23 @@ def test_replicates_in_one_launch_equal_separate_solves(precision, variant, reps
                         np.testing.assert_array_equal(first["bounces"], rays["bounces"])
 
 
+@pytest.mark.parametrize("power,reps", [(12, 20), (15, 16)])
+def test_per_replicate_schedule_equals_separate_solves(power, reps):
+    """EHSS_FLAG_PER_REPLICATE (and the automatic choice at 2^15 x 16, which is
+    per replicate) give the same totals as separate solves."""
+    c, r = scenes.globule(650, 1)
+    with GPUScene(c, r) as s:
+        sobols = [scramble(900 + k) for k in range(reps)]
+        say("per-replicate schedule", power, reps)
+        forced = s.solve_many(sobols, power, 64, histogram=True, per_replicate=True)
+        auto = s.solve_many(sobols, power, 64, histogram=True)
+        for k in (0, reps // 2, reps - 1):
+            one = s.solve(sobols[k], power, 64, histogram=True)
+            for key in KEYS:
+                assert forced[k][key] == one[key] == auto[k][key], (power, reps, k, key)
+            np.testing.assert_array_equal(forced[k]["histogram"], one["histogram"])
+
+
 def test_replicate_launch_async_and_limits():
     c, r = scenes.SCENES["c60"]()
     with GPUScene(c, r) as s:
