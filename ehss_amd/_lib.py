@@ -33,6 +33,7 @@ VARIANT_SHIFT = 8
 COOP_SHIFT = 12
 GROUP_SHIFT = 19
 STAGE1_SHIFT = 22
+PACKETS_SHIFT = 26
 
 RAY_ESCAPED = 0x1
 RAY_UNRESOLVED = 0x2

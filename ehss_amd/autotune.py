@@ -1,6 +1,7 @@
-"""Autotuner for the fused device solve: lanes per ray in the queue drain (rays
-per wavefront), queries of the first drain stage, cooperative-tail threshold,
-LDS-staged scene for the packet kernels, and BVH leaf size.
+"""Autotuner for the fused device solve: packet queries before the drain, lanes
+per ray in the queue drain (rays per wavefront), queries of the first drain
+stage, cooperative-tail threshold, LDS-staged scene for the packet kernels, and
+BVH leaf size.
 
 Every configuration returns the same physical results (the FP32 filter only
 skips work whose FP64 outcome is decided, and the nearest hit does not depend
@@ -26,13 +27,14 @@ from pathlib import Path
 from typing import Callable
 
 DEFAULT_SPACE = {
+    "packets": (3, 2, 4, 5),    # wave-uniform (packet) queries before the drain
     "group": (64, 8, 4, 1),     # lanes per ray in the first drain (64 / group rays per wavefront)
     "stage1": (2, 1, 4, 0),     # queries before a ray spills to whole-wave lanes (0: one drain)
     "coop": (8, 4, 16),         # live rays per wave from which the tail is finished cooperatively
     "lds": (False, True),       # packet kernels read the scene staged in LDS
     "leaf_size": (4, 2, 8),     # spheres per BVH leaf group
 }
-DEFAULT_START = {"group": None, "stage1": None, "coop": None, "lds": False, "leaf_size": 4}
+DEFAULT_START = {"packets": None, "group": None, "stage1": None, "coop": None, "lds": False, "leaf_size": 4}
 
 
 def default_cache_path() -> Path:
@@ -186,7 +188,7 @@ def tune_plan(structures, power: int, cap: int, sobols, precision: str = "fp32-f
             plans[k] = GPUPlan(scenes_for(cfg.get("leaf_size", 4)), sobols, [power] * len(structures),
                                [cap] * len(structures), precision=precision, pa=pa, graph=graph,
                                coop=cfg.get("coop"), group=cfg.get("group"), stage1=cfg.get("stage1"),
-                               lds=bool(cfg.get("lds", False)))
+                               lds=bool(cfg.get("lds", False)), packets=cfg.get("packets"))
             plans[k].run()  # first launch: captures the graph, uploads the job table
         return plans[k]
 
@@ -203,7 +205,8 @@ def tune_plan(structures, power: int, cap: int, sobols, precision: str = "fp32-f
     # Start from the library's own choice for this many rays, spelled out, so
     # that the stage1 axis is explored around it.
     at_size = len(structures) << power >= 1 << 18
-    start = {"group": 4 if at_size else 64, "stage1": 2 if at_size else 0, "coop": 8, "lds": False, "leaf_size": 4}
+    start = {"packets": 3, "group": 4 if at_size else 64, "stage1": 2 if at_size else 0, "coop": 8, "lds": False,
+             "leaf_size": 4}
     try:
         res = tune(measure, space=space, start=start, repeats=repeats, passes=passes, fingerprint=fingerprint, key=key,
                    cache_path=default_cache_path() if cache_path is None else cache_path, refresh=refresh)
@@ -215,6 +218,6 @@ def tune_plan(structures, power: int, cap: int, sobols, precision: str = "fp32-f
         if lf != leaf:
             for s in scs:
                 s.close()
-    kwargs = {k: res.config.get(k) for k in ("coop", "group", "stage1")}
+    kwargs = {k: res.config.get(k) for k in ("coop", "group", "stage1", "packets")}
     kwargs["lds"] = bool(res.config.get("lds", False))
     return res, scenes_for(leaf), kwargs

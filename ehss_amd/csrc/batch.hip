@@ -281,6 +281,7 @@ int create_batch(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols
         J.args.counters = W + (int64_t)kWords * j;
         J.args.coop = b->coop;
         J.args.group = b->group;
+        J.args.packets = packets_from_flags(flags);
         J.sob = sobols[j];
         make_ray_map(J.sob, powers[j], J.map.col);
         J.pkt_begin = pkt_off[j];

@@ -54,6 +54,8 @@ struct SolveArgs {
     int32_t coop;                  // a wave with at most this many live rays finishes them cooperatively
     int32_t group;                 // lanes per ray in the queue drain (power of two 1..64)
     long long* times;              // optional per-ray completion time stamps (device wall clock)
+    int32_t packets;               // variant 4: packet queries before queueing (1..15)
+    int32_t pad_;
 };
 
 // Words per replicate in the solve workspace: [0..5] kernel counters, [6] max
@@ -340,6 +342,17 @@ inline int group_from_flags(unsigned flags, int64_t rays) {
     return 1 << (f - 1);
 }
 
+// Packet queries of variant 4 (bits 26..29; 0 = default).  Measured at 2^20
+// rays per replicate, 16 replicates of the 650-atom globule: 1 query 0.70,
+// 2 1.23, 3 1.32, 4 1.31, 5 1.30 G rays/s; the 2000-atom ellipsoid and 18
+// globules of 200-5000 atoms gain up to 5 queries (0.76 / 0.89 / 0.90 / 0.93
+// G rays/s for 2 / 3 / 4 / 5 on the ellipsoid).
+constexpr int kDefaultPackets = 3;
+inline int packets_from_flags(unsigned flags) {
+    const int f = (int)((flags >> EHSS_PACKETS_SHIFT) & 0xFu);
+    return f == 0 ? kDefaultPackets : f;
+}
+
 inline int stage1_queries(unsigned flags, int64_t rays) {
     const int f = (int)((flags >> EHSS_STAGE1_SHIFT) & 0xFu);
     if (f == 0) return rays >= (1 << 18) ? 2 : 0;
@@ -383,14 +396,13 @@ __device__ __forceinline__ void finish_coop(RayCtx& R, int32_t job, SceneOf scen
 
 // Number of wave-uniform queries a ray gets in the two-phase variant (4) before
 // it is queued for the per-lane drain kernel.
-constexpr int kPacketQueries = 2;
 
 // Kernel variants (template kMode):
 //   0 plain FP64: reference traversal, separate reference PA traversal, reference counters;
 //   1 FP32 filter, one ray per lane, PA fused into the first query;
 //   2 FP32 filter, wave-uniform (packet) first query with fused PA, later bounces per lane;
 //   3 FP32 filter, packet traversal for every bounce;
-//   4 FP32 filter, packet traversal for the first kPacketQueries queries, then the rays
+//   4 FP32 filter, packet traversal for the first A.packets queries, then the rays
 //     still bouncing are compacted into a queue (wave ballot, one atomic per wave)
 //     and finished by the persistent per-lane drain kernel.
 // Variants 1-4 return bit-identical per-ray results (the filter only skips spheres
@@ -463,7 +475,7 @@ __device__ __forceinline__ void solve_ray_at(const SceneView& S, const SolveArgs
             retests += r;
         };
         if (kMode == 3 || kMode == 4) {
-            for (int k = 1; kMode == 3 || k < kPacketQueries; ++k) {
+            for (int k = 1; kMode == 3 || k < A.packets; ++k) {
                 const bool live = !R.done;
                 const uint64_t m = __ballot(live);
                 if (m == 0) break;

@@ -269,6 +269,7 @@ int create_plan(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols,
         J.args.counters = W + (int64_t)kPlanWords * j;
         J.args.coop = p->coop;
         J.args.group = p->group;
+        J.args.packets = packets_from_flags(flags);
         J.args.times = nullptr;
         J.sob = sobols[j];
         make_ray_map(J.sob, powers[j], J.map.col);

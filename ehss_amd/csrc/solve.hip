@@ -581,6 +581,8 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     A.counters = ctr;
     A.coop = coop_from_flags(flags);
     A.group = group_from_flags(flags, total);
+    A.packets = packets_from_flags(flags);
+    A.pad_ = 0;
     A.times = nullptr;
     if (flags & EHSS_FLAG_TIMESTAMPS) {
         EHSS_TRY(sc->ray_times.reserve(sizeof(long long) * total));

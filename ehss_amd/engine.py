@@ -110,11 +110,16 @@ def precision_code(precision) -> int:
     return int(precision)
 
 
-def coop_flags(coop: int | None, group: int | None = None, stage1: int | None = None) -> int:
-    """Flag bits of a tail threshold, of the drain's lanes per ray and of the
-    queries a ray gets in the first drain (None: library defaults; stage1 0: one
-    drain)."""
+def coop_flags(coop: int | None, group: int | None = None, stage1: int | None = None,
+               packets: int | None = None) -> int:
+    """Flag bits of a tail threshold, of the drain's lanes per ray, of the
+    queries a ray gets in the first drain and of the packet queries before the
+    drain (None: library defaults; stage1 0: one drain)."""
     flags = 0
+    if packets is not None:
+        if not 1 <= int(packets) <= 15:
+            raise ValueError("packets must be in 1..15")
+        flags |= int(packets) << L.PACKETS_SHIFT
     if stage1 is not None:
         if not 0 <= int(stage1) <= 14:
             raise ValueError("stage1 must be in 0..14")
@@ -242,19 +247,20 @@ class GPUScene:
     def solve(self, sobol: L.Sobol, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
               keep_rays: bool = False, histogram: bool = False, variant: int = 0, coop: int | None = None,
               timestamps: bool = False, group: int | None = None, stage1: int | None = None,
-              lds: bool = False) -> dict:
+              lds: bool = False, packets: int | None = None) -> dict:
         """One fused device solve of 2^power rays.  `variant` selects the FP32
         filter kernel (0 default, 1 per-lane, 2 packet first query, 3 packet
         throughout, 4 packet first queries then queue + persistent drain);
         `coop` is the tail threshold (a wave with at most that many live rays
         finishes them with wave-cooperative queries; 0 off, None default);
-        `group` is the lanes per ray of variant 4's drain.  All settings return
-        identical per-ray results."""
+        `group` is the lanes per ray of variant 4's drain, `stage1` the queries a
+        ray gets there before a whole wave takes it, `packets` the packet queries
+        before the drain.  All settings return identical per-ray results."""
         totals = L.Totals()
         hist = np.zeros(cap + 1, np.int64) if histogram else None
         ledger = np.zeros(cap + 1, np.float64) if histogram else None
         flags = (L.FLAG_PA if pa else 0) | (L.FLAG_KEEP_RAYS if keep_rays else 0) | (int(variant) << L.VARIANT_SHIFT)
-        flags |= coop_flags(coop, group, stage1) | (L.FLAG_TIMESTAMPS if timestamps else 0)
+        flags |= coop_flags(coop, group, stage1, packets) | (L.FLAG_TIMESTAMPS if timestamps else 0)
         flags |= L.FLAG_LDS_PACKETS if lds else 0
         L.call("ehss_solve", self.handle, ct.byref(sobol), int(power), int(cap), precision_code(precision),
                flags, ct.byref(totals), L.ptr(hist), L.ptr(ledger))
@@ -267,7 +273,7 @@ class GPUScene:
     def solve_many(self, sobols, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
                    histogram: bool = False, variant: int = 0, coop: int | None = None, group: int | None = None,
                    keep_rays: bool = False, stage1: int | None = None, lds: bool = False,
-                   per_replicate: bool = False) -> list[dict]:
+                   per_replicate: bool = False, packets: int | None = None) -> list[dict]:
         """len(sobols) independent solves in one launch; one result dict per
         scramble, each equal to solve() with that scramble.  per_replicate:
         never schedule the replicates across the lanes of a wave."""
@@ -277,7 +283,8 @@ class GPUScene:
         hist = np.zeros(reps * (cap + 1), np.int64) if histogram else None
         ledger = np.zeros(reps * (cap + 1), np.float64) if histogram else None
         flags = (L.FLAG_PA if pa else 0) | (L.FLAG_KEEP_RAYS if keep_rays else 0) | \
-            (int(variant) << L.VARIANT_SHIFT) | coop_flags(coop, group, stage1) | (L.FLAG_LDS_PACKETS if lds else 0)
+            (int(variant) << L.VARIANT_SHIFT) | coop_flags(coop, group, stage1, packets) | \
+            (L.FLAG_LDS_PACKETS if lds else 0)
         flags |= L.FLAG_PER_REPLICATE if per_replicate else 0
         L.call("ehss_solve_many", self.handle, reps, table, int(power), int(cap), precision_code(precision), flags,
                totals, L.ptr(hist), L.ptr(ledger))
@@ -293,11 +300,13 @@ class GPUScene:
         return out
 
     def solve_async(self, sobol, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
-                    variant: int = 0, coop: int | None = None, group: int | None = None) -> None:
+                    variant: int = 0, coop: int | None = None, group: int | None = None,
+                    stage1: int | None = None, packets: int | None = None) -> None:
         """Enqueue solve() -- or, given a list of scrambles, solve_many() -- on
         the scene's stream and return at once; collect with solve_wait().
         Launches on different scenes overlap on the device."""
-        flags = (L.FLAG_PA if pa else 0) | (int(variant) << L.VARIANT_SHIFT) | coop_flags(coop, group)
+        flags = (L.FLAG_PA if pa else 0) | (int(variant) << L.VARIANT_SHIFT) | \
+            coop_flags(coop, group, stage1, packets)
         if isinstance(sobol, (list, tuple)):
             reps = len(sobol)
             table = (L.Sobol * reps)(*sobol)
@@ -444,9 +453,9 @@ class GPUPlan(_MultiSolve):
 
     def __init__(self, scenes, sobols, powers, caps, precision="fp32-fixup", pa: bool = True, graph: bool = False,
                  variant: int = 0, coop: int | None = None, group: int | None = None, stage1: int | None = None,
-                 lds: bool = False):
+                 lds: bool = False, packets: int | None = None):
         flags = (L.FLAG_PA if pa else 0) | (L.FLAG_GRAPH if graph else 0) | (int(variant) << L.VARIANT_SHIFT)
-        flags |= coop_flags(coop, group, stage1) | (L.FLAG_LDS_PACKETS if lds else 0)
+        flags |= coop_flags(coop, group, stage1, packets) | (L.FLAG_LDS_PACKETS if lds else 0)
         self._create(scenes, sobols, powers, caps, flags, precision)
 
 

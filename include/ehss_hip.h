@@ -77,6 +77,10 @@ extern "C" {
  * EHSS_STAGE1(0) = a single drain).  q <= 14. */
 #define EHSS_STAGE1_SHIFT 22
 #define EHSS_STAGE1(q) (((unsigned)(q) + 1u) << EHSS_STAGE1_SHIFT)
+/* Variant 4: wave-uniform (packet) queries every ray gets before the rays still
+ * bouncing are queued, in bits 26..29 (0 = default 3).  1 <= q <= 15. */
+#define EHSS_PACKETS_SHIFT 26
+#define EHSS_PACKETS(q) ((unsigned)(q) << EHSS_PACKETS_SHIFT)
 
 /* Library / device. */
 int ehss_abi_version(void);
@@ -297,7 +301,7 @@ int ehss_batch_destroy(ehss_batch* batch);
 typedef struct ehss_plan ehss_plan;
 
 /* flags: EHSS_FLAG_PA, EHSS_FLAG_GRAPH, EHSS_VARIANT(v), EHSS_COOP(n),
- * EHSS_GROUP_LOG2(k), EHSS_STAGE1(q); powers 0..26 each, at most 2^30 rays and
+ * EHSS_GROUP_LOG2(k), EHSS_STAGE1(q), EHSS_PACKETS(q); powers 0..26 each, at most 2^30 rays and
  * 65535 jobs per plan. */
 int ehss_plan_create(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols, const int32_t* powers,
                      const int32_t* caps, int precision, unsigned flags, ehss_plan** out);

@@ -7,8 +7,9 @@ from ehss_amd import autotune
 
 
 def separable_cost(cfg):
-    """Unique minimum at group 4, stage1 2, coop 16, lds False, leaf 8."""
-    c = {64: 3.0, 8: 2.0, 4: 1.0, 1: 4.0, None: 3.0}[cfg["group"]]
+    """Unique minimum at packets 4, group 4, stage1 2, coop 16, lds False, leaf 8."""
+    c = {None: 0.3, 3: 0.3, 2: 0.6, 4: 0.1, 5: 0.2}[cfg["packets"]]
+    c += {64: 3.0, 8: 2.0, 4: 1.0, 1: 4.0, None: 3.0}[cfg["group"]]
     c += {None: 0.5, 0: 0.7, 1: 0.4, 2: 0.1, 4: 0.3}[cfg["stage1"]]
     c += {None: 0.2, 8: 0.2, 4: 0.3, 16: 0.05}[cfg["coop"]]
     c += 0.0 if not cfg["lds"] else 0.25
@@ -24,7 +25,7 @@ def test_coordinate_descent_finds_the_separable_minimum(tmp_path):
         return separable_cost(cfg)
 
     res = autotune.tune(measure, repeats=1, cache_path=tmp_path / "t.json", key="k")
-    assert res.config == {"group": 4, "stage1": 2, "coop": 16, "lds": False, "leaf_size": 8}
+    assert res.config == {"packets": 4, "group": 4, "stage1": 2, "coop": 16, "lds": False, "leaf_size": 8}
     assert res.seconds == pytest.approx(separable_cost(res.config))
     assert res.default_seconds == pytest.approx(separable_cost(autotune._norm(autotune.DEFAULT_START)))
     assert res.speedup > 1 and not res.cached

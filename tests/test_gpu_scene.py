@@ -132,6 +132,11 @@ def test_plain_and_fp32_fixup_kernels_are_bit_identical(oracle, name):
             res = sc.solve(scramble(7), 16, 64, precision=prec, pa=True, keep_rays=True, histogram=True,
                            variant=variant, coop=coop, group=group, stage1=stage1)
             runs[(prec, variant, coop, group, stage1)] = (res, sc.last_rays(1 << 16))
+        for packets in (1, 2, 5, 15):  # packet queries before variant 4's queue
+            say("kernels", name, "packets", packets)
+            res = sc.solve(scramble(7), 16, 64, pa=True, keep_rays=True, histogram=True, variant=4,
+                           packets=packets)
+            runs[("fp32-fixup", 4, "packets", packets, None)] = (res, sc.last_rays(1 << 16))
         for variant in (3, 4):  # packet phase reading an LDS-staged scene
             say("kernels", name, "lds packets", variant)
             res = sc.solve(scramble(7), 16, 64, pa=True, keep_rays=True, histogram=True, variant=variant, lds=True)
