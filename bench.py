@@ -323,14 +323,16 @@ def main():
         plain_batched = 2 * STEPS_PER_LAUNCH * n / (time.perf_counter() - t1)
         say("CCS at 1% RMSE")
         power_ccs, rel = rays_per_ccs(scene, scramble, replicate_seed)
-        ccs_sobols = [scramble(replicate_seed(7, k)) for k in range(4096)]
+        ccs_sobols = [scramble(replicate_seed(7, k)) for k in range(16384)]
         seq_fast = ccs_rate_sequential(scene, ccs_sobols[:400], power_ccs, "fp32-fixup")
         seq_plain = ccs_rate_sequential(scene, ccs_sobols[:100], power_ccs, "fp64")
-        rep_fast = ccs_rate_replicated(scene, ccs_sobols, power_ccs, "fp32-fixup", 512)
-        rep_plain = ccs_rate_replicated(scene, ccs_sobols[:1024], power_ccs, "fp64", 512)
-        copies = [GPUScene(centers, radii) for _ in range(4)]
-        ovl_fast = ccs_rate_overlapped(copies, ccs_sobols, power_ccs, "fp32-fixup", 256)
-        ovl_plain = ccs_rate_overlapped(copies, ccs_sobols[:1024], power_ccs, "fp64", 256)
+        ccs_rate_replicated(scene, ccs_sobols[:4096], power_ccs, "fp32-fixup", 4096)
+        rep_fast = ccs_rate_replicated(scene, ccs_sobols, power_ccs, "fp32-fixup", 4096)
+        rep_plain = ccs_rate_replicated(scene, ccs_sobols[:4096], power_ccs, "fp64", 4096)
+        copies = [GPUScene(centers, radii) for _ in range(2)]
+        ccs_rate_overlapped(copies, ccs_sobols[:8192], power_ccs, "fp32-fixup", 4096)
+        ovl_fast = ccs_rate_overlapped(copies, ccs_sobols, power_ccs, "fp32-fixup", 4096)
+        ovl_plain = ccs_rate_overlapped(copies, ccs_sobols[:8192], power_ccs, "fp64", 4096)
         for c in copies:
             c.close()
         extras["globule650"] = dict(
@@ -344,8 +346,8 @@ def main():
         extras["ccs_1pct"] = dict(
             rays_per_ccs=1 << power_ccs, rel_sd_at_that_count=rel,
             ccs_per_s_plain_fp64_single_solves=seq_plain, ccs_per_s_fp32_fixup_single_solves=seq_fast,
-            ccs_per_s_plain_fp64_512_per_launch=rep_plain, ccs_per_s_fp32_fixup_512_per_launch=rep_fast,
-            ccs_per_s_plain_fp64_4_streams_x_256=ovl_plain, ccs_per_s_fp32_fixup_4_streams_x_256=ovl_fast,
+            ccs_per_s_plain_fp64_4096_per_launch=rep_plain, ccs_per_s_fp32_fixup_4096_per_launch=rep_fast,
+            ccs_per_s_plain_fp64_2_streams_x_4096=ovl_plain, ccs_per_s_fp32_fixup_2_streams_x_4096=ovl_fast,
             batched_fp32_fixup_vs_plain_fp64_kernel=best_fast / seq_plain,
             batched_fp32_fixup_vs_batched_plain_fp64=best_fast / max(rep_plain, ovl_plain),
             rays_per_s_at_ccs_size=best_fast * (1 << power_ccs))
@@ -397,6 +399,14 @@ def main():
         scene.close()
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(centers, radii)
+    if cpu is not None:
+        ratios = dict(rays_per_s_vs_cpu=value / cpu["value"])
+        if "ccs_1pct" in extras:
+            cc = extras["ccs_1pct"]
+            cpu_ccs = cpu["value"] / cc["rays_per_ccs"]
+            best = max(cc["ccs_per_s_fp32_fixup_4096_per_launch"], cc["ccs_per_s_fp32_fixup_2_streams_x_4096"])
+            ratios.update(cpu_ccs_per_s=cpu_ccs, ccs_per_s_vs_cpu=best / cpu_ccs)
+        extras["vs_cpu_baseline"] = ratios
     line = dict(metric=METRIC, value=value, unit="rays/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
                 ms_per_step=ms_per_step, higher_is_better=True, scaling="weak", vs_baseline=None,
                 dtype="f32+f64" if args.precision == "fp32-fixup" else "f64", data="synthetic",

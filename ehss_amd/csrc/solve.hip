@@ -7,8 +7,10 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "ehss_solve.h"
@@ -300,22 +302,50 @@ bool map_for_partition(const ehss_sobol& sob, int m, const std::array<int, 4>& a
 bool map_for_partition(const ehss_sobol& sob, int m, const std::array<int, 4>& a, uint32_t cols[32],
                        uint32_t* rows);
 
+// fn(i) for i < n on up to 8 host threads when n is large enough to pay for
+// them (the per-replicate ray maps and tables of launches with many replicates).
+template <class F>
+void parallel_for(int n, F fn) {
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int threads = std::min({hw, 8, n / 128});
+    if (threads <= 1) {
+        for (int i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    auto work = [&] {
+        for (int i; (i = next.fetch_add(32)) < n;)
+            for (int k = i; k < std::min(n, i + 32); ++k) fn(k);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+}
+
 // Partition chosen for (m, spatial), found once on the first scramble and reused:
-// it does not depend on the scramble.  {-1,...} when none exists.
-std::mutex g_part_mutex;
-std::map<std::pair<int, int>, std::array<int, 4>> g_part_cache;
+// it does not depend on the scramble.  State 0 unknown, 1 found, 2 none; the
+// value is written before the state is published, so readers need no lock.
+std::atomic<int> g_part_state[31][31];
+std::array<int, 4> g_part_value[31][31];
+std::mutex g_part_mutex;  // serialises the searches
+
+void publish_partition(int m, int spatial, const std::array<int, 4>& a, int state) {
+    std::lock_guard<std::mutex> lock(g_part_mutex);
+    if (g_part_state[m][spatial].load(std::memory_order_acquire) != 0) return;
+    g_part_value[m][spatial] = a;
+    g_part_state[m][spatial].store(state, std::memory_order_release);
+}
 
 bool build_map(const ehss_sobol& sob, int m, int spatial, uint32_t cols[32], uint32_t* rows,
                std::array<int, 4>* part) {
-    {
-        std::lock_guard<std::mutex> lock(g_part_mutex);
-        auto it = g_part_cache.find({m, spatial});
-        if (it != g_part_cache.end()) {
-            if (it->second[0] < 0) return false;
-            if (map_for_partition(sob, m, it->second, cols, rows)) {
-                if (part) *part = it->second;
-                return true;
-            }
+    const int state = g_part_state[m][spatial].load(std::memory_order_acquire);
+    if (state == 2) return false;
+    if (state == 1) {
+        const std::array<int, 4> cached = g_part_value[m][spatial];
+        if (map_for_partition(sob, m, cached, cols, rows)) {
+            if (part) *part = cached;
+            return true;
         }
     }
     std::vector<std::array<int, 4>> parts;
@@ -335,14 +365,10 @@ bool build_map(const ehss_sobol& sob, int m, int spatial, uint32_t cols[32], uin
     for (const auto& a : parts) {
         if (!map_for_partition(sob, m, a, cols, rows)) continue;
         if (part) *part = a;
-        std::lock_guard<std::mutex> lock(g_part_mutex);
-        g_part_cache[{m, spatial}] = a;
+        publish_partition(m, spatial, a, 1);
         return true;
     }
-    {
-        std::lock_guard<std::mutex> lock(g_part_mutex);
-        g_part_cache[{m, spatial}] = {-1, -1, -1, -1};
-    }
+    publish_partition(m, spatial, {-1, -1, -1, -1}, 2);
     return false;
 }
 
@@ -379,7 +405,7 @@ void make_cross_tables(const ehss_sobol* sobs, const CrossMap* maps, int reps, i
     uint32_t* off = col + (size_t)m * 4 * rp;
     uint32_t* ucol = off + (size_t)4 * rp;
     uint32_t* uoff = ucol + (size_t)m * rp;
-    for (int r = 0; r < reps; ++r) {
+    parallel_for(reps, [&](int r) {
         const CrossMap& X = maps[r];
         const ehss_sobol& sb = sobs[r];
         for (int k = 0; k < m; ++k) {
@@ -403,7 +429,7 @@ void make_cross_tables(const ehss_sobol* sobs, const CrossMap* maps, int reps, i
         for (int k = 0; k < m; ++k)
             if ((X.cell_shift >> k) & 1u) u ^= ucol[(size_t)k * rp + r];
         uoff[r] = u;
-    }
+    });
 }
 
 // Cross-replicate maps for `reps` scrambles of 2^m points: the finest shared
@@ -411,22 +437,12 @@ void make_cross_tables(const ehss_sobol* sobs, const CrossMap* maps, int reps, i
 // False when none of at least m - 6 bits exists or the replicates disagree.
 bool make_cross_maps(const ehss_sobol* sobs, int reps, int m, CrossMap* out) {
     for (int spatial = m; spatial >= std::max(1, m - 6); --spatial) {
-        std::array<int, 4> part0{};
-        bool all = true;
-        for (int r = 0; r < reps && all; ++r) {
+        // Replicate 0 fixes the partition; the others must share it.
+        auto one = [&](int r, std::array<int, 4>* part) -> bool {
             CrossMap& X = out[r];
             memset(&X, 0, sizeof X);
-            std::array<int, 4> part{};
             uint32_t rows2[32];
-            if (!build_map(sobs[r], m, spatial, X.inv2, rows2, &part)) {
-                all = false;
-                break;
-            }
-            if (r == 0) part0 = part;
-            if (part != part0) {
-                all = false;
-                break;
-            }
+            if (!build_map(sobs[r], m, spatial, X.inv2, rows2, part)) return false;
             uint32_t cols1[32];
             make_ray_map(sobs[r], m, cols1, X.rows1);
             if (m < 6)  // the single-solve slot is the inverse Gray code of g
@@ -434,9 +450,19 @@ bool make_cross_maps(const ehss_sobol* sobs, int reps, int m, CrossMap* out) {
             // Digital-shift bits of the spatial rows, in T2's row order.
             int k = 0;
             for (int d = 0; d < 4; ++d)
-                for (int p = 0; p < part[d]; ++p, ++k) X.cell_shift |= ((sobs[r].shift[d] >> (29 - p)) & 1u) << k;
-        }
-        if (all) return true;
+                for (int p = 0; p < (*part)[d]; ++p, ++k)
+                    X.cell_shift |= ((sobs[r].shift[d] >> (29 - p)) & 1u) << k;
+            return true;
+        };
+        std::array<int, 4> part0{};
+        if (!one(0, &part0)) continue;
+        std::atomic<bool> all{true};
+        parallel_for(reps - 1, [&](int i) {
+            if (!all.load(std::memory_order_relaxed)) return;
+            std::array<int, 4> part{};
+            if (!one(i + 1, &part) || part != part0) all.store(false, std::memory_order_relaxed);
+        });
+        if (all.load()) return true;
     }
     return false;
 }
@@ -542,10 +568,10 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     ehss_sobol* h_sob = sc->host_table.as<ehss_sobol>();
     RayMap* h_map = reinterpret_cast<RayMap*>(h_sob + reps);
     CrossMap* h_cross = reinterpret_cast<CrossMap*>(h_map + reps);
-    for (int r = 0; r < reps; ++r) {
+    parallel_for(reps, [&](int r) {
         h_sob[r] = sobs[r];
         make_ray_map(sobs[r], power, h_map[r].col);
-    }
+    });
     // Many replicates: schedule them across lanes when a shared partition exists.
     // Measured on the 650-atom globule: the cross schedule wins for small nets
     // (2^11 rays from 16 replicates, 2^14 from 64) and loses for large ones

@@ -90,3 +90,18 @@ def test_cross_replicate_schedule(oracle, power):
     q = np.stack(cells)  # (reps, n, 4)
     spread = [np.ptp(q[:, :, d].astype(np.float64) / 2 ** 30, axis=0).max() for d in range(4)]
     assert np.prod(np.maximum(spread, 1e-12)) <= 2.0 ** -(power - 6) + 1e-12
+
+
+@pytest.mark.parametrize("power", [5, 11, 14])
+def test_cross_maps_of_many_replicates_equal_one_at_a_time(power):
+    """The host builds the maps of large launches on several threads; the result
+    equals building each replicate's map alone."""
+    sob_list = [scramble(11 + 13 * k) for k in range(40)] * 13  # 520 replicates
+    sobs = (L.Sobol * len(sob_list))(*sob_list)
+    many = np.zeros((len(sob_list), 68), np.uint32)
+    L.call("ehss_cross_map", sobs, len(sob_list), power, L.ptr(many))
+    for r in range(40):
+        one = np.zeros((1, 68), np.uint32)
+        L.call("ehss_cross_map", ct.byref(sobs[r]), 1, power, L.ptr(one))
+        for k in range(13):
+            np.testing.assert_array_equal(many[r + 40 * k], one[0])
