@@ -17,6 +17,7 @@ from types import SimpleNamespace
 import numpy as np
 
 from .engine import GPUScene
+from .sources import CompactBoundary
 from .state import EHSSResult, Spheres
 
 
@@ -90,8 +91,6 @@ class PreparedStaticGPUEHSS:
     def __init__(self, spheres, groups=None, leaf_size=4, threads=1, boundary="sphere", simplify=False,
                  coverage_depth=4, precision="fp32-fixup"):
         check_threads(threads)
-        if boundary != "sphere":
-            raise ValueError("Only the enclosing-sphere impact boundary is supported")
         if simplify:
             raise ValueError("Atom simplification is not part of the static path")
         start = time.perf_counter()
@@ -107,8 +106,15 @@ class PreparedStaticGPUEHSS:
         self.scene = GPUScene(s.centers, s.radii, leaf_size=leaf_size, groups=groups)
         build_s = time.perf_counter() - tick
         self.model = SimpleNamespace(spheres=s, tree=_LazyTree(self.scene))
+        tick = time.perf_counter()
+        try:
+            self.boundary = CompactBoundary.fit(s, boundary)
+        except ValueError:
+            self.scene.close()
+            raise
+        boundary_s = time.perf_counter() - tick
         self.engine = GPUEHSSTransport(self.model, self.scene, threads=threads, precision=precision)
-        self.setup = dict(total_s=time.perf_counter() - start, group_s=0.0, bounds_s=build_s, boundary_s=0.0,
+        self.setup = dict(total_s=time.perf_counter() - start, group_s=0.0, bounds_s=build_s, boundary_s=boundary_s,
                           engine_s=0.0, device_build_s=build_s, threads=threads, geometry_threads=1,
                           preparation="static (device build)", response_metadata=False,
                           exclusivity_computed=False)
