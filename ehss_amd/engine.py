@@ -87,6 +87,14 @@ def sobol_points(seed: int, dim: int, m: int) -> np.ndarray:
     return q.astype(np.float64) * 2.0 ** -BITS
 
 
+def sobol_table(sobols):
+    """ctypes array of ehss_sobol for a list of scrambles (returned unchanged when
+    it already is one), so launches that reuse a set of scrambles convert it once."""
+    if isinstance(sobols, ct.Array) and sobols._type_ is L.Sobol:
+        return sobols
+    return (L.Sobol * len(sobols))(*sobols)
+
+
 def replicate_seed(seed: int, repeat: int) -> int:
     """ccs_cli.py:190: (seed + 104729 * repeat) mod 2^32."""
     return (int(seed) + 104729 * int(repeat)) % (2 ** 32)
@@ -273,12 +281,17 @@ class GPUScene:
     def solve_many(self, sobols, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
                    histogram: bool = False, variant: int = 0, coop: int | None = None, group: int | None = None,
                    keep_rays: bool = False, stage1: int | None = None, lds: bool = False,
-                   per_replicate: bool = False, packets: int | None = None) -> list[dict]:
+                   per_replicate: bool = False, packets: int | None = None, as_array: bool = False):
         """len(sobols) independent solves in one launch; one result dict per
         scramble, each equal to solve() with that scramble.  per_replicate:
-        never schedule the replicates across the lanes of a wave."""
-        reps = len(sobols)
-        table = (L.Sobol * reps)(*sobols)
+        never schedule the replicates across the lanes of a wave.  as_array: the
+        totals as one NumPy structured array (fields of ehss_totals) instead of
+        dicts, for launches of thousands of replicates.  `sobols` may be a
+        prebuilt sobol_table()."""
+        if as_array and histogram:
+            raise ValueError("as_array returns totals only")
+        table = sobol_table(sobols)
+        reps = len(table)
         totals = (L.Totals * reps)()
         hist = np.zeros(reps * (cap + 1), np.int64) if histogram else None
         ledger = np.zeros(reps * (cap + 1), np.float64) if histogram else None
@@ -288,6 +301,8 @@ class GPUScene:
         flags |= L.FLAG_PER_REPLICATE if per_replicate else 0
         L.call("ehss_solve_many", self.handle, reps, table, int(power), int(cap), precision_code(precision), flags,
                totals, L.ptr(hist), L.ptr(ledger))
+        if as_array:
+            return np.frombuffer(totals, dtype=L.TOTALS_DTYPE, count=reps).copy()
         return self._split(totals, reps, cap, hist, ledger)
 
     @staticmethod
@@ -307,9 +322,9 @@ class GPUScene:
         Launches on different scenes overlap on the device."""
         flags = (L.FLAG_PA if pa else 0) | (int(variant) << L.VARIANT_SHIFT) | \
             coop_flags(coop, group, stage1, packets)
-        if isinstance(sobol, (list, tuple)):
-            reps = len(sobol)
-            table = (L.Sobol * reps)(*sobol)
+        if isinstance(sobol, (list, tuple, ct.Array)):
+            table = sobol_table(sobol)
+            reps = len(table)
             L.call("ehss_solve_many_async", self.handle, reps, table, int(power), int(cap),
                    precision_code(precision), flags)
         else:
@@ -323,15 +338,19 @@ class GPUScene:
         L.call("ehss_solve_query", self.handle, ct.byref(flag))
         return bool(flag.value)
 
-    def solve_wait(self, histogram: bool = False):
+    def solve_wait(self, histogram: bool = False, as_array: bool = False):
         """Result of the pending launch: a dict for solve_async(sobol), a list of
-        dicts for solve_async([sobols])."""
+        dicts for solve_async([sobols]) (as_array: one structured array)."""
+        if as_array and histogram:
+            raise ValueError("as_array returns totals only")
         cap, reps = getattr(self, "_pending", (0, 0))
         n = max(reps, 1)
         totals = (L.Totals * n)()
         hist = np.zeros(n * (cap + 1), np.int64) if histogram else None
         ledger = np.zeros(n * (cap + 1), np.float64) if histogram else None
         L.call("ehss_solve_wait", self.handle, totals, L.ptr(hist), L.ptr(ledger))
+        if as_array:
+            return np.frombuffer(totals, dtype=L.TOTALS_DTYPE, count=n).copy()
         out = self._split(totals, n, cap, hist, ledger)
         return out if reps else out[0]
 

@@ -121,6 +121,27 @@ def test_per_replicate_schedule_equals_separate_solves(power, reps):
             np.testing.assert_array_equal(forced[k]["histogram"], one["histogram"])
 
 
+def test_array_totals_and_prebuilt_tables_equal_dicts():
+    from ehss_amd.engine import sobol_table
+    c, r = scenes.SCENES["c60"]()
+    with GPUScene(c, r) as s:
+        sobols = [scramble(40 + k) for k in range(300)]
+        table = sobol_table(sobols)
+        assert sobol_table(table) is table
+        say("array totals")
+        dicts = s.solve_many(sobols, 11, 64, pa=False)
+        arr = s.solve_many(table, 11, 64, pa=False, as_array=True)
+        assert len(arr) == 300
+        for k in (0, 150, 299):
+            for key in KEYS:
+                assert arr[key][k] == dicts[k][key], (k, key)
+        s.solve_async(table, 11, 64, pa=False)
+        again = s.solve_wait(as_array=True)
+        np.testing.assert_array_equal(again["omega"], arr["omega"])
+        with pytest.raises(ValueError):
+            s.solve_many(table, 11, 64, as_array=True, histogram=True)
+
+
 def test_replicate_launch_async_and_limits():
     c, r = scenes.SCENES["c60"]()
     with GPUScene(c, r) as s:
