@@ -7,9 +7,10 @@ Sobol rays -> impact-disk source -> BVH any-hit PA -> reflect-until-escape trace
 (cap 64) -> FP64 compensated (1 - cos chi) reduction.  The scene and its trees
 are resident in HBM before the timed region; each step uses its own replicate
 seed.  Steps are issued STEPS_PER_LAUNCH at a time as one multi-replicate launch
-(ehss_solve_many), whose per-step totals are bit-identical to separate solves.
-`value` is traced rays per second of the whole timed region (host wall clock);
-the rate with one step per launch is reported beside it.
+(ehss_solve_many), whose per-step totals are bit-identical to separate solves,
+with two launches in flight on two streams.  `value` is traced rays per second
+of the whole timed region (host wall clock); the rate with one step per launch
+is reported beside it.
 
 Side measurements (not part of `value`): the plain FP64 kernel on the same
 workload, CCS/s at 1 % relative RMSE (single stream, and with overlapped
@@ -260,30 +261,59 @@ def main():
     dev = L.device_info()
     say(f"device {dev['name']} CUs {dev['compute_units']}")
     scene = GPUScene(centers, radii)
-    seeds = [replicate_seed(SEED, k) for k in range(args.warmup + args.steps)]
+    # Warm-up launches are whole launches, at least one per stream, so that no
+    # workspace grows inside the timed region.
+    warm = max(args.warmup, 2 * STEPS_PER_LAUNCH)
+    seeds = [replicate_seed(SEED, k) for k in range(warm + args.steps)]
     sobols = [scramble(s) for s in seeds]
     n = 1 << POWER
 
     # Warmup, then exactly K timed steps in launches of STEPS_PER_LAUNCH (the
-    # last one takes the remainder); every launch returns after its stream has
-    # drained, so the region is bracketed by device synchronisation.
-    def run_steps(first, count):
-        ms, rays, out = [], [], None
-        for b in range(first, first + count, STEPS_PER_LAUNCH):
-            batch = sobols[b:min(b + STEPS_PER_LAUNCH, first + count)]
-            out = scene.solve_many(batch, POWER, CAP, precision=args.precision, pa=True)
-            ms.append(out[0]["kernel_ms"])
-            rays.append(len(batch) * n)
-        return ms, rays, out
+    # last one takes the remainder), two launches in flight on two streams,
+    # each over its own resident copy of the scene (the long paths of one
+    # launch overlap the packet phase of the next).  The region ends when the
+    # last launch has been collected, i.e. after device synchronisation.
+    from ehss_amd.engine import sobol_table
+    scene2 = GPUScene(centers, radii)
 
-    run_steps(0, args.warmup)
+    def batches(first, count):
+        return [sobol_table(sobols[b:min(b + STEPS_PER_LAUNCH, first + count)])
+                for b in range(first, first + count, STEPS_PER_LAUNCH)]
+
+    def run_overlapped(tabs):
+        pending, it, out = [], iter(tabs), None
+        for sc in (scene, scene2):
+            tb = next(it, None)
+            if tb is None:
+                break
+            sc.solve_async(tb, POWER, CAP, precision=args.precision, pa=True)
+            pending.append(sc)
+        while pending:
+            sc = pending.pop(0)
+            out = sc.solve_wait()
+            tb = next(it, None)
+            if tb is not None:
+                sc.solve_async(tb, POWER, CAP, precision=args.precision, pa=True)
+                pending.append(sc)
+        return out
+
+    run_overlapped(batches(0, warm))
+    timed = batches(warm, args.steps)
     t0 = time.perf_counter()
-    kernel_ms, launch_rays, many = run_steps(args.warmup, args.steps)
+    res = run_overlapped(timed)[-1]
     elapsed = time.perf_counter() - t0
-    res = many[-1]
     ms_per_step = 1e3 * elapsed / args.steps
     value = n * args.steps / elapsed
     say(f"{args.steps} steps, {ms_per_step:.3f} ms/step, {value:.4g} rays/s")
+    scene2.close()
+
+    # Kernel time per launch for the roofline: the same launches one at a time
+    # on one stream, HIP events around the trace kernels.
+    kernel_ms, launch_rays = [], []
+    for tb in timed:
+        out = scene.solve_many(tb, POWER, CAP, precision=args.precision, pa=True, as_array=True)
+        kernel_ms.append(float(out["kernel_ms"][0]))
+        launch_rays.append(len(tb) * n)
     t1 = time.perf_counter()
     for k in range(20):
         scene.solve(sobols[k], POWER, CAP, precision=args.precision, pa=True)
@@ -417,7 +447,8 @@ def main():
                 config=dict(workload="globule650: 650-atom synthetic globular molecule (seed 1), 2^20 scrambled-Sobol "
                                      "rays per step, cap 64, BVH any-hit PA, fused device solve",
                             atoms=len(radii), rays_per_step=n, cap=CAP, precision=args.precision,
-                            steps_per_launch=STEPS_PER_LAUNCH, parallelism="single GPU"),
+                            steps_per_launch=STEPS_PER_LAUNCH, launches_in_flight=2,
+                            parallelism="single GPU"),
                 roofline=roofline, cpu_baseline=cpu, device=dev["name"], **extras)
     print(json.dumps(line))
     return 0
