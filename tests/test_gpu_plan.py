@@ -163,3 +163,27 @@ def test_autotuned_plan_matches_solves_alone(tmp_path):
     finally:
         for s in scs:
             s.close()
+
+
+def test_two_plans_in_flight_over_shared_scenes():
+    """Two plans over the same resident scenes, launched together on their own
+    streams, give the results of each run alone."""
+    jobs = _jobs()[:4]
+    sc = [GPUScene(c, r) for (c, r), _, _ in jobs]
+    powers = [p for _, p, _ in jobs]
+    caps = [c for _, _, c in jobs]
+    try:
+        a_sob = [scramble(500 + j) for j in range(len(jobs))]
+        b_sob = [scramble(600 + j) for j in range(len(jobs))]
+        with GPUPlan(sc, a_sob, powers, caps, graph=True) as a, GPUPlan(sc, b_sob, powers, caps) as b:
+            alone_a, alone_b = a.run(histogram=True), b.run(histogram=True)
+            say("two plans in flight")
+            a.launch()
+            b.launch()
+            got_b, got_a = b.wait(histogram=True), a.wait(histogram=True)
+        for j in range(len(jobs)):
+            _same(alone_a[j], got_a[j], ("a", j))
+            _same(alone_b[j], got_b[j], ("b", j))
+    finally:
+        for s in sc:
+            s.close()
