@@ -439,23 +439,25 @@ __global__ void k_custom_offsets(BuildArgs A, const int32_t* cg, int32_t ng) {
 int build_scene_device(int32_t n, int32_t leaf_size, const double* d_centers, const double* d_radii, double4* d_sph,
                        int32_t* d_groups, int32_t* d_offsets, int32_t* d_eids, Node64* d_nodes, NodePA* d_pa,
                        int32_t* d_pa_ids, const int32_t* d_custom_groups, int32_t custom_ng, BuiltScene* out,
-                       hipStream_t s) {
-    // Temporaries.
-    DevBuf keys, ids2, pseg, pseg2, skey, rank, seg, ctl, flag, ndepth, nright, leaf_node, pabox;
+                       BuildWork& work, hipStream_t s) {
+    // Work arrays (grow-only, kept by the caller).
+    DevBuf &keys = work.keys, &ids2 = work.ids2, &pseg = work.pseg, &pseg2 = work.pseg2, &skey = work.skey,
+           &rank = work.rank, &seg = work.seg, &ctl = work.ctl, &flag = work.flag, &ndepth = work.ndepth,
+           &nright = work.nright, &leaf_node = work.leaf_node, &pabox = work.pabox;
     const int32_t max_segs = 2 * n + 2;
-    EHSS_TRY(keys.alloc(sizeof(double) * 3 * n));
-    EHSS_TRY(ids2.alloc(sizeof(int32_t) * n));
-    EHSS_TRY(pseg.alloc(sizeof(int32_t) * n));
-    EHSS_TRY(pseg2.alloc(sizeof(int32_t) * n));
-    EHSS_TRY(skey.alloc(sizeof(double) * n));
-    EHSS_TRY(rank.alloc(sizeof(int32_t) * n));
-    EHSS_TRY(seg.alloc(sizeof(int32_t) * 9 * (size_t)max_segs));
-    EHSS_TRY(ctl.alloc(sizeof(Ctl)));
-    EHSS_TRY(flag.alloc(sizeof(int32_t) * n));
-    EHSS_TRY(ndepth.alloc(sizeof(int32_t) * max_segs));
-    EHSS_TRY(nright.alloc(sizeof(int32_t) * max_segs));
-    EHSS_TRY(leaf_node.alloc(sizeof(int32_t) * (n + 1)));
-    EHSS_TRY(pabox.alloc(sizeof(NodePA) * max_segs));
+    EHSS_TRY(keys.reserve(sizeof(double) * 3 * n));
+    EHSS_TRY(ids2.reserve(sizeof(int32_t) * n));
+    EHSS_TRY(pseg.reserve(sizeof(int32_t) * n));
+    EHSS_TRY(pseg2.reserve(sizeof(int32_t) * n));
+    EHSS_TRY(skey.reserve(sizeof(double) * n));
+    EHSS_TRY(rank.reserve(sizeof(int32_t) * n));
+    EHSS_TRY(seg.reserve(sizeof(int32_t) * 9 * (size_t)max_segs));
+    EHSS_TRY(ctl.reserve(sizeof(Ctl)));
+    EHSS_TRY(flag.reserve(sizeof(int32_t) * n));
+    EHSS_TRY(ndepth.reserve(sizeof(int32_t) * max_segs));
+    EHSS_TRY(nright.reserve(sizeof(int32_t) * max_segs));
+    EHSS_TRY(leaf_node.reserve(sizeof(int32_t) * (n + 1)));
+    EHSS_TRY(pabox.reserve(sizeof(NodePA) * max_segs));
     EHSS_HIP(hipMemsetAsync(ctl.p, 0, sizeof(Ctl), s));
 
     BuildArgs A;

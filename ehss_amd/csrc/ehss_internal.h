@@ -141,10 +141,15 @@ int trace_host_rays(const SceneView& view, int64_t nr, int cap, int precision, i
 struct BuiltScene {
     int32_t n_groups = 0, n_nodes = 0, pa_nodes = 0, max_depth = 0;
 };
+// Device work arrays of a tree build, kept by the scene (grow-only) so that a
+// rebuild for a new conformer allocates and frees nothing.
+struct BuildWork {
+    DevBuf keys, ids2, pseg, pseg2, skey, rank, seg, ctl, flag, ndepth, nright, leaf_node, pabox;
+};
 int build_scene_device(int32_t n, int32_t leaf_size, const double* d_centers, const double* d_radii, double4* d_sph,
                        int32_t* d_groups, int32_t* d_offsets, int32_t* d_eids, Node64* d_nodes, NodePA* d_pa,
                        int32_t* d_pa_ids, const int32_t* d_custom_groups, int32_t custom_ng, BuiltScene* out,
-                       hipStream_t s);
+                       BuildWork& work, hipStream_t s);
 
 // Impact disk of ehss_state.py:46-47 computed on the host in NumPy's order:
 // center = sequential sum / n, radius = max(sqrt((dx^2 + dy^2) + dz^2) + r) * (1 + 1e-9).
@@ -188,7 +193,8 @@ struct ehss_scene {
     ehss::HostBuf host_table;  // scrambles and ray maps of the pending launch (staging)
     // solve workspace (grow-only)
     ehss::DevBuf ray_info, ray_contrib, partials, hist, ledger, counters, queue, queue2, ray_times, tables,
-        drain_stripes;
+        drain_stripes, custom_groups;
+    ehss::BuildWork build_work;  // tree-build work arrays, reused by ehss_scene_rebuild
     int64_t last_rays = 0, last_times = 0;
     uint32_t last_map[32] = {0};  // ray map of the last solve (slot -> ray index)
     int last_power = 0;

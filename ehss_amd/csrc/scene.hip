@@ -65,7 +65,7 @@ int build_frontier(ehss_scene* sc, hipStream_t s) {
         if (!split || next.size() > (size_t)kFrontier) break;
         f.swap(next);
     }
-    EHSS_TRY(sc->frontier.alloc(sizeof(int32_t) * f.size()));
+    EHSS_TRY(sc->frontier.reserve(sizeof(int32_t) * f.size()));
     EHSS_HIP(hipMemcpyAsync(sc->frontier.p, f.data(), sizeof(int32_t) * f.size(), hipMemcpyHostToDevice, s));
     EHSS_HIP(hipStreamSynchronize(s));  // f is a local
     sc->view.frontier = sc->frontier.as<int32_t>();
@@ -73,11 +73,10 @@ int build_frontier(ehss_scene* sc, hipStream_t s) {
     return EHSS_OK;
 }
 
-int create_scene(int64_t n, const double* centers, const double* radii, const int64_t* groups, int leaf_size,
-                 ehss_scene** out) {
-    EHSS_TRY(ensure_init());
-    EHSS_CHECK(out, "null output handle");
-    *out = nullptr;
+// Validates a structure and its optional groups (ehss_static.py:65-77); cg
+// receives the groups as int32 and *custom_ng their count (0 without groups).
+int check_structure(int64_t n, const double* centers, const double* radii, const int64_t* groups, int leaf_size,
+                    std::vector<int32_t>& cg, int32_t* custom_ng) {
     EHSS_CHECK(n >= 1 && n < (1LL << 28), "atom count %lld out of range", (long long)n);
     EHSS_CHECK(centers && radii, "null centers/radii");
     EHSS_CHECK(leaf_size >= 1, "Positive integer leaf size required");
@@ -86,8 +85,8 @@ int create_scene(int64_t n, const double* centers, const double* radii, const in
                        isfinite(radii[i]) && radii[i] > 0,
                    "Finite centers and positive finite radii required (atom %lld)", (long long)i);
     }
-    std::vector<int32_t> cg;
-    int32_t custom_ng = 0;
+    *custom_ng = 0;
+    cg.clear();
     if (groups) {
         // ehss_static.py:71-77: one integer group per atom, contiguous nonempty ids.
         int64_t gmax = -1;
@@ -100,21 +99,17 @@ int create_scene(int64_t n, const double* centers, const double* radii, const in
         for (int64_t g = 0; g <= gmax; ++g) EHSS_CHECK(seen[g], "Contiguous nonempty groups required");
         cg.resize((size_t)n);
         for (int64_t i = 0; i < n; ++i) cg[i] = (int32_t)groups[i];
-        custom_ng = (int32_t)(gmax + 1);
+        *custom_ng = (int32_t)(gmax + 1);
     }
-    ehss_scene* sc = new ehss_scene();
-    struct Guard {
-        ehss_scene*& p;
-        ~Guard() {
-            if (p) {
-                if (p->ev0) (void)hipEventDestroy(p->ev0);
-                if (p->ev1) (void)hipEventDestroy(p->ev1);
-                if (p->ev_done) (void)hipEventDestroy(p->ev_done);
-                if (p->stream) (void)hipStreamDestroy(p->stream);
-                delete p;
-            }
-        }
-    } guard{sc};
+    return EHSS_OK;
+}
+
+// Builds (or rebuilds) the scene's trees and filter arrays for a structure on
+// the scene's stream.  Every device array is grow-only, so a rebuild for a
+// structure no larger than the scene's largest so far allocates nothing.
+int build_into(ehss_scene* sc, int64_t n, const double* centers, const double* radii, const std::vector<int32_t>& cg,
+               int32_t custom_ng, int leaf_size) {
+    hipStream_t s = sc->stream;
     sc->n = (int32_t)n;
     sc->leaf_size = leaf_size;
     enclosing_disk(n, centers, radii, sc->center, &sc->radius);
@@ -125,38 +120,33 @@ int create_scene(int64_t n, const double* centers, const double* radii, const in
         scale = std::max(scale, e);
     }
     sc->scale = scale;
-    EHSS_HIP(hipStreamCreateWithFlags(&sc->stream, hipStreamNonBlocking));
-    EHSS_HIP(hipEventCreate(&sc->ev0));
-    EHSS_HIP(hipEventCreate(&sc->ev1));
-    EHSS_HIP(hipEventCreateWithFlags(&sc->ev_done, hipEventDisableTiming));
-    hipStream_t s = sc->stream;
-    EHSS_TRY(sc->centers.alloc(sizeof(double) * 3 * n));
-    EHSS_TRY(sc->radii.alloc(sizeof(double) * n));
-    EHSS_TRY(sc->sph.alloc(sizeof(double4) * n));
-    EHSS_TRY(sc->groups.alloc(sizeof(int32_t) * n));
-    EHSS_TRY(sc->offsets.alloc(sizeof(int32_t) * (n + 1)));
-    EHSS_TRY(sc->eids.alloc(sizeof(int32_t) * n));
-    EHSS_TRY(sc->nodes.alloc(sizeof(Node64) * (2 * n)));
-    EHSS_TRY(sc->pa.alloc(sizeof(NodePA) * (2 * n + 2)));
-    EHSS_TRY(sc->pa_ids.alloc(sizeof(int32_t) * n));
+    EHSS_TRY(sc->centers.reserve(sizeof(double) * 3 * n));
+    EHSS_TRY(sc->radii.reserve(sizeof(double) * n));
+    EHSS_TRY(sc->sph.reserve(sizeof(double4) * n));
+    EHSS_TRY(sc->groups.reserve(sizeof(int32_t) * n));
+    EHSS_TRY(sc->offsets.reserve(sizeof(int32_t) * (n + 1)));
+    EHSS_TRY(sc->eids.reserve(sizeof(int32_t) * n));
+    EHSS_TRY(sc->nodes.reserve(sizeof(Node64) * (2 * n)));
+    EHSS_TRY(sc->pa.reserve(sizeof(NodePA) * (2 * n + 2)));
+    EHSS_TRY(sc->pa_ids.reserve(sizeof(int32_t) * n));
     EHSS_HIP(hipMemcpyAsync(sc->centers.p, centers, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
     EHSS_HIP(hipMemcpyAsync(sc->radii.p, radii, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    DevBuf dcg;
-    if (groups) {
-        EHSS_TRY(dcg.alloc(sizeof(int32_t) * n));
-        EHSS_HIP(hipMemcpyAsync(dcg.p, cg.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+    if (custom_ng) {
+        EHSS_TRY(sc->custom_groups.reserve(sizeof(int32_t) * n));
+        EHSS_HIP(hipMemcpyAsync(sc->custom_groups.p, cg.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
     }
     BuiltScene built;
     EHSS_TRY(build_scene_device((int32_t)n, leaf_size, sc->centers.as<double>(), sc->radii.as<double>(),
                                 sc->sph.as<double4>(), sc->groups.as<int32_t>(), sc->offsets.as<int32_t>(),
                                 sc->eids.as<int32_t>(), sc->nodes.as<Node64>(), sc->pa.as<NodePA>(),
-                                sc->pa_ids.as<int32_t>(), groups ? dcg.as<int32_t>() : nullptr, custom_ng, &built,
-                                s));
+                                sc->pa_ids.as<int32_t>(), custom_ng ? sc->custom_groups.as<int32_t>() : nullptr,
+                                custom_ng, &built, sc->build_work, s));
     sc->ng = built.n_groups;
     sc->nn = built.n_nodes;
     sc->pa_nn = built.pa_nodes;
     sc->max_depth = built.max_depth;
     SceneView& v = sc->view;
+    v = SceneView();
     v.n_atoms = sc->n;
     v.n_groups = sc->ng;
     v.n_nodes = sc->nn;
@@ -172,10 +162,10 @@ int create_scene(int64_t n, const double* centers, const double* radii, const in
     v.cz = sc->center[2];
     v.scale = sc->scale;
     set_filter_margins(v);
-    EHSS_TRY(sc->nodes32.alloc(sizeof(Node32) * sc->nn));
-    EHSS_TRY(sc->sph32.alloc(sizeof(float4) * n));
-    EHSS_TRY(sc->sphs.alloc(sizeof(double4) * n));
-    EHSS_TRY(sc->leaves32.alloc(sizeof(Node32) * sc->ng));
+    EHSS_TRY(sc->nodes32.reserve(sizeof(Node32) * sc->nn));
+    EHSS_TRY(sc->sph32.reserve(sizeof(float4) * n));
+    EHSS_TRY(sc->sphs.reserve(sizeof(double4) * n));
+    EHSS_TRY(sc->leaves32.reserve(sizeof(Node32) * sc->ng));
     EHSS_TRY(launch_make_filter(v, sc->nodes32.as<Node32>(), sc->sph32.as<float4>(), sc->sphs.as<double4>(),
                                 sc->leaves32.as<Node32>(), s));
     v.nodes32 = sc->nodes32.as<Node32>();
@@ -184,6 +174,38 @@ int create_scene(int64_t n, const double* centers, const double* radii, const in
     v.leaves32 = sc->leaves32.as<Node32>();
     EHSS_TRY(build_frontier(sc, s));
     EHSS_HIP(hipStreamSynchronize(s));
+    // Per-ray records of an earlier structure no longer apply.
+    sc->last_rays = 0;
+    sc->last_times = 0;
+    return EHSS_OK;
+}
+
+int create_scene(int64_t n, const double* centers, const double* radii, const int64_t* groups, int leaf_size,
+                 ehss_scene** out) {
+    EHSS_TRY(ensure_init());
+    EHSS_CHECK(out, "null output handle");
+    *out = nullptr;
+    std::vector<int32_t> cg;
+    int32_t custom_ng = 0;
+    EHSS_TRY(check_structure(n, centers, radii, groups, leaf_size, cg, &custom_ng));
+    ehss_scene* sc = new ehss_scene();
+    struct Guard {
+        ehss_scene*& p;
+        ~Guard() {
+            if (p) {
+                if (p->ev0) (void)hipEventDestroy(p->ev0);
+                if (p->ev1) (void)hipEventDestroy(p->ev1);
+                if (p->ev_done) (void)hipEventDestroy(p->ev_done);
+                if (p->stream) (void)hipStreamDestroy(p->stream);
+                delete p;
+            }
+        }
+    } guard{sc};
+    EHSS_HIP(hipStreamCreateWithFlags(&sc->stream, hipStreamNonBlocking));
+    EHSS_HIP(hipEventCreate(&sc->ev0));
+    EHSS_HIP(hipEventCreate(&sc->ev1));
+    EHSS_HIP(hipEventCreateWithFlags(&sc->ev_done, hipEventDisableTiming));
+    EHSS_TRY(build_into(sc, n, centers, radii, cg, custom_ng, leaf_size));
     *out = sc;
     sc = nullptr;  // release the guard
     return EHSS_OK;
@@ -211,6 +233,16 @@ int ehss_scene_create_grouped(int64_t n, const double* centers, const double* ra
                               int leaf_size, ehss_scene** out) {
     EHSS_CHECK(groups, "null groups");
     return create_scene(n, centers, radii, groups, leaf_size, out);
+}
+
+int ehss_scene_rebuild(ehss_scene* scene, int64_t n, const double* centers, const double* radii, int leaf_size) {
+    EHSS_CHECK(scene, "null scene");
+    if (scene->pending) return set_error(EHSS_ERR_BUSY, "a solve is pending on this scene");
+    std::vector<int32_t> cg;
+    int32_t custom_ng = 0;
+    EHSS_TRY(check_structure(n, centers, radii, nullptr, leaf_size, cg, &custom_ng));
+    EHSS_HIP(hipStreamSynchronize(scene->stream));
+    return build_into(scene, n, centers, radii, cg, custom_ng, leaf_size);
 }
 
 int ehss_scene_destroy(ehss_scene* scene) {

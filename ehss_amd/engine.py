@@ -7,6 +7,7 @@ C structures.
 from __future__ import annotations
 
 import ctypes as ct
+import functools
 import time
 
 import numpy as np
@@ -20,8 +21,19 @@ BITS = 30
 _JOE_KUO = ((1, 0, (1,)), (2, 1, (1, 3)), (3, 1, (1, 3, 1)), (3, 2, (1, 1, 1)))
 
 
+@functools.lru_cache(maxsize=None)
+def _direction_numbers(dim: int, bits: int) -> np.ndarray:
+    v = _direction_numbers_uncached(dim, bits)
+    v.setflags(write=False)
+    return v
+
+
 def direction_numbers(dim: int = 4, bits: int = BITS) -> np.ndarray:
     """v[d, j] = m_j << (bits - 1 - j), m_j from the Bratley-Fox recurrence."""
+    return _direction_numbers(int(dim), int(bits)).copy()
+
+
+def _direction_numbers_uncached(dim: int, bits: int) -> np.ndarray:
     if not 1 <= dim <= len(_JOE_KUO) + 1:
         raise ValueError(f"dimension must be 1..{len(_JOE_KUO) + 1}")
     v = np.zeros((dim, bits), dtype=np.int64)
@@ -50,26 +62,26 @@ def scramble_matrices(seed: int, dim: int = 4, bits: int = BITS):
     shift_bits = rng.integers(0, 2, size=(dim, bits), dtype=np.uint32).astype(np.int64)
     shift = shift_bits @ (np.int64(1) << np.arange(bits, dtype=np.int64))
     ltm = np.tril(rng.integers(0, 2, size=(dim, bits, bits), dtype=np.uint32)).astype(np.int64)
-    v = direction_numbers(dim, bits)
+    idx = np.arange(bits)
+    ltm[:, idx, idx] = 1                                    # unit diagonal
+    v = _direction_numbers(int(dim), int(bits))
     msb = np.int64(1) << (bits - 1 - np.arange(bits, dtype=np.int64))  # weight of MSB-first bit p
-    sv = np.zeros_like(v)
-    for d in range(dim):
-        lmat = ltm[d].copy()
-        np.fill_diagonal(lmat, 1)
-        vb = (v[d][:, None] & msb[None, :]) != 0           # vb[j, k]: MSB-first bit k of v[d, j]
-        out = (vb.astype(np.int64) @ lmat.T) & 1            # out[j, p] = parity(L[p, :] . vb[j, :])
-        sv[d] = out @ msb
+    vb = ((v[:, :, None] & msb[None, None, :]) != 0).astype(np.float64)  # vb[d, j, k]: MSB-first bit k of v[d, j]
+    # out[d, j, p] = parity(L[d, p, :] . vb[d, j, :]); the dot products are
+    # small integers (<= bits), exact in FP64.
+    out = np.matmul(vb, ltm.transpose(0, 2, 1).astype(np.float64)).astype(np.int64) & 1
+    sv = out @ msb
     return sv.astype(np.uint32), shift.astype(np.uint32)
 
 
 def scramble(seed: int) -> L.Sobol:
     """ehss_sobol structure for a 4-D scrambled net with the given seed."""
     sv, shift = scramble_matrices(seed, 4)
+    words = np.zeros(4 * 32 + 4, dtype=np.uint32)
+    words[:4 * 32].reshape(4, 32)[:, :BITS] = sv
+    words[4 * 32:] = shift
     s = L.Sobol()
-    for d in range(4):
-        for j in range(BITS):
-            s.sv[d][j] = int(sv[d, j])
-        s.shift[d] = int(shift[d])
+    ct.memmove(ct.addressof(s), words.ctypes.data, words.nbytes)
     return s
 
 
@@ -166,6 +178,24 @@ class GPUScene:
             L.call("ehss_scene_create_grouped", len(r), L.ptr(c), L.ptr(r), L.ptr(g), leaf_size, ct.byref(handle))
         self.build_s = time.perf_counter() - t0
         self._h = handle
+        info = L.SceneInfo()
+        L.call("ehss_scene_get_info", self._h, ct.byref(info))
+        self.info = {name: (list(getattr(info, name)) if name == "center" else getattr(info, name))
+                     for name, _ in L.SceneInfo._fields_}
+
+    def rebuild(self, centers, radii, leaf_size: int | None = None) -> None:
+        """Replace the structure (a new conformer) in place: the device arrays
+        are reused and only grow (ehss_scene_rebuild)."""
+        c, r = _f64(centers), _f64(radii)
+        if c.ndim != 2 or c.shape[1] != 3 or r.shape != (len(c),) or len(c) == 0:
+            raise ValueError("Expected nonempty centers (N,3) and radii (N,).")
+        leaf = self.info["leaf_size"] if leaf_size is None else leaf_size
+        if type(leaf) is not int or leaf < 1:
+            raise ValueError("Positive integer leaf size required")
+        t0 = time.perf_counter()
+        L.call("ehss_scene_rebuild", self.handle, len(r), L.ptr(c), L.ptr(r), leaf)
+        self.build_s = time.perf_counter() - t0
+        self.centers, self.radii = c, r
         info = L.SceneInfo()
         L.call("ehss_scene_get_info", self._h, ct.byref(info))
         self.info = {name: (list(getattr(info, name)) if name == "center" else getattr(info, name))

@@ -17,6 +17,10 @@ Each request is a structure (centres, radii).  The loop
   first launch of a request runs min_reps replicates together; each later one
   runs as many as the spread so far predicts are still needed, so a request
   takes one or two launches rather than one per replicate;
+* keeps the scenes of finished requests idle (up to max_inflight scenes in all)
+  and rebuilds an idle one in place for the next request (ehss_scene_rebuild:
+  the device arrays only grow), so that a request costs no device allocation
+  or free -- a free synchronises the whole device;
 * records arrival, admission and completion times, from which p50/p99 latency
   and CCS/s follow.
 
@@ -107,8 +111,10 @@ class Server:
     """Continuous-batching CCS server over device-resident scenes.
 
     scene_factory(centers, radii) must return an object with the GPUScene
-    asynchronous interface (solve_async, solve_done, solve_wait, close);
-    scramble(seed) and replicate_seed(seed, k) give the per-replicate nets."""
+    asynchronous interface (solve_async, solve_done, solve_wait, close); when
+    it also has rebuild(centers, radii) and device_bytes(), finished scenes are
+    reused.  scramble(seed) and replicate_seed(seed, k) give the per-replicate
+    nets."""
 
     def __init__(self, budget_bytes: int, max_inflight: int = 16, power: int = 12, cap: int = 64,
                  target_rel: float = 0.01, min_reps: int = 4, max_reps: int = 64, precision: str = "fp32-fixup",
@@ -129,6 +135,8 @@ class Server:
         self.poll_s = poll_s
         self.peak_bytes = 0
         self.peak_inflight = 0
+        self.scenes_built = 0
+        self.scenes_reused = 0
 
     def footprint(self, req: Request) -> int:
         return footprint_bytes(len(req.radii), self.power, self.cap, self.max_reps)
@@ -165,8 +173,36 @@ class Server:
         waiting: collections.deque = collections.deque()
         live: dict[int, _Live] = {}
         done: list[Result] = []
-        used = 0
+        idle: list = []  # (device bytes, scene) of finished requests, kept for reuse
+        used = 0         # footprints of live requests plus the bytes of idle scenes
         t0 = clock()
+
+        def take_scene(req, fp):
+            """A scene holding req: an idle one rebuilt in place, or a new one.
+            Returns (scene, bytes accounted for it)."""
+            if idle:
+                held, scene = idle.pop()
+                try:
+                    scene.rebuild(req.centers, req.radii)
+                except Exception:
+                    idle.append((held, scene))
+                    raise
+                self.scenes_reused += 1
+                return scene, max(fp, held)
+            scene = self.scene_factory(req.centers, req.radii)
+            self.scenes_built += 1
+            return scene, fp
+
+        def release_scene(scene):
+            nonlocal used
+            if hasattr(scene, "rebuild") and hasattr(scene, "device_bytes") and \
+                    len(idle) + len(live) < self.max_inflight:
+                held = int(scene.device_bytes())
+                idle.append((held, scene))
+                used += held
+            else:
+                scene.close()
+
         while pending or waiting or live:
             now = clock() - t0
             while pending and pending[0].arrival <= now:
@@ -180,18 +216,29 @@ class Server:
                     done.append(Result(req.rid, len(req.radii), arrival=req.arrival, finished=clock() - t0,
                                        error=f"footprint {fp} B exceeds the device budget {self.budget} B"))
                     continue
+                # Memory held by idle scenes goes first when a request does not fit.
+                while used + fp > self.budget and idle:
+                    held, scene = idle.pop(0)
+                    scene.close()
+                    used -= held
                 if used + fp > self.budget:
                     break
                 waiting.popleft()
                 res = Result(req.rid, len(req.radii), arrival=req.arrival, admitted=clock() - t0)
+                if idle:
+                    used -= idle[-1][0]  # the scene about to be reused is accounted as live
+                    held_before = idle[-1][0]
+                else:
+                    held_before = 0
                 try:
-                    scene = self.scene_factory(req.centers, req.radii)
+                    scene, acct = take_scene(req, fp)
                 except Exception as exc:  # invalid structure: report, keep serving
+                    used += held_before   # the idle scene stays idle
                     res.error, res.finished = str(exc), clock() - t0
                     done.append(res)
                     continue
-                item = _Live(req, res, scene, fp)
-                used += fp
+                item = _Live(req, res, scene, acct)
+                used += acct
                 live[req.rid] = item
                 self._launch(item)
             self.peak_bytes = max(self.peak_bytes, used)
@@ -213,9 +260,9 @@ class Server:
                     res.ccs = float(np.mean(res.omegas))
                     res.rel_se = _rel_se(res.omegas)
                     res.finished = clock() - t0
-                    item.scene.close()
                     used -= item.footprint
                     del live[rid]
+                    release_scene(item.scene)
                     done.append(res)
                 else:
                     self._launch(item)
@@ -224,6 +271,8 @@ class Server:
                     sleep(max(0.0, min(pending[0].arrival - (clock() - t0), 0.01)))
                 else:
                     sleep(self.poll_s)
+        for _, scene in idle:
+            scene.close()
         return done
 
 

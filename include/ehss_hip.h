@@ -136,6 +136,15 @@ int ehss_scene_create(int64_t n, const double* centers, const double* radii, int
  * the PA tree is still the median-split tree of leaf_size. */
 int ehss_scene_create_grouped(int64_t n, const double* centers, const double* radii, const int64_t* groups,
                               int leaf_size, ehss_scene** out);
+/* Replaces the scene's structure by another (a new conformer), as
+ * PreparedCPUEHSS.update(spheres) does for the reference's prepared geometry
+ * (ehss_cpu_workload.py:35-46): the trees and filter arrays are rebuilt on the
+ * scene's stream into the scene's existing device arrays, which only grow, so a
+ * structure no larger than the largest one the scene has held allocates and
+ * frees nothing.  Fails with EHSS_ERR_BUSY while a solve is pending; per-ray
+ * records of the previous structure are dropped.  Same validation as
+ * ehss_scene_create. */
+int ehss_scene_rebuild(ehss_scene* scene, int64_t n, const double* centers, const double* radii, int leaf_size);
 int ehss_scene_destroy(ehss_scene* scene);
 int ehss_scene_get_info(const ehss_scene* scene, ehss_scene_info* info);
 /* Copies the static tree back (arrays sized from ehss_scene_get_info):

@@ -3,6 +3,7 @@ device Sobol source and the fused solve, against the oracle."""
 import numpy as np
 import pytest
 
+from ehss_amd import _lib as L
 from ehss_amd.engine import GPUScene, scramble
 from tests import scenes
 from tests.gpu_helpers import say
@@ -202,3 +203,36 @@ def test_caller_groups_build(oracle):
     for bad in ([0, 2], [-1, 0]):
         with pytest.raises(L.EHSSError, match="Contiguous"):
             GPUScene(np.array([[0., 0, 0], [2, 0, 0]]), np.ones(2), groups=bad)
+
+
+def test_rebuild_in_place_equals_a_fresh_scene():
+    """ehss_scene_rebuild: trees, PA tree and solves equal a scene created for
+    the new structure, growing and shrinking; busy while a solve is pending."""
+    structs = [scenes.globule(300, 4), scenes.globule(900, 6), scenes.SCENES["c60"](), scenes.globule(650, 1)]
+    with GPUScene(*structs[0]) as sc:
+        for k, (c, r) in enumerate(structs[1:]):
+            say("rebuild", k, len(r))
+            sc.rebuild(c, r)
+            with GPUScene(c, r) as fresh:
+                a, b = sc.export_tree(), fresh.export_tree()
+                for key in a:
+                    np.testing.assert_array_equal(a[key], b[key], err_msg=key)
+                pa, pb = sc.export_pa_tree(), fresh.export_pa_tree()
+                for key in pa:
+                    np.testing.assert_array_equal(pa[key], pb[key], err_msg=key)
+                assert sc.info["atoms"] == len(r) and sc.info["radius"] == fresh.info["radius"]
+                x = sc.solve(scramble(5), 14, 64, histogram=True)
+                y = fresh.solve(scramble(5), 14, 64, histogram=True)
+                for key in ("omega", "pa", "escaped", "unresolved", "max_bounces"):
+                    assert x[key] == y[key], (k, key)
+                np.testing.assert_array_equal(x["histogram"], y["histogram"])
+        sc.solve_async(scramble(1), 12, 64)
+        with pytest.raises(L.EHSSError, match="pending"):
+            sc.rebuild(*structs[0])
+        sc.solve_wait()
+        sc.solve(scramble(2), 12, 64, keep_rays=True)
+        sc.rebuild(*structs[0])
+        with pytest.raises(L.EHSSError, match="KEEP_RAYS"):
+            sc.last_rays(1 << 12)
+        with pytest.raises(ValueError):
+            sc.rebuild(np.zeros((0, 3)), np.zeros(0))

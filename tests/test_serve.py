@@ -142,3 +142,58 @@ def test_limits_are_validated():
     with pytest.raises(ValueError):
         serve.Server(1 << 30, min_reps=5, max_reps=4, scene_factory=object, scramble=object,
                      replicate_seed=object)
+
+
+class ReusableScene(FakeScene):
+    """FakeScene with in-place rebuild (ehss_scene_rebuild) and a byte count."""
+    rebuilds = 0
+
+    def rebuild(self, centers, radii):
+        assert self.busy_until is None and not self.closed, "rebuild of a busy or closed scene"
+        if len(radii) == 0:
+            raise ValueError("Expected nonempty centers (N,3) and radii (N,).")
+        self.n = max(self.n, len(radii))
+        self.base = math.pi * (1.0 + float(np.max(np.linalg.norm(centers, axis=1)) + np.max(radii))) ** 2
+        ReusableScene.rebuilds += 1
+
+    def device_bytes(self):
+        return serve.footprint_bytes(self.n, 10, 64, 64)
+
+
+def test_finished_scenes_are_rebuilt_for_later_requests_with_the_same_results():
+    reqs = stream(count=40)
+    clock = Clock()
+    plain = {r.rid: r for r in make_server(clock, max_inflight=4, power=10).run(reqs, clock=clock, sleep=clock.sleep)}
+    clock = Clock()
+    ReusableScene.rebuilds = 0
+    srv = serve.Server(1 << 40, scene_factory=lambda c, r: ReusableScene(clock, c, r), scramble=lambda s: s,
+                       replicate_seed=lambda s, k: (s + 104729 * k) % 2 ** 32, max_inflight=4, power=10)
+    got = {r.rid: r for r in srv.run(reqs, clock=clock, sleep=clock.sleep)}
+    assert FakeScene.live == 0
+    assert srv.scenes_reused > 0 and srv.scenes_built <= 4 and ReusableScene.rebuilds == srv.scenes_reused
+    for rid, r in plain.items():
+        assert got[rid].omegas == r.omegas and got[rid].ccs == r.ccs
+
+
+def test_idle_scenes_give_way_to_the_budget():
+    reqs = stream(count=30, rate=1e6)
+    fps = sorted(serve.footprint_bytes(len(r.radii), 10, 64, 64) for r in reqs)
+    budget = fps[-1] + fps[-2]
+    clock = Clock()
+    srv = serve.Server(budget, scene_factory=lambda c, r: ReusableScene(clock, c, r), scramble=lambda s: s,
+                       replicate_seed=lambda s, k: (s + 104729 * k) % 2 ** 32, max_inflight=8, power=10)
+    res = srv.run(reqs, clock=clock, sleep=clock.sleep)
+    assert all(r.error is None for r in res) and len(res) == 30
+    assert srv.peak_bytes <= budget and FakeScene.live == 0
+
+
+def test_invalid_structure_with_an_idle_scene_keeps_serving():
+    clock = Clock()
+    srv = serve.Server(1 << 40, scene_factory=lambda c, r: ReusableScene(clock, c, r), scramble=lambda s: s,
+                       replicate_seed=lambda s, k: (s + 104729 * k) % 2 ** 32, max_inflight=2, power=10)
+    good = stream(count=3, rate=10.0)
+    bad = serve.Request(77, np.zeros((0, 3)), np.zeros(0), arrival=good[-1].arrival + 1.0)
+    after = serve.Request(78, good[0].centers, good[0].radii, arrival=bad.arrival + 0.1, seed=5)
+    res = {r.rid: r for r in srv.run(good + [bad, after], clock=clock, sleep=clock.sleep)}
+    assert res[77].error and "nonempty" in res[77].error
+    assert res[78].error is None and FakeScene.live == 0
