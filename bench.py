@@ -42,6 +42,7 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 POWER, CAP, SEED = 20, 64, 1
 STEPS_PER_LAUNCH = 16     # 2^20-ray steps per launch: 1.25 G rays/s vs 0.53 one per launch
+CCS_PER_LAUNCH = 8192     # 2,048-ray CCS estimates per launch (0.63 M CCS/s with two launches in flight)
 COPY_BYTES = 256 << 20    # calibration copy of the traffic probe (bytes read = bytes written)
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_solve_globule650.txt"
 
@@ -374,16 +375,17 @@ def main():
         plain_batched = 2 * STEPS_PER_LAUNCH * n / (time.perf_counter() - t1)
         say("CCS at 1% RMSE")
         power_ccs, rel = rays_per_ccs(scene, scramble, replicate_seed)
-        ccs_sobols = [scramble(replicate_seed(7, k)) for k in range(16384)]
+        per = CCS_PER_LAUNCH
+        ccs_sobols = [scramble(replicate_seed(7, k)) for k in range(4 * per)]
         seq_fast = ccs_rate_sequential(scene, ccs_sobols[:400], power_ccs, "fp32-fixup")
         seq_plain = ccs_rate_sequential(scene, ccs_sobols[:100], power_ccs, "fp64")
-        ccs_rate_replicated(scene, ccs_sobols[:4096], power_ccs, "fp32-fixup", 4096)
-        rep_fast = ccs_rate_replicated(scene, ccs_sobols, power_ccs, "fp32-fixup", 4096)
-        rep_plain = ccs_rate_replicated(scene, ccs_sobols[:4096], power_ccs, "fp64", 4096)
+        ccs_rate_replicated(scene, ccs_sobols[:per], power_ccs, "fp32-fixup", per)
+        rep_fast = ccs_rate_replicated(scene, ccs_sobols, power_ccs, "fp32-fixup", per)
+        rep_plain = ccs_rate_replicated(scene, ccs_sobols[:per], power_ccs, "fp64", per)
         copies = [GPUScene(centers, radii) for _ in range(2)]
-        ccs_rate_overlapped(copies, ccs_sobols[:8192], power_ccs, "fp32-fixup", 4096)
-        ovl_fast = ccs_rate_overlapped(copies, ccs_sobols, power_ccs, "fp32-fixup", 4096)
-        ovl_plain = ccs_rate_overlapped(copies, ccs_sobols[:8192], power_ccs, "fp64", 4096)
+        ccs_rate_overlapped(copies, ccs_sobols[:2 * per], power_ccs, "fp32-fixup", per)
+        ovl_fast = ccs_rate_overlapped(copies, ccs_sobols, power_ccs, "fp32-fixup", per)
+        ovl_plain = ccs_rate_overlapped(copies, ccs_sobols[:2 * per], power_ccs, "fp64", per)
         for c in copies:
             c.close()
         extras["globule650"] = dict(
@@ -397,8 +399,9 @@ def main():
         extras["ccs_1pct"] = dict(
             rays_per_ccs=1 << power_ccs, rel_sd_at_that_count=rel,
             ccs_per_s_plain_fp64_single_solves=seq_plain, ccs_per_s_fp32_fixup_single_solves=seq_fast,
-            ccs_per_s_plain_fp64_4096_per_launch=rep_plain, ccs_per_s_fp32_fixup_4096_per_launch=rep_fast,
-            ccs_per_s_plain_fp64_2_streams_x_4096=ovl_plain, ccs_per_s_fp32_fixup_2_streams_x_4096=ovl_fast,
+            replicates_per_launch=per,
+            ccs_per_s_plain_fp64_one_launch=rep_plain, ccs_per_s_fp32_fixup_one_launch=rep_fast,
+            ccs_per_s_plain_fp64_2_streams=ovl_plain, ccs_per_s_fp32_fixup_2_streams=ovl_fast,
             batched_fp32_fixup_vs_plain_fp64_kernel=best_fast / seq_plain,
             batched_fp32_fixup_vs_batched_plain_fp64=best_fast / max(rep_plain, ovl_plain),
             rays_per_s_at_ccs_size=best_fast * (1 << power_ccs))
@@ -438,14 +441,17 @@ def main():
         for k, na in enumerate(sizes):
             xyz, el = molecules.globule(int(na), 500 + k)
             bases.append((xyz, molecules.radii_for(el)))
-        reqs = serve.poisson_stream(bases, rate_hz=200.0, count=120, seed=9)
-        srv = serve.Server(int(0.8 * dev["total_mem"]), max_inflight=16, power=12, cap=64, target_rel=0.01,
-                           min_reps=4, max_reps=64)
-        results = srv.run(reqs)
-        stats = serve.latency_summary(results)
-        extras["serving"] = dict(stats, rate_hz=200.0, requests=len(reqs), base_atoms=[int(s) for s in sizes],
-                                 rays_per_replicate=1 << 12, peak_inflight=srv.peak_inflight,
-                                 peak_admitted_bytes=srv.peak_bytes)
+        runs = {}
+        for rate, count in ((200.0, 120), (2000.0, 600)):
+            reqs = serve.poisson_stream(bases, rate_hz=rate, count=count, seed=9)
+            srv = serve.Server(int(0.8 * dev["total_mem"]), max_inflight=16, power=12, cap=64, target_rel=0.01,
+                               min_reps=4, max_reps=64)
+            results = srv.run(reqs)
+            runs[rate] = dict(serve.latency_summary(results), rate_hz=rate, requests=len(reqs),
+                              peak_inflight=srv.peak_inflight, peak_admitted_bytes=srv.peak_bytes,
+                              scenes_built=srv.scenes_built, scenes_rebuilt=srv.scenes_reused)
+        extras["serving"] = dict(runs[200.0], base_atoms=[int(s) for s in sizes], rays_per_replicate=1 << 12,
+                                 saturated_2000_hz=runs[2000.0])
     else:
         scene.close()
 
@@ -455,7 +461,7 @@ def main():
         if "ccs_1pct" in extras:
             cc = extras["ccs_1pct"]
             cpu_ccs = cpu["value"] / cc["rays_per_ccs"]
-            best = max(cc["ccs_per_s_fp32_fixup_4096_per_launch"], cc["ccs_per_s_fp32_fixup_2_streams_x_4096"])
+            best = max(cc["ccs_per_s_fp32_fixup_one_launch"], cc["ccs_per_s_fp32_fixup_2_streams"])
             ratios.update(cpu_ccs_per_s=cpu_ccs, ccs_per_s_vs_cpu=best / cpu_ccs)
         extras["vs_cpu_baseline"] = ratios
     line = dict(metric=METRIC, value=value, unit="rays/s", n_gpus=1, steps=args.steps, warmup=args.warmup,

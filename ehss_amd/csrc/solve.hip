@@ -497,7 +497,7 @@ namespace {
 // Workspace layout in scene->counters: [0..5] kernel counters, [6] max bounces,
 // [8..] final Partial.
 constexpr int kDefaultVariant = 4;
-constexpr int kMaxReplicates = 4096;
+constexpr int kMaxReplicates = 16384;
 constexpr int kCrossMinReps = 16;  // replicates per launch from which they are scheduled across lanes
 #ifndef EHSS_DRAIN_LDS_KB
 #define EHSS_DRAIN_LDS_KB 144

@@ -217,7 +217,7 @@ int ehss_solve(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, i
  * the replicates of ccs_cli.py:186-227): totals gets one entry per replicate,
  * histograms / ledgers (may be NULL) cap + 1 bins per replicate, replicate-major.
  * Each replicate's totals are bit-identical to ehss_solve with its scramble.
- * kernel_ms is the device time of the whole launch.  1 <= replicates <= 4096
+ * kernel_ms is the device time of the whole launch.  1 <= replicates <= 16384
  * and replicates x 2^power <= 2^30.  Per-ray records (EHSS_FLAG_KEEP_RAYS) are
  * those of the first replicate. */
 int ehss_solve_many(ehss_scene* scene, int replicates, const ehss_sobol* sobols, int power, int cap, int precision,
