@@ -105,6 +105,7 @@ _SIGS = {
     "ehss_scene_create": (_INT, [_I64, _P, _P, _INT, ct.POINTER(_P)]),
     "ehss_scene_create_grouped": (_INT, [_I64, _P, _P, _P, _INT, ct.POINTER(_P)]),
     "ehss_scene_destroy": (_INT, [_P]),
+    "ehss_scramble": (_INT, [ct.c_uint64, ct.POINTER(Sobol)]),
     "ehss_scene_rebuild": (_INT, [_P, ct.c_int64, _P, _P, _INT]),
     "ehss_scene_get_info": (_INT, [_P, ct.POINTER(SceneInfo)]),
     "ehss_scene_export_tree": (_INT, [_P] + [_P] * 7),

@@ -16,7 +16,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libehss_hip.so"
 
-SOURCES = ["capi.hip", "trace.hip", "build.hip", "scene.hip", "solve.hip", "batch.hip", "multi.hip", "pagrid.hip"]
+SOURCES = ["capi.hip", "trace.hip", "build.hip", "scene.hip", "solve.hip", "batch.hip", "multi.hip", "pagrid.hip", "scramble.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950",

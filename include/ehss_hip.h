@@ -192,6 +192,12 @@ typedef struct {
     double kernel_ms;      /* device time of the trace kernel(s) (HIP events) */
 } ehss_totals;
 
+/* Host-only: the scrambled net of scipy.stats.qmc.Sobol(4, scramble=True,
+ * seed) (_qmc.py:1808-1823; the ehss_state.py:44 call), built from NumPy's
+ * default_rng(seed) stream (SeedSequence + PCG64, Lemire bounded draws) as SciPy
+ * does.  Replaces the host scramble of the reference's source per replicate. */
+int ehss_scramble(uint64_t seed, ehss_sobol* out);
+
 /* Host-only: the wave-coherent ray map used by ehss_solve for 2^power rays
  * (cols[32]; no reference counterpart).  Thread t = 64 w + l traces the ray with
  * Gray-code index g = XOR of cols[k] over set bits k of (w | l << (power - 6)),
