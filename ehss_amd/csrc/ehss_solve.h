@@ -334,7 +334,7 @@ inline int coop_from_flags(unsigned flags) {
 // Drain shape from flags: lanes per ray of the first drain (bits 19..21,
 // log2 + 1) and the queries a ray gets there before it moves to the whole-wave
 // second drain (bits 22..25, q + 1; 0 queries = one drain).  Defaults (field
-// 0): from 2^18 rays, where the queue fills the chip, 4 lanes per ray for 2
+// 0): from 2^18 rays, where the queue fills the chip, 4 lanes per ray for 6
 // queries, then the second drain; below, one drain with a wave per ray.
 inline int group_from_flags(unsigned flags, int64_t rays) {
     const int f = (int)((flags >> EHSS_GROUP_SHIFT) & 0x7u);
@@ -353,9 +353,13 @@ inline int packets_from_flags(unsigned flags) {
     return f == 0 ? kDefaultPackets : f;
 }
 
+// Measured at 16 x 2^20 and 8192 x 2^11 rays of the 650-atom globule with 4
+// lanes per ray: 1 / 2 / 4 / 6 / 8 / 12 first-stage queries give 12.40 / 12.10 /
+// 11.89 / 11.83 / 12.02 / 11.91 ms per launch (and 11.43 ms at 6 for the small
+// nets, against 11.75 at 2).
 inline int stage1_queries(unsigned flags, int64_t rays) {
     const int f = (int)((flags >> EHSS_STAGE1_SHIFT) & 0xFu);
-    if (f == 0) return rays >= (1 << 18) ? 2 : 0;
+    if (f == 0) return rays >= (1 << 18) ? 6 : 0;
     return f - 1;
 }
 

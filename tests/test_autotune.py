@@ -10,7 +10,7 @@ def separable_cost(cfg):
     """Unique minimum at packets 4, group 4, stage1 2, coop 16, lds False, leaf 8."""
     c = {None: 0.3, 3: 0.3, 2: 0.6, 4: 0.1, 5: 0.2}[cfg["packets"]]
     c += {64: 3.0, 8: 2.0, 4: 1.0, 1: 4.0, None: 3.0}[cfg["group"]]
-    c += {None: 0.5, 0: 0.7, 1: 0.4, 2: 0.1, 4: 0.3}[cfg["stage1"]]
+    c += {None: 0.5, 0: 0.7, 1: 0.4, 2: 0.1, 4: 0.3, 6: 0.2, 12: 0.35}[cfg["stage1"]]
     c += {None: 0.2, 8: 0.2, 4: 0.3, 16: 0.05}[cfg["coop"]]
     c += 0.0 if not cfg["lds"] else 0.25
     c += {4: 0.5, 2: 0.9, 8: 0.2}[cfg["leaf_size"]]
