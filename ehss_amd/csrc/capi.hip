@@ -305,7 +305,14 @@ int ehss_device_info(char* name, size_t name_size, int* cu_count, int64_t* total
     EHSS_TRY(ensure_init());
     hipDeviceProp_t prop;
     EHSS_HIP(hipGetDeviceProperties(&prop, g_device));
-    if (name && name_size) snprintf(name, name_size, "%s (%s)", prop.name, prop.gcnArchName);
+    // The runtime may leave the marketing name empty; the architecture string
+    // and the CU count name the part then.
+    if (name && name_size) {
+        if (prop.name[0])
+            snprintf(name, name_size, "%s (%s)", prop.name, prop.gcnArchName);
+        else
+            snprintf(name, name_size, "%s, %d CUs", prop.gcnArchName, prop.multiProcessorCount);
+    }
     if (cu_count) *cu_count = prop.multiProcessorCount;
     if (total_mem) *total_mem = (int64_t)prop.totalGlobalMem;
     if (arch_major) *arch_major = prop.major;
