@@ -47,8 +47,9 @@ __device__ __forceinline__ bool grazing(const Query& q, double tol) {
     return (q.graze < INFINITY && q.graze <= q.best + tol) || q.second - q.best <= tol;
 }
 
+template <class C = uint64_t>
 __device__ __forceinline__ void nearest_plain(const SceneView& S, const V3& o, const V3& d, int32_t last, double tol,
-                                              Query& q, uint64_t& boxes, uint64_t& spheres) {
+                                              Query& q, C& boxes, C& spheres) {
     query_init(q);
     int32_t node = 0;
     const int32_t nn = S.n_nodes;
@@ -181,8 +182,9 @@ __device__ __forceinline__ bool needs_exact(const SceneView& S, const Ray32& r, 
     return !((double)tlo > best + tol);  // entry64 > best + tol: cannot win, tie or graze
 }
 
+template <class C = uint64_t>
 __device__ __forceinline__ void nearest_fast(const SceneView& S, const V3& o, const V3& d, int32_t last, double tol,
-                                             Query& q, uint64_t& boxes, uint64_t& spheres, uint64_t& retests,
+                                             Query& q, C& boxes, C& spheres, C& retests,
                                              PAState* pa = nullptr) {
     query_init(q);
     const Ray32 r = ray32(S, o, d);
@@ -277,10 +279,10 @@ __device__ __forceinline__ float4 load_sphere32_lds(const float4* base, int32_t 
     return make_float4(f[0], f[1], f[2], f[3]);
 }
 
-template <int kLds = 0>
+template <int kLds = 0, class C = uint64_t>
 __device__ __forceinline__ void nearest_packet(const SceneView& S, const V3& o, const V3& d, int32_t last,
-                                               double tol, bool live, Query& q, uint64_t& boxes,
-                                               uint64_t& spheres, uint64_t& retests, PAState* pa = nullptr) {
+                                               double tol, bool live, Query& q, C& boxes,
+                                               C& spheres, C& retests, PAState* pa = nullptr) {
     query_init(q);
     const Ray32 r = ray32(S, o, d);
     auto node_at = [&](int32_t i) -> Node32 {
@@ -355,9 +357,10 @@ __device__ __forceinline__ void group_merge_query(Query& q, int G) {
 }
 
 // Spheres of one leaf (slots first .. first + count - 1) against the lane's query.
+template <class C = uint64_t>
 __device__ __forceinline__ void scan_leaf(const SceneView& S, const Ray32& r, const V3& o, const V3& d,
                                           int32_t first, int32_t count, int32_t last, double tol, Query& q,
-                                          float& bestf, uint64_t& spheres, uint64_t& retests) {
+                                          float& bestf, C& spheres, C& retests) {
     for (int32_t k = first; k < first + count; ++k) {
         const int32_t wanted = S.ids[k];
         if (wanted == last) continue;
@@ -370,9 +373,10 @@ __device__ __forceinline__ void scan_leaf(const SceneView& S, const Ray32& r, co
     }
 }
 
+template <class C = uint64_t>
 __device__ __forceinline__ void nearest_group(const SceneView& S, const V3& o, const V3& d, int32_t last, double tol,
-                                              int G, int sub, Query& q, uint64_t& boxes, uint64_t& spheres,
-                                              uint64_t& retests) {
+                                              int G, int sub, Query& q, C& boxes, C& spheres,
+                                              C& retests) {
     query_init(q);
     const Ray32 r = ray32(S, o, d);
     float bestf = INFINITY;
@@ -414,16 +418,18 @@ __device__ __forceinline__ void nearest_group(const SceneView& S, const V3& o, c
 }
 
 // The whole wave on one ray.
+template <class C = uint64_t>
 __device__ __forceinline__ void nearest_coop(const SceneView& S, const V3& o, const V3& d, int32_t last, double tol,
-                                             Query& q, uint64_t& boxes, uint64_t& spheres, uint64_t& retests) {
+                                             Query& q, C& boxes, C& spheres, C& retests) {
     nearest_group(S, o, d, last, tol, kWave, threadIdx.x & (kWave - 1), q, boxes, spheres, retests);
 }
 
 // pa_spatial.py:114-130.  Counters follow the reference (stop at the first hit);
 // after a near-threshold first hit the scan continues, uncounted, looking for a
 // robust hit so that the grazing flag does not depend on traversal order.
+template <class C = uint64_t>
 __device__ __forceinline__ bool pa_plain(const SceneView& S, const V3& o, const V3& d, int32_t owner, double tol,
-                                         uint64_t& boxes, uint64_t& tests, bool& graze) {
+                                         C& boxes, C& tests, bool& graze) {
     int32_t node = 0;
     const int32_t stop = S.pa[0].end;
     bool hit = false, counting = true, critical_seen = false;
@@ -462,8 +468,9 @@ __device__ __forceinline__ bool pa_plain(const SceneView& S, const V3& o, const 
 // clear miss when disc < -m; robust hit when disc > m and b <= -m_p (then
 // -b + sqrt(disc) > 0); clear miss when b >= m_p and cc > m (origin outside,
 // centre behind); otherwise the FP64 test decides.  Margins include tol.
+template <class C = uint64_t>
 __device__ __forceinline__ bool pa_fast(const SceneView& S, const V3& o, const V3& d, int32_t owner, double tol,
-                                        uint64_t& boxes, uint64_t& tests, uint64_t& retests, bool& graze) {
+                                        C& boxes, C& tests, C& retests, bool& graze) {
     const Ray32 r = ray32(S, o, d);
     const float ftol = (float)tol * 1.01f;
     const float m_p = S.m_p + ftol;

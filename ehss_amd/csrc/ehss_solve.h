@@ -17,6 +17,8 @@
 // the LMS scramble is lower triangular), so a wave holds 64 rays from one small
 // cell of the 4-D net: nearby directions and impact points, coherent traversal.
 #pragma once
+#include <type_traits>
+
 #include "ehss_query.h"
 
 // Minimum waves per SIMD the packet kernels and the queue drains are compiled
@@ -36,7 +38,15 @@
 #ifndef EHSS_PLAN_DRAIN_WAVES
 #define EHSS_PLAN_DRAIN_WAVES 4
 #endif
-#define EHSS_SOLVE_KERNEL __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EHSS_SOLVE_WAVES)))
+// The plain FP64 kernel (variant 0) is compiled for 4 waves per SIMD: it runs
+// 17.0 ms per 2^20-ray solve there against 20.5 ms at 6 (its spills cost more
+// than the extra waves hide).
+#ifndef EHSS_PLAIN_WAVES
+#define EHSS_PLAIN_WAVES 4
+#endif
+#define EHSS_SOLVE_KERNEL                 \
+    __launch_bounds__(256) __attribute__( \
+        (amdgpu_waves_per_eu(kMode == 0 ? EHSS_PLAIN_WAVES : EHSS_SOLVE_WAVES)))
 #define EHSS_DRAIN_KERNEL __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EHSS_DRAIN_WAVES)))
 #define EHSS_PLAN_DRAIN_KERNEL __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EHSS_PLAN_DRAIN_WAVES)))
 
@@ -390,7 +400,7 @@ __device__ __forceinline__ void finish_coop(RayCtx& R, int32_t job, SceneOf scen
         const SceneView& S = scene_of(j);
         const V3 o{lane_value(R.o.x, src), lane_value(R.o.y, src), lane_value(R.o.z, src)};
         const V3 d{lane_value(R.d.x, src), lane_value(R.d.y, src), lane_value(R.d.z, src)};
-        uint64_t boxes = 0, spheres = 0, retests = 0;
+        uint32_t boxes = 0, spheres = 0, retests = 0;  // one query
         Query q;
         nearest_coop(S, o, d, lane_value(R.last, src), lane_value(R.tol, src), q, boxes, spheres, retests);
         account(j, boxes, spheres, retests);
@@ -417,7 +427,11 @@ template <int kMode, int kPA, int kLds = 0, class Sob>
 __device__ __forceinline__ void solve_ray_at(const SceneView& S, const SolveArgs& A, Sob sob, bool valid, uint32_t g,
                                              uint32_t t, uint64_t* ctr, RayState* queue, uint32_t* queue_count,
                                              int32_t job) {
-    uint64_t boxes = 0, spheres = 0, retests = 0, pboxes = 0, ptests = 0, pretests = 0;
+    // Work counts of this ray in this kernel.  Variant 4 runs a bounded number of
+    // queries here (the packet queries and the PA test), so 32 bits hold them and
+    // save registers; the full-path variants keep 64 bits.
+    using Cnt = std::conditional_t<kMode == 4, uint32_t, uint64_t>;
+    Cnt boxes = 0, spheres = 0, retests = 0, pboxes = 0, ptests = 0, pretests = 0;
     RayCtx R;
     R.i = t;
     R.o = V3{0., 0., 0.};
