@@ -4,6 +4,7 @@
 // nearest_fast:   the same query through an FP32 filter with an exact FP64 re-test of
 //                 every sphere the filter cannot exclude (identical result).
 // nearest_packet: nearest_fast with a wave-uniform node sequence and scalar loads.
+// nearest_packet_ordered: nearest_packet visiting children front to back.
 // pa_plain:       pa_spatial.py:114-130 over the PA tree (FP64, reference counters).
 // pa_fast:        any-hit through the FP32 filter over the static tree, FP64 re-test
 //                 of undecided spheres (identical hit flag).
@@ -106,13 +107,19 @@ __device__ __forceinline__ Ray32 ray32(const SceneView& S, const V3& o, const V3
 }
 
 // Inflated-box slab test: true unless the ray segment [0, bestf] provably misses.
-__device__ __forceinline__ bool box32(const Ray32& r, const Node32& n, float bestf) {
+// `near` receives the slab entry distance (used to choose a visiting order).
+__device__ __forceinline__ bool box32_near(const Ray32& r, const Node32& n, float bestf, float& near) {
     const float tx0 = __builtin_fmaf(n.lo[0], r.ix, -r.oix), tx1 = __builtin_fmaf(n.hi[0], r.ix, -r.oix);
     const float ty0 = __builtin_fmaf(n.lo[1], r.iy, -r.oiy), ty1 = __builtin_fmaf(n.hi[1], r.iy, -r.oiy);
     const float tz0 = __builtin_fmaf(n.lo[2], r.iz, -r.oiz), tz1 = __builtin_fmaf(n.hi[2], r.iz, -r.oiz);
-    const float near = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.f));
+    near = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.f));
     const float far = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
     return near <= far && near <= bestf;
+}
+
+__device__ __forceinline__ bool box32(const Ray32& r, const Node32& n, float bestf) {
+    float near;
+    return box32_near(r, n, bestf, near);
 }
 
 // FP32 upper bound of best + tol used for box pruning.
@@ -279,28 +286,39 @@ __device__ __forceinline__ float4 load_sphere32_lds(const float4* base, int32_t 
     return make_float4(f[0], f[1], f[2], f[3]);
 }
 
+// Wave-uniform reads of the scene for the packet queries: scalar loads, or LDS
+// broadcasts when the block has staged the scene (kLds).
+template <int kLds>
+struct PacketMem {
+    static __device__ __forceinline__ Node32 node(const SceneView& S, int32_t i) {
+        if constexpr (kLds) return load_node_lds(S.nodes32, i);
+        else return load_node(S.nodes32, i);
+    }
+    static __device__ __forceinline__ int32_t offset(const SceneView& S, int32_t g) {
+        if constexpr (kLds) return __builtin_amdgcn_readfirstlane(((LInt32)S.offsets)[g]);
+        else return ((CInt32)S.offsets)[g];
+    }
+    static __device__ __forceinline__ int32_t id(const SceneView& S, int32_t k) {
+        if constexpr (kLds) return __builtin_amdgcn_readfirstlane(((LInt32)S.ids)[k]);
+        else return ((CInt32)S.ids)[k];
+    }
+    static __device__ __forceinline__ float4 sphere(const SceneView& S, int32_t k) {
+        if constexpr (kLds) return load_sphere32_lds(S.sph32, k);
+        else return load_sphere32(S.sph32, k);
+    }
+};
+
 template <int kLds = 0, class C = uint64_t>
 __device__ __forceinline__ void nearest_packet(const SceneView& S, const V3& o, const V3& d, int32_t last,
                                                double tol, bool live, Query& q, C& boxes,
                                                C& spheres, C& retests, PAState* pa = nullptr) {
     query_init(q);
     const Ray32 r = ray32(S, o, d);
-    auto node_at = [&](int32_t i) -> Node32 {
-        if constexpr (kLds) return load_node_lds(S.nodes32, i);
-        else return load_node(S.nodes32, i);
-    };
-    auto offset_at = [&](int32_t g) -> int32_t {
-        if constexpr (kLds) return __builtin_amdgcn_readfirstlane(((LInt32)S.offsets)[g]);
-        else return ((CInt32)S.offsets)[g];
-    };
-    auto id_at = [&](int32_t k) -> int32_t {
-        if constexpr (kLds) return __builtin_amdgcn_readfirstlane(((LInt32)S.ids)[k]);
-        else return ((CInt32)S.ids)[k];
-    };
-    auto sphere_at = [&](int32_t k) -> float4 {
-        if constexpr (kLds) return load_sphere32_lds(S.sph32, k);
-        else return load_sphere32(S.sph32, k);
-    };
+    typedef PacketMem<kLds> M;
+    auto node_at = [&](int32_t i) { return M::node(S, i); };
+    auto offset_at = [&](int32_t g) { return M::offset(S, g); };
+    auto id_at = [&](int32_t k) { return M::id(S, k); };
+    auto sphere_at = [&](int32_t k) { return M::sphere(S, k); };
     float bestf = INFINITY;
     int32_t node = 0;
     const int32_t nn = S.n_nodes;
@@ -328,6 +346,107 @@ __device__ __forceinline__ void nearest_packet(const SceneView& S, const V3& o, 
             }
         }
         ++node;
+    }
+}
+
+// Packet traversal in front-to-back order: at an internal node both children's
+// boxes are tested; when lanes enter both, the packet descends first into the
+// child that the first lane entering both reaches first, and keeps the other on
+// a wave stack held across the lanes of one register (entry k in lane k).  The
+// group tree halves its range at every level, so at most 31 entries are ever
+// pushed.  A stacked node is re-tested against the lanes' bounds when it is
+// popped.  Reaching the near hits first shrinks every lane's bound sooner, so
+// fewer far subtrees are entered; every sphere that can win is still evaluated
+// in FP64 on every lane that needs it, so the answer is nearest_packet's.
+template <int kLds = 0, class C = uint64_t>
+__device__ __forceinline__ void nearest_packet_ordered(const SceneView& S, const V3& o, const V3& d, int32_t last,
+                                                       double tol, bool live, Query& q, C& boxes, C& spheres,
+                                                       C& retests, PAState* pa = nullptr) {
+    typedef PacketMem<kLds> M;
+    query_init(q);
+    if (__ballot(live) == 0) return;
+    const Ray32 r = ray32(S, o, d);
+    float bestf = INFINITY;
+    int32_t stack = 0;
+    int32_t sp = 0;
+    int32_t node = 0;
+    Node32 n = M::node(S, 0);
+    boxes += live ? 1 : 0;
+    bool hit = live && box32(r, n, bestf);
+    if (__ballot(hit) == 0) return;
+    for (;;) {
+        bool descend = false;
+        if (n.leaf >= 0) {
+            const int32_t g = n.leaf;
+            const int32_t k1 = M::offset(S, g + 1);
+            for (int32_t k = M::offset(S, g); k < k1; ++k) {
+                const int32_t wanted = M::id(S, k);
+                const float4 s = M::sphere(S, k);
+                if (!hit || wanted == last) continue;
+                ++spheres;
+                if (!needs_exact(S, r, s, q.best, tol, pa)) continue;
+                ++retests;
+                const int32_t before = q.atom;
+                eval64(o, d, load_sphere64(S.sphs, k), wanted, tol, q, pa);
+                if (q.atom != before) bestf = box_bound(S, q.best, tol);
+            }
+        } else {
+            const int32_t a = node + 1;
+            const Node32 na = M::node(S, a);
+            const int32_t b = na.end;
+            const Node32 nb = M::node(S, b);
+            boxes += live ? 2 : 0;
+            float neara, nearb;
+            const bool ha = hit && box32_near(r, na, bestf, neara);
+            const bool hb = hit && box32_near(r, nb, bestf, nearb);
+            const uint64_t ma = __ballot(ha), mb = __ballot(hb);
+            if (ma && mb) {
+                // Order by the first lane that enters both children; when no lane
+                // does, visit first the child more lanes enter.
+                const uint64_t both = ma & mb;
+                bool a_first;
+                if (both) {
+                    const int lb = __builtin_amdgcn_readfirstlane(__builtin_ctzll(both));
+                    const float ra = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(neara), lb));
+                    const float rb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nearb), lb));
+                    a_first = !(rb < ra);
+                } else {
+                    a_first = __popcll(ma) >= __popcll(mb);
+                }
+                const int32_t far_child = a_first ? b : a;
+                stack = (int32_t)(threadIdx.x & 63) == sp ? far_child : stack;
+                ++sp;
+                node = a_first ? a : b;
+                n = a_first ? na : nb;
+                hit = a_first ? ha : hb;
+                descend = true;
+            } else if (ma) {
+                node = a;
+                n = na;
+                hit = ha;
+                descend = true;
+            } else if (mb) {
+                node = b;
+                n = nb;
+                hit = hb;
+                descend = true;
+            }
+        }
+        if (descend) continue;
+        // Pop until a stacked node is entered by some lane.
+        bool found = false;
+        while (sp > 0) {
+            --sp;
+            node = __builtin_amdgcn_readlane(stack, sp);
+            n = M::node(S, node);
+            boxes += live ? 1 : 0;
+            hit = live && box32(r, n, bestf);
+            if (__ballot(hit)) {
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
     }
 }
 
