@@ -452,11 +452,12 @@ __device__ __forceinline__ void nearest_packet_ordered(const SceneView& S, const
 
 // Cooperative nearest query: one ray, a group of G lanes (G a power of two <= 64,
 // groups aligned within the wave).  The ray's state must be the same on every
-// lane of the group.  Lane `sub` of the group walks the subtrees rooted at
-// frontier entries sub, sub + G, ... with the nearest_fast rules and its own
-// running best (small scenes have every leaf in the frontier, so each lane scans
-// leaf boxes with four independent loads in flight); the group then merges its
-// per-lane queries.  Every sphere that can win (exact entry within best + tol) is
+// lane of the group.  Up to kBruteGroups groups, lane `sub` scans the leaf boxes
+// sub, sub + G, ... with four independent loads in flight; larger scenes have
+// lane `sub` walk the subtrees rooted at frontier entries sub, sub + G, ...
+// Either way each lane follows the nearest_fast rules with its own running
+// best, and the group then merges its per-lane queries.  Every sphere that can
+// win (exact entry within best + tol) is
 // evaluated in FP64 on some lane, and the winner is the lexicographic minimum of
 // (entry, atom id), so the answer equals nearest_fast's; `second` and `graze`
 // stay conservative.
@@ -492,6 +493,13 @@ __device__ __forceinline__ void scan_leaf(const SceneView& S, const Ray32& r, co
     }
 }
 
+// Scanning every leaf box beats walking the frontier subtrees (a chain of
+// dependent loads per lane) for every scene measured on MI355X, up to 8000 groups
+// (24000 atoms; 4 and 64 lanes per ray alike).
+#ifndef EHSS_BRUTE_GROUPS
+#define EHSS_BRUTE_GROUPS 16384
+#endif
+constexpr int32_t kBruteGroups = EHSS_BRUTE_GROUPS;
 template <class C = uint64_t>
 __device__ __forceinline__ void nearest_group(const SceneView& S, const V3& o, const V3& d, int32_t last, double tol,
                                               int G, int sub, Query& q, C& boxes, C& spheres,
@@ -500,7 +508,7 @@ __device__ __forceinline__ void nearest_group(const SceneView& S, const V3& o, c
     const Ray32 r = ray32(S, o, d);
     float bestf = INFINITY;
     const int32_t ng = S.n_groups;
-    if (ng <= kFrontier) {
+    if (ng <= kBruteGroups) {
         // Every leaf box, four independent loads in flight per lane.
         for (int32_t g0 = sub; g0 < ng; g0 += 4 * G) {
             Node32 leaf[4];

@@ -155,6 +155,28 @@ def test_plain_and_fp32_fixup_kernels_are_bit_identical(oracle, name):
         assert again["omega"] == b[0]["omega"]
 
 
+def test_large_scene_drain_walks_frontier_subtrees():
+    """Above 16384 groups the drain's cooperative queries walk the frontier
+    subtrees instead of scanning every leaf box; the answers still equal the
+    plain FP64 kernel's bit for bit."""
+    c, r = scenes.globule(20000, 3)
+    with GPUScene(c, r, leaf_size=1) as sc:
+        assert sc.info["groups"] > 16384
+        runs = {}
+        for key, kw in (("fp64", dict(precision="fp64")), ("default", {}), ("g4", dict(group=4, stage1=6)),
+                        ("coop", dict(group=64, stage1=0))):
+            say("large scene", key)
+            res = sc.solve(scramble(5), 14, 64, pa=True, keep_rays=True, histogram=True, **kw)
+            runs[key] = (res, sc.last_rays(1 << 14))
+        a = runs["fp64"]
+        phys = ~np.uint32(16)
+        for key, b in runs.items():
+            assert a[0]["omega"] == b[0]["omega"] and a[0]["pa"] == b[0]["pa"], key
+            np.testing.assert_array_equal(a[1]["contributions"], b[1]["contributions"], err_msg=key)
+            np.testing.assert_array_equal(a[1]["flags"] & phys, b[1]["flags"] & phys, err_msg=key)
+            np.testing.assert_array_equal(a[0]["histogram"], b[0]["histogram"], err_msg=key)
+
+
 def test_single_sphere_solve_is_pi_r2():
     c, r = np.zeros((1, 3)), np.array([3.1])
     with GPUScene(c, r) as sc:
