@@ -37,12 +37,17 @@ struct alignas(64) Node64 {
 };
 
 // FP32 filter node: outward-rounded box inflated by the scene's filter margin.
+// `leaf` is the leaf's group, or for an internal node -2 - (its right child), so
+// that a traversal can fetch both children at once (the left child is node + 1).
 struct alignas(32) Node32 {
     float lo[3];
     float hi[3];
     int32_t end;
     int32_t leaf;
 };
+
+// The encoding is its own inverse: right_child(right_child(r)) == r.
+__host__ __device__ constexpr int32_t right_child(int32_t internal_leaf) { return -2 - internal_leaf; }
 
 // PA any-hit tree node (pa_spatial.py:17-28): unpadded box, skip pointer,
 // first id slot and count (count 0 for internal nodes).

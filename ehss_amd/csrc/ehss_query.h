@@ -372,7 +372,7 @@ __device__ __forceinline__ void nearest_packet_ordered(const SceneView& S, const
     int32_t node = 0;
     Node32 n = M::node(S, 0);
     boxes += live ? 1 : 0;
-    bool hit = live && box32(r, n, bestf);
+    bool hit = box32(r, n, bestf) & live;
     if (__ballot(hit) == 0) return;
     for (;;) {
         bool descend = false;
@@ -391,14 +391,15 @@ __device__ __forceinline__ void nearest_packet_ordered(const SceneView& S, const
                 if (q.atom != before) bestf = box_bound(S, q.best, tol);
             }
         } else {
-            const int32_t a = node + 1;
+            const int32_t a = node + 1, b = right_child(n.leaf);
             const Node32 na = M::node(S, a);
-            const int32_t b = na.end;
             const Node32 nb = M::node(S, b);
             boxes += live ? 2 : 0;
+            // Both tests on every lane, then masked: no branch, so both
+            // children's records are fetched together.
             float neara, nearb;
-            const bool ha = hit && box32_near(r, na, bestf, neara);
-            const bool hb = hit && box32_near(r, nb, bestf, nearb);
+            const bool ha = box32_near(r, na, bestf, neara) & hit;
+            const bool hb = box32_near(r, nb, bestf, nearb) & hit;
             const uint64_t ma = __ballot(ha), mb = __ballot(hb);
             if (ma && mb) {
                 // Order by the first lane that enters both children; when no lane
@@ -440,7 +441,7 @@ __device__ __forceinline__ void nearest_packet_ordered(const SceneView& S, const
             node = __builtin_amdgcn_readlane(stack, sp);
             n = M::node(S, node);
             boxes += live ? 1 : 0;
-            hit = live && box32(r, n, bestf);
+            hit = box32(r, n, bestf) & live;
             if (__ballot(hit)) {
                 found = true;
                 break;

@@ -90,8 +90,9 @@ __global__ void __launch_bounds__(256) k_pa_query(SceneView S, PAIO io) {
     }
 }
 
-// FP32 filter arrays: boxes rounded outward and inflated by m_box; spheres in
-// slot order, recentred on the scene centre.
+// FP32 filter arrays: boxes rounded outward and inflated by m_box, internal
+// nodes linked to their right child; spheres in slot order, recentred on the
+// scene centre.
 __global__ void k_make_filter(SceneView S, Node32* nodes32, float4* sph32, double4* sphs, Node32* leaves32) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < S.n_nodes) {
@@ -104,7 +105,7 @@ __global__ void k_make_filter(SceneView S, Node32* nodes32, float4* sph32, doubl
             m.hi[k] = __double2float_ru(n.hi[k] - c[k]) + S.m_box;
         }
         m.end = n.end;
-        m.leaf = n.leaf;
+        m.leaf = n.leaf >= 0 ? n.leaf : right_child(S.nodes[i + 1].end);
         nodes32[i] = m;
         if (leaves32 && n.leaf >= 0) {
             m.end = S.offsets[n.leaf];
