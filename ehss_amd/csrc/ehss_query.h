@@ -185,7 +185,12 @@ __device__ __forceinline__ bool needs_exact(const SceneView& S, const Ray32& r, 
     const float mdisc = S.m_disc + 2.01f * s.w * ftol;
     if (disc < -mdisc) return false;  // disc64 < -2 r tol: clear miss
     if (pa && disc > mdisc && p <= -(S.m_p + ftol)) pa->decided = pa->hit = true;  // robust hit in both tests
-    const float tlo = (-p - S.m_p) - sqrtf(fmaxf(disc, 0.f) + S.m_disc);
+    // Upper bound of the square root from the hardware one (v_sqrt_f32, within
+    // 1 ulp): the factor 1 + 2^-21, less the product's own rounding, still
+    // exceeds 3 ulp (an ulp is at most 2^-23 of the value).  Two instructions
+    // instead of the correctly rounded sequence.
+    const float root = __builtin_amdgcn_sqrtf(fmaxf(disc, 0.f) + S.m_disc) * (1.0f + 0x1p-21f);
+    const float tlo = (-p - S.m_p) - root;
     return !((double)tlo > best + tol);  // entry64 > best + tol: cannot win, tie or graze
 }
 
