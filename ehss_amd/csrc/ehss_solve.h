@@ -25,17 +25,18 @@
 // for (the register budget follows: 512 / waves VGPRs per lane).  Both kernels
 // are latency-bound, so resident waves are what hides their memory round trips.
 // Measured on the 650-atom globule (2^20 rays) and on 18 globules of 200-5000
-// atoms (2^20 rays each): packet kernels at 6 waves (a few spilled registers)
-// beat 4; the single-scene drain, whose LDS-staged scene already limits it to
-// 3 blocks per CU, gains nothing from more; the multi-scene drain, which reads
-// the scenes from HBM, gains 5-8 % at 4.
+// atoms (2^20 rays each): packet kernels at 7 waves (72 VGPRs, 108 B spilled
+// per lane) run 9.68 ms per 16 x 2^20 rays against 9.97 at 6 and 9.79 at 8; the
+// single-scene drain, whose LDS-staged scene already limits it to 3 blocks per
+// CU, gains nothing from more; the multi-scene drain, which reads the scenes
+// from HBM, gains 5-8 % at 4.
 // Packet queries of the fast kernel visit children front to back (0: the
 // stackless preorder walk).
 #ifndef EHSS_ORDERED_PACKETS
 #define EHSS_ORDERED_PACKETS 1
 #endif
 #ifndef EHSS_SOLVE_WAVES
-#define EHSS_SOLVE_WAVES 6
+#define EHSS_SOLVE_WAVES 7
 #endif
 #ifndef EHSS_DRAIN_WAVES
 #define EHSS_DRAIN_WAVES 3
