@@ -105,7 +105,10 @@ __global__ void k_make_filter(SceneView S, Node32* nodes32, float4* sph32, doubl
             m.hi[k] = __double2float_ru(n.hi[k] - c[k]) + S.m_box;
         }
         m.end = n.end;
-        m.leaf = n.leaf >= 0 ? n.leaf : right_child(S.nodes[i + 1].end);
+        // A caller's tree (ehss_trace) may end in an internal node with no
+        // successor; it keeps -1 (only scene trees are walked by the packet
+        // queries that follow the link).
+        m.leaf = n.leaf >= 0 ? n.leaf : (i + 1 < S.n_nodes ? right_child(S.nodes[i + 1].end) : -1);
         nodes32[i] = m;
         if (leaves32 && n.leaf >= 0) {
             m.end = S.offsets[n.leaf];
