@@ -203,13 +203,29 @@ def batched18(molecules, scramble, replicate_seed, power=20, cap=64, launches=10
             s.close()
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+# Host facts of the CPU baseline (model, one-core rate), reported beside it.
+cpu_host = {}
+
+
 def cpu_baseline(centers, radii, sample_power=21):
     """Reference CPU kernel (src/native/ehss_cpu.cpp, built from the reference
     sources into oracle/_ref) on a bounded sample of the same workload, all
-    host cores; the project's C restatement of the whole path when the
-    reference build is absent."""
+    host cores, and on one core; the project's C restatement of the whole path
+    when the reference build is absent."""
     from oracle import oracle as O
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cpu_host.update(model=cpu_model(), cores_used=cores, host_cpus=os.cpu_count())
     ref = O.ref_lib()
     if ref is not None:
         src = O.external_source(centers, radii, sample_power, SEED)
@@ -221,6 +237,16 @@ def cpu_baseline(centers, radii, sample_power=21):
                     lib_handle=ref, threads=cores)
             times.append(time.perf_counter() - t0)
         t = float(np.median(times))
+        # The same kernel on one core, on a sixteenth of the sample.
+        one = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            O.trace(tree, centers, radii, src["origins"][: 1 << (sample_power - 4)],
+                    src["directions"][: 1 << (sample_power - 4)], cap=CAP, record=False, lib_handle=ref, threads=1)
+            one.append(time.perf_counter() - t0)
+        cpu_host.update(rays_per_s_one_core=(1 << (sample_power - 4)) / float(np.median(one)),
+                        one_core_sample=f"the first 2^{sample_power - 4} rays of the same sample, 1 OpenMP thread, "
+                                        f"median of 3")
         return dict(value=(1 << sample_power) / t, unit="rays/s", cores=cores, kind="reference",
                     sample=f"reference ehss_trace (src/native/ehss_cpu.cpp, g++ -O2 -fopenmp) on 2^{sample_power} "
                            f"rays of the same molecule, seed and cap; trace only (source, PA and readout "
@@ -457,7 +483,7 @@ def main():
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(centers, radii)
     if cpu is not None:
-        ratios = dict(rays_per_s_vs_cpu=value / cpu["value"])
+        ratios = dict(rays_per_s_vs_cpu=value / cpu["value"], cpu_host=dict(cpu_host))
         if "ccs_1pct" in extras:
             cc = extras["ccs_1pct"]
             cpu_ccs = cpu["value"] / cc["rays_per_ccs"]
