@@ -499,13 +499,6 @@ __device__ __forceinline__ void scan_leaf(const SceneView& S, const Ray32& r, co
     }
 }
 
-// Scanning every leaf box beats walking the frontier subtrees (a chain of
-// dependent loads per lane) for every scene measured on MI355X, up to 8000 groups
-// (24000 atoms; 4 and 64 lanes per ray alike).
-#ifndef EHSS_BRUTE_GROUPS
-#define EHSS_BRUTE_GROUPS 16384
-#endif
-constexpr int32_t kBruteGroups = EHSS_BRUTE_GROUPS;
 template <class C = uint64_t>
 __device__ __forceinline__ void nearest_group(const SceneView& S, const V3& o, const V3& d, int32_t last, double tol,
                                               int G, int sub, Query& q, C& boxes, C& spheres,
