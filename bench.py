@@ -41,7 +41,7 @@ METRIC = "EHSS traced rays/s + CCS/s at 1% RMSE on 1 MI355X; p50/p99 serve laten
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 POWER, CAP, SEED = 20, 64, 1
-STEPS_PER_LAUNCH = 16     # 2^20-ray steps per launch: 1.25 G rays/s vs 0.53 one per launch
+STEPS_PER_LAUNCH = 16     # 2^20-ray steps per launch: 1.85 G rays/s (2 in flight) vs 0.55 one per launch
 CCS_PER_LAUNCH = 8192     # 2,048-ray CCS estimates per launch (0.63 M CCS/s with two launches in flight)
 COPY_BYTES = 256 << 20    # calibration copy of the traffic probe (bytes read = bytes written)
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_solve_globule650.txt"

@@ -358,11 +358,12 @@ inline int group_from_flags(unsigned flags, int64_t rays) {
     return 1 << (f - 1);
 }
 
-// Packet queries of variant 4 (bits 26..29; 0 = default).  Measured at 2^20
-// rays per replicate, 16 replicates of the 650-atom globule: 1 query 0.70,
-// 2 1.23, 3 1.32, 4 1.31, 5 1.30 G rays/s; the 2000-atom ellipsoid and 18
-// globules of 200-5000 atoms gain up to 5 queries (0.76 / 0.89 / 0.90 / 0.93
-// G rays/s for 2 / 3 / 4 / 5 on the ellipsoid).
+// Packet queries of variant 4 (bits 26..29; 0 = default).  Measured with the
+// front-to-back walk at 2^20 rays per replicate, 16 replicates of the 650-atom
+// globule (6 first-stage drain queries): 2 / 3 / 4 / 5 queries take 9.96 /
+// 9.66 / 9.87 / 10.14 ms per launch; the 2000-atom ellipsoid (4 x 2^22 rays,
+// cap 128) keeps gaining up to 5 (15.39 / 12.65 / 11.58 / 11.39 ms), which the
+// autotuner picks up for such sets.
 constexpr int kDefaultPackets = 3;
 inline int packets_from_flags(unsigned flags) {
     const int f = (int)((flags >> EHSS_PACKETS_SHIFT) & 0xFu);
@@ -372,7 +373,8 @@ inline int packets_from_flags(unsigned flags) {
 // Measured at 16 x 2^20 and 8192 x 2^11 rays of the 650-atom globule with 4
 // lanes per ray: 1 / 2 / 4 / 6 / 8 / 12 first-stage queries give 12.40 / 12.10 /
 // 11.89 / 11.83 / 12.02 / 11.91 ms per launch (and 11.43 ms at 6 for the small
-// nets, against 11.75 at 2).
+// nets, against 11.75 at 2); with the front-to-back walk, 4 / 6 / 8 give 9.77 /
+// 9.66 / 9.68 ms.
 inline int stage1_queries(unsigned flags, int64_t rays) {
     const int f = (int)((flags >> EHSS_STAGE1_SHIFT) & 0xFu);
     if (f == 0) return rays >= (1 << 18) ? 6 : 0;
