@@ -42,9 +42,10 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 POWER, CAP, SEED = 20, 64, 1
 STEPS_PER_LAUNCH = 16     # 2^20-ray steps per launch: 1.85 G rays/s (2 in flight) vs 0.55 one per launch
-CCS_PER_LAUNCH = 8192     # 2,048-ray CCS estimates per launch (0.63 M CCS/s with two launches in flight)
+CCS_PER_LAUNCH = 8192     # 2,048-ray CCS estimates per launch (0.68 M CCS/s with two launches in flight)
 COPY_BYTES = 256 << 20    # calibration copy of the traffic probe (bytes read = bytes written)
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_solve_globule650.txt"
+VALU_FILE = ROOT / "profiles" / "valu_issue_solve_globule650.md"
 
 
 def say(*a):
@@ -65,6 +66,17 @@ def read_traffic():
         return None
     for line in TRAFFIC_FILE.read_text().splitlines():
         if line.startswith("bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def read_valu_issue():
+    """Fraction of the packet kernel's time spent issuing VALU instructions,
+    from the committed rocprofv3 SQ-counter summary, or None."""
+    if not VALU_FILE.exists():
+        return None
+    for line in VALU_FILE.read_text().splitlines():
+        if line.startswith("valu_issue_frac:"):
             return float(line.split(":", 1)[1])
     return None
 
@@ -383,6 +395,7 @@ def main():
                     per_ray=dict(box=plain["box_tests"] / n, sphere=plain["sphere_tests"] / n,
                                  reflections=refl / n, pa_box=plain["pa_box_tests"] / n,
                                  pa_sphere=plain["pa_sphere_tests"] / n))
+    roofline["valu_issue_frac_profiled"] = read_valu_issue()
     if roofline["traffic"]:
         roofline["traffic_gbs"] = round(roofline["traffic"] / avg_kernel_s / 1e9, 2)
         roofline["traffic_frac_of_hbm_peak"] = round(roofline["traffic_gbs"] / HBM_PEAK_GBS, 6)
