@@ -20,7 +20,10 @@ Each request is a structure (centres, radii).  The loop
 * keeps the scenes of finished requests idle (up to max_inflight scenes in all)
   and rebuilds an idle one in place for the next request (ehss_scene_rebuild:
   the device arrays only grow), so that a request costs no device allocation
-  or free -- a free synchronises the whole device;
+  or free -- a free synchronises the whole device.  The rebuilds run on worker
+  threads (the library call releases the interpreter lock and each scene has
+  its own stream), so the loop keeps launching and collecting meanwhile; a
+  request's first launch follows its rebuild;
 * records arrival, admission and completion times, from which p50/p99 latency
   and CCS/s follow.
 
@@ -33,6 +36,7 @@ from __future__ import annotations
 import collections
 import math
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -98,6 +102,8 @@ class _Live:
     res: Result
     scene: object
     footprint: int
+    rebuild: object = None  # pending in-place rebuild (a Future), if any
+    held: int = 0           # device bytes the reused scene held when idle
 
 
 def _rel_se(values) -> float:
@@ -114,13 +120,16 @@ class Server:
     asynchronous interface (solve_async, solve_done, solve_wait, close); when
     it also has rebuild(centers, radii) and device_bytes(), finished scenes are
     reused.  scramble(seed) and replicate_seed(seed, k) give the per-replicate
-    nets."""
+    nets.  rebuild_workers: threads that rebuild reused scenes (0: in the loop
+    itself)."""
 
     def __init__(self, budget_bytes: int, max_inflight: int = 16, power: int = 12, cap: int = 64,
                  target_rel: float = 0.01, min_reps: int = 4, max_reps: int = 64, precision: str = "fp32-fixup",
-                 pa: bool = True, scene_factory=None, scramble=None, replicate_seed=None, poll_s: float = 5e-5):
-        if budget_bytes <= 0 or max_inflight < 1 or min_reps < 2 or max_reps < min_reps:
+                 pa: bool = True, scene_factory=None, scramble=None, replicate_seed=None, poll_s: float = 5e-5,
+                 rebuild_workers: int = 4):
+        if budget_bytes <= 0 or max_inflight < 1 or min_reps < 2 or max_reps < min_reps or rebuild_workers < 0:
             raise ValueError("invalid server limits")
+        self.rebuild_workers = int(rebuild_workers)
         if scene_factory is None or scramble is None or replicate_seed is None:
             from .engine import GPUScene, replicate_seed as rs, scramble as sc
             scene_factory = scene_factory or GPUScene
@@ -176,22 +185,27 @@ class Server:
         idle: list = []  # (device bytes, scene) of finished requests, kept for reuse
         used = 0         # footprints of live requests plus the bytes of idle scenes
         t0 = clock()
+        pool = ThreadPoolExecutor(self.rebuild_workers) if self.rebuild_workers > 0 else None
 
         def take_scene(req, fp):
-            """A scene holding req: an idle one rebuilt in place, or a new one.
-            Returns (scene, bytes accounted for it)."""
+            """A scene holding req: an idle one rebuilt in place (on a worker
+            thread when there is a pool: the pending Future is returned), or a
+            new one.  Returns (scene, bytes accounted for it, Future or None,
+            bytes the scene held when idle)."""
             if idle:
                 held, scene = idle.pop()
+                if pool is not None:
+                    return scene, max(fp, held), pool.submit(scene.rebuild, req.centers, req.radii), held
                 try:
                     scene.rebuild(req.centers, req.radii)
                 except Exception:
                     idle.append((held, scene))
                     raise
                 self.scenes_reused += 1
-                return scene, max(fp, held)
+                return scene, max(fp, held), None, held
             scene = self.scene_factory(req.centers, req.radii)
             self.scenes_built += 1
-            return scene, fp
+            return scene, fp, None, 0
 
         def release_scene(scene):
             nonlocal used
@@ -203,74 +217,97 @@ class Server:
             else:
                 scene.close()
 
-        while pending or waiting or live:
-            now = clock() - t0
-            while pending and pending[0].arrival <= now:
-                waiting.append(pending.popleft())
-            # Admission: first come first served, by device-memory footprint.
-            while waiting and len(live) < self.max_inflight:
-                req = waiting[0]
-                fp = self.footprint(req)
-                if fp > self.budget:
+        try:
+            while pending or waiting or live:
+                now = clock() - t0
+                while pending and pending[0].arrival <= now:
+                    waiting.append(pending.popleft())
+                # Admission: first come first served, by device-memory footprint.
+                while waiting and len(live) < self.max_inflight:
+                    req = waiting[0]
+                    fp = self.footprint(req)
+                    if fp > self.budget:
+                        waiting.popleft()
+                        done.append(Result(req.rid, len(req.radii), arrival=req.arrival, finished=clock() - t0,
+                                           error=f"footprint {fp} B exceeds the device budget {self.budget} B"))
+                        continue
+                    # Memory held by idle scenes goes first when a request does not fit.
+                    while used + fp > self.budget and idle:
+                        held, scene = idle.pop(0)
+                        scene.close()
+                        used -= held
+                    if used + fp > self.budget:
+                        break
                     waiting.popleft()
-                    done.append(Result(req.rid, len(req.radii), arrival=req.arrival, finished=clock() - t0,
-                                       error=f"footprint {fp} B exceeds the device budget {self.budget} B"))
-                    continue
-                # Memory held by idle scenes goes first when a request does not fit.
-                while used + fp > self.budget and idle:
-                    held, scene = idle.pop(0)
-                    scene.close()
-                    used -= held
-                if used + fp > self.budget:
-                    break
-                waiting.popleft()
-                res = Result(req.rid, len(req.radii), arrival=req.arrival, admitted=clock() - t0)
-                if idle:
-                    used -= idle[-1][0]  # the scene about to be reused is accounted as live
-                    held_before = idle[-1][0]
-                else:
-                    held_before = 0
-                try:
-                    scene, acct = take_scene(req, fp)
-                except Exception as exc:  # invalid structure: report, keep serving
-                    used += held_before   # the idle scene stays idle
-                    res.error, res.finished = str(exc), clock() - t0
-                    done.append(res)
-                    continue
-                item = _Live(req, res, scene, acct)
-                used += acct
-                live[req.rid] = item
-                self._launch(item)
-            self.peak_bytes = max(self.peak_bytes, used)
-            self.peak_inflight = max(self.peak_inflight, len(live))
-            # Progress: collect every finished solve, relaunch or complete.
-            progressed = False
-            for rid in list(live):
-                item = live[rid]
-                if not item.scene.solve_done():
-                    continue
-                progressed = True
-                res = item.res
-                for out in item.scene.solve_wait():
-                    res.omegas.append(out["omega"])
-                    res.pa.append(out["pa"])
-                    res.unresolved += int(out["unresolved"])
-                    res.rays += int(out["rays"])
-                if self._converged(res):
-                    res.ccs = float(np.mean(res.omegas))
-                    res.rel_se = _rel_se(res.omegas)
-                    res.finished = clock() - t0
-                    used -= item.footprint
-                    del live[rid]
-                    release_scene(item.scene)
-                    done.append(res)
-                else:
-                    self._launch(item)
-            if not progressed:
-                if not live and not waiting and pending:
-                    sleep(max(0.0, min(pending[0].arrival - (clock() - t0), 0.01)))
-                else:
-                    sleep(self.poll_s)
+                    res = Result(req.rid, len(req.radii), arrival=req.arrival, admitted=clock() - t0)
+                    if idle:
+                        used -= idle[-1][0]  # the scene about to be reused is accounted as live
+                        held_before = idle[-1][0]
+                    else:
+                        held_before = 0
+                    try:
+                        scene, acct, rebuild, held = take_scene(req, fp)
+                    except Exception as exc:  # invalid structure: report, keep serving
+                        used += held_before   # the idle scene stays idle
+                        res.error, res.finished = str(exc), clock() - t0
+                        done.append(res)
+                        continue
+                    item = _Live(req, res, scene, acct, rebuild, held)
+                    used += acct
+                    live[req.rid] = item
+                    if rebuild is None:
+                        self._launch(item)
+                self.peak_bytes = max(self.peak_bytes, used)
+                self.peak_inflight = max(self.peak_inflight, len(live))
+                # Progress: collect every finished solve, relaunch or complete.
+                progressed = False
+                for rid in list(live):
+                    item = live[rid]
+                    if item.rebuild is not None:
+                        if not item.rebuild.done():
+                            continue
+                        progressed = True
+                        pending_rebuild, item.rebuild = item.rebuild, None
+                        try:
+                            pending_rebuild.result()
+                        except Exception as exc:  # invalid structure: report; the scene stays idle
+                            item.res.error, item.res.finished = str(exc), clock() - t0
+                            used -= item.footprint
+                            del live[rid]
+                            idle.append((item.held, item.scene))
+                            used += item.held
+                            done.append(item.res)
+                            continue
+                        self.scenes_reused += 1
+                        self._launch(item)
+                        continue
+                    if not item.scene.solve_done():
+                        continue
+                    progressed = True
+                    res = item.res
+                    for out in item.scene.solve_wait():
+                        res.omegas.append(out["omega"])
+                        res.pa.append(out["pa"])
+                        res.unresolved += int(out["unresolved"])
+                        res.rays += int(out["rays"])
+                    if self._converged(res):
+                        res.ccs = float(np.mean(res.omegas))
+                        res.rel_se = _rel_se(res.omegas)
+                        res.finished = clock() - t0
+                        used -= item.footprint
+                        del live[rid]
+                        release_scene(item.scene)
+                        done.append(res)
+                    else:
+                        self._launch(item)
+                if not progressed:
+                    if not live and not waiting and pending:
+                        sleep(max(0.0, min(pending[0].arrival - (clock() - t0), 0.01)))
+                    else:
+                        sleep(self.poll_s)
+        finally:
+            if pool is not None:
+                pool.shutdown(wait=True)
         for _, scene in idle:
             scene.close()
         return done

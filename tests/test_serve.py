@@ -197,3 +197,22 @@ def test_invalid_structure_with_an_idle_scene_keeps_serving():
     res = {r.rid: r for r in srv.run(good + [bad, after], clock=clock, sleep=clock.sleep)}
     assert res[77].error and "nonempty" in res[77].error
     assert res[78].error is None and FakeScene.live == 0
+
+
+def test_rebuilds_on_worker_threads_and_in_the_loop_give_the_same_results():
+    """rebuild_workers > 0 rebuilds reused scenes on threads, 0 in the loop; a
+    request's results do not depend on which."""
+    reqs = stream(count=40)
+    got = {}
+    for workers in (0, 3):
+        clock = Clock()
+        ReusableScene.rebuilds = 0
+        srv = serve.Server(1 << 40, scene_factory=lambda c, r: ReusableScene(clock, c, r), scramble=lambda s: s,
+                           replicate_seed=lambda s, k: (s + 104729 * k) % 2 ** 32, max_inflight=4, power=10,
+                           rebuild_workers=workers)
+        got[workers] = {r.rid: r for r in srv.run(reqs, clock=clock, sleep=clock.sleep)}
+        assert srv.scenes_reused > 0 and ReusableScene.rebuilds == srv.scenes_reused and FakeScene.live == 0
+    for rid, r in got[0].items():
+        assert r.error is None and got[3][rid].omegas == r.omegas and got[3][rid].ccs == r.ccs
+    with pytest.raises(ValueError):
+        serve.Server(1 << 30, rebuild_workers=-1, scene_factory=object, scramble=object, replicate_seed=object)
