@@ -126,7 +126,7 @@ class Server:
     def __init__(self, budget_bytes: int, max_inflight: int = 16, power: int = 12, cap: int = 64,
                  target_rel: float = 0.01, min_reps: int = 4, max_reps: int = 64, precision: str = "fp32-fixup",
                  pa: bool = True, scene_factory=None, scramble=None, replicate_seed=None, poll_s: float = 5e-5,
-                 rebuild_workers: int = 8):
+                 rebuild_workers: int = 12):
         if budget_bytes <= 0 or max_inflight < 1 or min_reps < 2 or max_reps < min_reps or rebuild_workers < 0:
             raise ValueError("invalid server limits")
         self.rebuild_workers = int(rebuild_workers)

@@ -216,3 +216,19 @@ def test_rebuilds_on_worker_threads_and_in_the_loop_give_the_same_results():
         assert r.error is None and got[3][rid].omegas == r.omegas and got[3][rid].ccs == r.ccs
     with pytest.raises(ValueError):
         serve.Server(1 << 30, rebuild_workers=-1, scene_factory=object, scramble=object, replicate_seed=object)
+
+
+def test_a_failed_launch_is_raised_not_reported_as_a_bad_structure():
+    class FailingScene(ReusableScene):
+        def solve_async(self, *a, **k):
+            raise RuntimeError("device failure")
+
+    before = FakeScene.live
+    for workers in (0, 2):
+        clock = Clock()
+        srv = serve.Server(1 << 40, scene_factory=lambda c, r: FailingScene(clock, c, r), scramble=lambda s: s,
+                           replicate_seed=lambda s, k: (s + 104729 * k) % 2 ** 32, max_inflight=2, power=10,
+                           rebuild_workers=workers)
+        with pytest.raises(RuntimeError, match="device failure"):
+            srv.run(stream(count=3, rate=10.0), clock=clock, sleep=clock.sleep)
+    FakeScene.live = before  # the aborted runs leave their scenes to the caller
