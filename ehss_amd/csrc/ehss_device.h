@@ -19,8 +19,7 @@ constexpr int kWave = 64;                         // wavefront width on gfx950
 constexpr int kFrontier = 8 * kWave;              // cooperative-query subtree roots, at most
 // Cooperative queries scan every leaf box up to kBruteGroups groups: that beats
 // walking the frontier subtrees (a chain of dependent loads per lane) for every
-// scene measured on MI355X, up to 8000 groups (24000 atoms).  Scenes at or below
-// it need no frontier.
+// scene measured on MI355X, up to 8000 groups (24000 atoms).
 constexpr int32_t kBruteGroups = 16384;
 
 // std::max / std::min semantics: the first argument wins unless strictly beaten.

@@ -46,11 +46,6 @@ namespace {
 // left child is node + 1, the right child starts where the left subtree ends)
 // while the list stays within kFrontier entries (eight per lane).
 int build_frontier(ehss_scene* sc, hipStream_t s) {
-    if (sc->ng <= kBruteGroups) {  // the leaf boxes are scanned instead
-        sc->view.frontier = nullptr;
-        sc->view.n_frontier = 0;
-        return EHSS_OK;
-    }
     std::vector<Node64> nodes((size_t)sc->nn);
     EHSS_HIP(hipMemcpyAsync(nodes.data(), sc->nodes.p, sizeof(Node64) * sc->nn, hipMemcpyDeviceToHost, s));
     EHSS_HIP(hipStreamSynchronize(s));
