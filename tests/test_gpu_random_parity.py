@@ -1,0 +1,62 @@
+"""Randomised parity sweep: the FP32-filter kernel (default schedule, and with
+one packet query or one lane per drained ray) against the plain FP64 kernel,
+bit for bit (Omega, PA, every ray's contribution and physical flags, the bounce
+histogram), on random molecules across scales, sizes and degenerate placements:
+clouds at scales 1e-3 to 1e3, tangent-sphere lattices, coincident centres and
+radii spread over four decades.  The FP32 filter's margins are relative to the
+scene scale, so every scale must give the plain kernel's bits."""
+import numpy as np
+import pytest
+
+from ehss_amd.engine import GPUScene, scramble
+from tests.gpu_helpers import say
+
+pytestmark = pytest.mark.gpu
+
+PHYS = ~np.uint32(16)  # every flag bit except the kernel-specific grazing diagnostic
+
+
+def molecule(kind, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "cloud":
+        n = int(rng.integers(1, 2500))
+        scale = (1e-3, 0.1, 1.0, 100.0, 1000.0)[seed % 5]
+        c = rng.normal(scale=scale * n ** (1 / 3), size=(n, 3))
+        r = rng.uniform(0.5, 2.0, size=n) * scale
+    elif kind == "lattice":  # touching spheres: exact tangencies between neighbours
+        k = int(rng.integers(2, 9))
+        g = np.arange(k, dtype=np.float64) * 2.0
+        c = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+        r = np.ones(len(c))
+    elif kind == "coincident":  # repeated centres with different radii
+        n = int(rng.integers(10, 400))
+        base = rng.normal(scale=3.0, size=(n, 3))
+        c = np.concatenate([base, base[: n // 2]])
+        r = np.concatenate([rng.uniform(1.0, 2.0, n), rng.uniform(0.5, 2.5, n // 2)])
+    else:  # "radii": four decades of radii
+        n = int(rng.integers(50, 1500))
+        c = rng.normal(scale=8.0, size=(n, 3))
+        r = 10.0 ** rng.uniform(-3, 1, size=n)
+    return c, r
+
+
+CASES = [(kind, seed) for kind in ("cloud", "lattice", "coincident", "radii") for seed in range(5)]
+
+
+@pytest.mark.parametrize("kind,seed", CASES)
+def test_fp32_filter_kernels_equal_the_plain_kernel(kind, seed):
+    c, r = molecule(kind, 1000 + seed)
+    say("random parity", kind, seed, len(r), "atoms")
+    power = 13
+    with GPUScene(c, r) as sc:
+        runs = {}
+        for key, kw in (("fp64", dict(precision="fp64")), ("default", {}), ("packets1", dict(packets=1)),
+                        ("lane", dict(group=1, stage1=0))):
+            res = sc.solve(scramble(seed + 17), power, 64, pa=True, keep_rays=True, histogram=True, **kw)
+            runs[key] = (res, sc.last_rays(1 << power))
+        a = runs["fp64"]
+        for key, b in runs.items():
+            assert a[0]["omega"] == b[0]["omega"] and a[0]["pa"] == b[0]["pa"], (kind, seed, key)
+            np.testing.assert_array_equal(a[1]["contributions"], b[1]["contributions"], err_msg=f"{kind}/{seed}/{key}")
+            np.testing.assert_array_equal(a[1]["flags"] & PHYS, b[1]["flags"] & PHYS, err_msg=f"{kind}/{seed}/{key}")
+            np.testing.assert_array_equal(a[0]["histogram"], b[0]["histogram"], err_msg=f"{kind}/{seed}/{key}")
