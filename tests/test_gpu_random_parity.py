@@ -1,5 +1,6 @@
-"""Randomised parity sweep: the FP32-filter kernel (default schedule, and with
-one packet query or one lane per drained ray) against the plain FP64 kernel,
+"""Randomised parity sweep: the FP32-filter kernel (default schedule, with one
+packet query, with one lane per drained ray, and with the packet queries reading
+an LDS-staged scene) against the plain FP64 kernel,
 bit for bit (Omega, PA, every ray's contribution and physical flags, the bounce
 histogram), on random molecules across scales, sizes and degenerate placements:
 clouds at scales 1e-3 to 1e3, tangent-sphere lattices, coincident centres and
@@ -51,7 +52,7 @@ def test_fp32_filter_kernels_equal_the_plain_kernel(kind, seed):
     with GPUScene(c, r) as sc:
         runs = {}
         for key, kw in (("fp64", dict(precision="fp64")), ("default", {}), ("packets1", dict(packets=1)),
-                        ("lane", dict(group=1, stage1=0))):
+                        ("lane", dict(group=1, stage1=0)), ("lds", dict(lds=True))):
             res = sc.solve(scramble(seed + 17), power, 64, pa=True, keep_rays=True, histogram=True, **kw)
             runs[key] = (res, sc.last_rays(1 << power))
         a = runs["fp64"]
