@@ -398,7 +398,10 @@ int ehss_plan_set_sobol(ehss_plan* plan, int job, const ehss_sobol* sobol) {
     return EHSS_OK;
 }
 
-int ehss_plan_launch(ehss_plan* plan) { return launch_plan(plan); }
+int ehss_plan_launch(ehss_plan* plan) {
+    EHSS_TRY(ensure_init());
+    return launch_plan(plan);
+}
 
 int ehss_plan_query(ehss_plan* plan, int* done) {
     EHSS_CHECK(plan && done, "null plan/done");

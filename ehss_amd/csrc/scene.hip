@@ -236,6 +236,7 @@ int ehss_scene_create_grouped(int64_t n, const double* centers, const double* ra
 }
 
 int ehss_scene_rebuild(ehss_scene* scene, int64_t n, const double* centers, const double* radii, int leaf_size) {
+    EHSS_TRY(ensure_init());  // the calling thread (a serving worker) may not have selected the device
     EHSS_CHECK(scene, "null scene");
     if (scene->pending) return set_error(EHSS_ERR_BUSY, "a solve is pending on this scene");
     std::vector<int32_t> cg;
@@ -315,6 +316,7 @@ int ehss_scene_export_pa_tree(const ehss_scene* scene, double* lo, double* hi, i
 int ehss_scene_trace(ehss_scene* scene, int64_t nr, int cap, int precision, int record, double* p, double* d,
                      const int64_t* lasts, int64_t* counts, uint8_t* escaped, uint8_t* unresolved,
                      int32_t* history, int64_t* counters) {
+    EHSS_TRY(ensure_init());
     EHSS_CHECK(scene, "null scene");
     EHSS_CHECK(nr == 0 || (p && d && lasts && counts && escaped && unresolved), "null ray buffer");
     for (int64_t r = 0; r < nr; ++r)
@@ -349,6 +351,7 @@ int ehss_scene_trace(ehss_scene* scene, int64_t nr, int cap, int precision, int 
 
 int ehss_scene_pa_query(ehss_scene* scene, int64_t nr, int precision, const double* origins,
                         const double* directions, const int64_t* owners, uint8_t* hits, int64_t* counters) {
+    EHSS_TRY(ensure_init());
     EHSS_CHECK(scene, "null scene");
     EHSS_CHECK(nr >= 0 && (nr == 0 || (origins && directions && hits)), "null ray buffer");
     EHSS_CHECK(precision == EHSS_PRECISION_FP64 || precision == EHSS_PRECISION_FP32_FIXUP, "unknown precision");

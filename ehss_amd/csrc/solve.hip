@@ -532,6 +532,7 @@ uint32_t slot_ray(const uint32_t cols[32], int power, uint32_t t) {
 // scene may be pending.
 int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, int cap, int precision,
                   unsigned flags) {
+    EHSS_TRY(ensure_init());  // the calling thread may not have selected the device
     EHSS_CHECK(sc && sobs, "null scene/sobol");
     if (sc->pending) return set_error(EHSS_ERR_BUSY, "a solve is already pending on this scene");
     EHSS_CHECK(reps >= 1 && reps <= kMaxReplicates, "replicate count %d out of range 1..%d", reps, kMaxReplicates);
@@ -895,6 +896,7 @@ int ehss_scene_last_rays(ehss_scene* scene, int64_t n, uint32_t* flags, double* 
 
 int ehss_source_rays(ehss_scene* scene, const ehss_sobol* sobol, int power, double* u, double* origins,
                      double* directions) {
+    EHSS_TRY(ensure_init());
     EHSS_CHECK(scene && sobol, "null scene/sobol");
     EHSS_CHECK(power >= 0 && power <= 26, "power %d out of range 0..26", power);
     hipStream_t s = scene->stream;
