@@ -123,14 +123,10 @@ _SIGS = {
     "ehss_ray_map": (_INT, [ct.POINTER(Sobol), _INT, _P]),
     "ehss_cross_map": (_INT, [_P, _INT, _INT, _P]),
     "ehss_source_rays": (_INT, [_P, ct.POINTER(Sobol), _INT, _P, _P, _P]),
+    "ehss_trig_compat_info": (_INT, [_P, _P, ct.c_size_t, _P, ct.c_size_t]),
+    "ehss_sobol_sincos": (_INT, [_I64, _P, _P, _P]),
     "ehss_pa_grid": (_INT, [_I64, _P, _P, _INT, _P, _INT, _P, _P]),
     "ehss_pa_grid_count": (_INT, [_I64, _P, _P, _P, ct.c_double, _INT, _P]),
-    "ehss_batch_create": (_INT, [_INT, _P, _P, _P, _P, _INT, ct.c_uint, _INT, ct.POINTER(_P)]),
-    "ehss_batch_set_sobol": (_INT, [_P, _INT, ct.POINTER(Sobol)]),
-    "ehss_batch_launch": (_INT, [_P]),
-    "ehss_batch_query": (_INT, [_P, ct.POINTER(_INT)]),
-    "ehss_batch_wait": (_INT, [_P, _P, _P, _P]),
-    "ehss_batch_destroy": (_INT, [_P]),
     "ehss_plan_create": (_INT, [_INT, _P, _P, _P, _P, _INT, ct.c_uint, ct.POINTER(_P)]),
     "ehss_plan_set_sobol": (_INT, [_P, _INT, ct.POINTER(Sobol)]),
     "ehss_plan_launch": (_INT, [_P]),
@@ -196,3 +192,21 @@ def device_info() -> dict:
          ct.byref(arch_minor))
     return dict(name=name.value.decode(), compute_units=cu.value, total_mem=mem.value,
                 arch=f"{arch_major.value}.{arch_minor.value}")
+
+
+def trig_compat_info() -> dict:
+    """The host-libm compatibility table of the source angles: entries listed
+    (-1: no table, cos/sin stay correctly rounded), its libm and file."""
+    n = ct.c_int64()
+    libm = ct.create_string_buffer(64)
+    path = ct.create_string_buffer(4096)
+    call("ehss_trig_compat_info", ct.byref(n), libm, len(libm), path, len(path))
+    return dict(entries=n.value, libm=libm.value.decode(), path=path.value.decode())
+
+
+def sobol_sincos(q: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Device sin and cos of the Sobol angles 2 pi q 2^-30 (q: 30-bit words)."""
+    q = np.ascontiguousarray(q, dtype=np.uint32)
+    s, c = np.empty(len(q)), np.empty(len(q))
+    call("ehss_sobol_sincos", len(q), ptr(q), ptr(s), ptr(c))
+    return s, c

@@ -1,5 +1,6 @@
 // capi.hip -- C-ABI entry points of libehss_hip.so: errors, device set-up, and
 // the reference's ehss_trace seam (src/native/ehss_cpu.cpp:42-46).
+#include <dlfcn.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -19,6 +20,80 @@ thread_local std::string g_error;
 std::mutex g_init_mutex;
 int g_device = -1;
 hipStream_t g_stream = nullptr;
+
+// Compatibility table (ehss_trig.h): one device allocation, kept until the
+// process ends.
+void* g_trig_mem = nullptr;
+ehss_trig::CompatTable g_trig{nullptr, nullptr};
+int64_t g_trig_entries = -1;
+std::string g_trig_libm, g_trig_path;
+
+struct TrigFileHeader {
+    char magic[8];
+    uint32_t key_bits, shift, buckets, entries, not_sure, libm_ulp2;
+    char libm[32];
+};
+static_assert(sizeof(TrigFileHeader) == 64, "trig table header is 64 bytes");
+
+std::string library_dir() {
+    Dl_info info;
+    if (dladdr((void*)&ensure_init, &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        const size_t cut = p.rfind('/');
+        if (cut != std::string::npos) return p.substr(0, cut);
+    }
+    return ".";
+}
+
+// Reads the table written by ehss_trig_compat_generate (trig_host.cpp) from
+// $EHSS_TRIG_COMPAT or trig_compat.bin beside this library.  A missing file
+// leaves the table off; a malformed one is an error.  Caller holds the init lock.
+int load_trig_compat() {
+    const char* env = getenv("EHSS_TRIG_COMPAT");
+    g_trig_path = env ? env : library_dir() + "/trig_compat.bin";
+    FILE* f = g_trig_path.empty() ? nullptr : fopen(g_trig_path.c_str(), "rb");
+    if (!f) return EHSS_OK;
+    TrigFileHeader h;
+    std::vector<uint32_t> off;
+    std::vector<uint16_t> ent;
+    bool ok = fread(&h, sizeof h, 1, f) == 1 && memcmp(h.magic, "EHSSTRG1", 8) == 0 && h.key_bits == 30 &&
+              h.shift == (uint32_t)ehss_trig::kCompatShift && h.buckets == ehss_trig::kCompatBuckets &&
+              h.libm_ulp2 == 0;
+    if (ok) {
+        off.resize((size_t)h.buckets + 1);
+        ent.resize(h.entries);
+        ok = fread(off.data(), sizeof(uint32_t), off.size(), f) == off.size() &&
+             (ent.empty() || fread(ent.data(), sizeof(uint16_t), ent.size(), f) == ent.size()) &&
+             fgetc(f) == EOF;
+    }
+    fclose(f);
+    for (size_t b = 0; ok && b < h.buckets; ++b) {
+        ok = off[b] <= off[b + 1] && off[b + 1] <= h.entries;
+        for (uint32_t k = off[b]; ok && k + 1 < off[b + 1]; ++k) ok = (ent[k] >> 4) < (ent[k + 1] >> 4);
+    }
+    ok = ok && off[0] == 0 && off[h.buckets] == h.entries;
+    if (!ok) return set_error(EHSS_ERR_INVALID, "malformed trig compatibility table %s", g_trig_path.c_str());
+    const size_t off_bytes = (sizeof(uint32_t) * off.size() + 255) & ~(size_t)255;
+    const size_t bytes = off_bytes + sizeof(uint16_t) * ent.size();
+    EHSS_HIP(hipMalloc(&g_trig_mem, bytes));
+    EHSS_HIP(hipMemcpy(g_trig_mem, off.data(), sizeof(uint32_t) * off.size(), hipMemcpyHostToDevice));
+    if (!ent.empty())
+        EHSS_HIP(hipMemcpy((char*)g_trig_mem + off_bytes, ent.data(), sizeof(uint16_t) * ent.size(),
+                           hipMemcpyHostToDevice));
+    g_trig.offsets = (const uint32_t*)g_trig_mem;
+    g_trig.entries = (const uint16_t*)((char*)g_trig_mem + off_bytes);
+    g_trig_entries = h.entries;
+    h.libm[sizeof h.libm - 1] = 0;
+    g_trig_libm = h.libm;
+    return EHSS_OK;
+}
+
+__global__ void k_sobol_sincos(int64_t n, const uint32_t* q, ehss_trig::CompatTable t, double* s, double* c) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = 6.283185307179586 * ((double)q[i] * 9.313225746154785e-10);
+    ehss_trig::sincos_sobol(x, q[i], t, &s[i], &c[i]);
+}
 }  // namespace
 
 int set_error(int code, const char* fmt, ...) {
@@ -75,11 +150,14 @@ int ensure_init() {
         return set_error(EHSS_ERR_HIP, "no HIP device available (%s)", hipGetErrorString(e));
     if (g_device < 0) g_device = 0;
     EHSS_HIP(hipSetDevice(g_device));
+    if (!g_trig_mem) EHSS_TRY(load_trig_compat());
     EHSS_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     return EHSS_OK;
 }
 
 hipStream_t lib_stream() { return g_stream; }
+
+ehss_trig::CompatTable trig_compat() { return g_trig; }
 
 namespace {
 
@@ -317,6 +395,34 @@ int ehss_device_info(char* name, size_t name_size, int* cu_count, int64_t* total
     if (total_mem) *total_mem = (int64_t)prop.totalGlobalMem;
     if (arch_major) *arch_major = prop.major;
     if (arch_minor) *arch_minor = prop.minor;
+    return EHSS_OK;
+}
+
+int ehss_trig_compat_info(int64_t* entries, char* libm, size_t libm_size, char* path, size_t path_size) {
+    EHSS_TRY(ensure_init());
+    if (entries) *entries = g_trig_entries;
+    if (libm && libm_size) snprintf(libm, libm_size, "%s", g_trig_libm.c_str());
+    if (path && path_size) snprintf(path, path_size, "%s", g_trig_path.c_str());
+    return EHSS_OK;
+}
+
+int ehss_sobol_sincos(int64_t n, const uint32_t* q, double* s, double* c) {
+    EHSS_TRY(ensure_init());
+    EHSS_CHECK(n >= 0 && q && s && c, "invalid ehss_sobol_sincos arguments");
+    for (int64_t i = 0; i < n; ++i) EHSS_CHECK(q[i] < (1u << 30), "q[%lld] is not a 30-bit Sobol word", (long long)i);
+    if (n == 0) return EHSS_OK;
+    hipStream_t st = lib_stream();
+    DevBuf dq, ds, dc;
+    EHSS_TRY(dq.alloc(sizeof(uint32_t) * (size_t)n));
+    EHSS_TRY(ds.alloc(sizeof(double) * (size_t)n));
+    EHSS_TRY(dc.alloc(sizeof(double) * (size_t)n));
+    EHSS_HIP(hipMemcpyAsync(dq.p, q, sizeof(uint32_t) * (size_t)n, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_sobol_sincos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, dq.as<uint32_t>(),
+                       g_trig, ds.as<double>(), dc.as<double>());
+    EHSS_HIP(hipGetLastError());
+    EHSS_HIP(hipMemcpyAsync(s, ds.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, st));
+    EHSS_HIP(hipMemcpyAsync(c, dc.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, st));
+    EHSS_HIP(hipStreamSynchronize(st));
     return EHSS_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "ehss_device.h"
+#include "ehss_trig.h"
 #include "../../include/ehss_hip.h"
 
 namespace ehss {
@@ -34,8 +35,11 @@ int hip_error(hipError_t e, const char* what, const char* file, int line);
         if (s_ != EHSS_OK) return s_; \
     } while (0)
 
-int ensure_init();           // selects the device and creates the library stream
+int ensure_init();           // selects the device, loads the trig table, creates the library stream
 hipStream_t lib_stream();
+// Host-libm compatibility table of the Sobol angles on the device (null
+// pointers when no table file was found: cos/sin then stay correctly rounded).
+ehss_trig::CompatTable trig_compat();
 
 // Owning device buffer.
 struct DevBuf {

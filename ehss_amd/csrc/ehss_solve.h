@@ -1,11 +1,12 @@
 // ehss_solve.h -- the per-ray body of the fused solve, shared by the single-scene
-// kernel (solve.hip) and the multi-job persistent kernel (batch.hip).
+// kernels (solve.hip) and the multi-structure plan kernels (multi.hip).
 //
 // One replicate of solve_frame (ccs_cli.py:196-219) without leaving the GPU:
 //   1. u = scrambled Sobol point (SciPy qmc.Sobol, LMS + shift, 30 bits, Gray-code
 //      order: q_i = shift ^ XOR_{bit j of gray(i)} sv[j]), exact k * 2^-30;
 //   2. external_source (ehss_state.py:42-52, inside_out_pa.py:45-58) in FP64 with
-//      NumPy's operation order (only cos/sin come from the device library);
+//      NumPy's operation order; cos/sin are correctly rounded (ehss_trig.h) and
+//      corrected to the host libm's bits by its compatibility table;
 //   3. optional any-hit PA (pa_spatial.py:114-142);
 //   4. the reflect-until-escape trace (ehss_cpu.cpp:54-79);
 //   5. a per-ray record (bounces, flags, w (1 - u.v)) in ray-index order, reduced
@@ -20,6 +21,7 @@
 #include <type_traits>
 
 #include "ehss_query.h"
+#include "ehss_trig.h"
 
 // Minimum waves per SIMD the packet kernels and the queue drains are compiled
 // for (the register budget follows: 512 / waves VGPRs per lane).  Both kernels
@@ -72,6 +74,7 @@ struct SolveArgs {
     long long* times;              // optional per-ray completion time stamps (device wall clock)
     int32_t packets;               // variant 4: packet queries before queueing (1..15)
     int32_t pad_;
+    ehss_trig::CompatTable trig;   // host-libm compatibility table of the Sobol angles (may be null)
 };
 
 // Words per replicate in the solve workspace: [0..5] kernel counters, [6] max
@@ -194,15 +197,17 @@ __device__ __forceinline__ void sobol_point_gray(Sob sob, uint32_t g, double u[4
 }
 
 // ehss_state.py:45-49 with unit_sphere / tangent_frames (inside_out_pa.py:45-58),
-// in the order NumPy evaluates the expressions.
-__device__ __forceinline__ void source_ray(const double u[4], double cx, double cy, double cz, double R, V3& o,
-                                           V3& d) {
+// in the order NumPy evaluates the expressions.  The angles are 2pi u for the
+// Sobol coordinates u = q 2^-30; their cos/sin are the host libm's bits
+// (ehss_trig::sincos_sobol), so the rays equal NumPy's bit for bit.
+__device__ __forceinline__ void source_ray(const double u[4], double cx, double cy, double cz, double R,
+                                           ehss_trig::CompatTable trig, V3& o, V3& d) {
     const double two_pi = 6.283185307179586;  // 2 * np.pi
     const double z = 1 - 2 * u[0];
     const double phi = two_pi * u[1];
     const double rho = sqrt(smax(0., 1 - z * z));
     double s, c;
-    sincos(phi, &s, &c);
+    ehss_trig::sincos_sobol(phi, (uint32_t)(u[1] * 1073741824.0), trig, &s, &c);
     d.x = rho * c;
     d.y = rho * s;
     d.z = z;
@@ -221,7 +226,7 @@ __device__ __forceinline__ void source_ray(const double u[4], double cx, double 
     const double r = R * sqrt(u[2]);
     const double ph = two_pi * u[3];
     double sp, cp;
-    sincos(ph, &sp, &cp);
+    ehss_trig::sincos_sobol(ph, (uint32_t)(u[3] * 1073741824.0), trig, &sp, &cp);
     o.x = (cx - R * d.x) + r * (cp * t0 + sp * b0);
     o.y = (cy - R * d.y) + r * (cp * t1 + sp * b1);
     o.z = (cz - R * d.z) + r * (cp * t2 + sp * b2);
@@ -340,7 +345,7 @@ __device__ __forceinline__ int32_t ray_load(const RayState* q, const SceneView& 
     return st.job;
 }
 
-// Tail threshold from ehss_solve / ehss_batch_create flags (bits 12..18).
+// Tail threshold from ehss_solve / ehss_plan_create flags (bits 12..18).
 constexpr int kDefaultCoop = 8;
 inline int coop_from_flags(unsigned flags) {
     const int f = (int)((flags >> EHSS_COOP_SHIFT) & 0x7Fu);
@@ -447,7 +452,7 @@ __device__ __forceinline__ void solve_ray_at(const SceneView& S, const SolveArgs
     if (valid) {
         double u[4];
         sobol_point_gray(sob, g, u);
-        source_ray(u, A.cx, A.cy, A.cz, A.radius, R.o, R.d);
+        source_ray(u, A.cx, A.cy, A.cz, A.radius, A.trig, R.o, R.d);
     }
     R.inc = R.d;
     R.flags = 0;

@@ -270,6 +270,7 @@ int create_plan(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols,
         J.args.coop = p->coop;
         J.args.group = p->group;
         J.args.packets = packets_from_flags(flags);
+        J.args.trig = trig_compat();
         J.args.times = nullptr;
         J.sob = sobols[j];
         make_ray_map(J.sob, powers[j], J.map.col);

@@ -189,15 +189,15 @@ __global__ void __launch_bounds__(256) k_finalize(const Partial* partials, int64
     }
 }
 
-__global__ void k_source_rays(int64_t n, double cx, double cy, double cz, double R, ehss_sobol sob, double* u_out,
-                              double* o_out, double* d_out) {
+__global__ void k_source_rays(int64_t n, double cx, double cy, double cz, double R, ehss_trig::CompatTable trig,
+                              ehss_sobol sob, double* u_out, double* o_out, double* d_out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double u[4];
     const uint32_t ii = (uint32_t)i;
     sobol_point_gray(sob, ii ^ (ii >> 1), u);
     V3 o, d;
-    source_ray(u, cx, cy, cz, R, o, d);
+    source_ray(u, cx, cy, cz, R, trig, o, d);
     if (u_out)
         for (int k = 0; k < 4; ++k) u_out[4 * i + k] = u[k];
     if (o_out) {
@@ -610,6 +610,7 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     A.group = group_from_flags(flags, total);
     A.packets = packets_from_flags(flags);
     A.pad_ = 0;
+    A.trig = trig_compat();
     A.times = nullptr;
     if (flags & EHSS_FLAG_TIMESTAMPS) {
         EHSS_TRY(sc->ray_times.reserve(sizeof(long long) * total));
@@ -906,7 +907,8 @@ int ehss_source_rays(ehss_scene* scene, const ehss_sobol* sobol, int power, doub
     if (origins) EHSS_TRY(bo.alloc(sizeof(double) * 3 * n));
     if (directions) EHSS_TRY(bd.alloc(sizeof(double) * 3 * n));
     hipLaunchKernelGGL(k_source_rays, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, scene->center[0],
-                       scene->center[1], scene->center[2], scene->radius, *sobol, bu.as<double>(), bo.as<double>(),
+                       scene->center[1], scene->center[2], scene->radius, trig_compat(), *sobol, bu.as<double>(),
+                       bo.as<double>(),
                        bd.as<double>());
     EHSS_HIP(hipGetLastError());
     if (u) EHSS_HIP(hipMemcpyAsync(u, bu.p, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, s));

@@ -516,16 +516,3 @@ class GPUPlan(_MultiSolve):
         flags = (L.FLAG_PA if pa else 0) | (L.FLAG_GRAPH if graph else 0) | (int(variant) << L.VARIANT_SHIFT)
         flags |= coop_flags(coop, group, stage1, packets) | (L.FLAG_LDS_PACKETS if lds else 0)
         self._create(scenes, sobols, powers, caps, flags, precision)
-
-
-class GPUBatch(_MultiSolve):
-    """The persistent-kernel form of GPUPlan (ehss_batch_*); the library
-    currently refuses to create it (see ehss_batch_create)."""
-
-    _prefix = "batch"
-
-    def __init__(self, scenes, sobols, powers, caps, precision="fp32-fixup", pa: bool = True, graph: bool = False,
-                 variant: int = 0, coop: int | None = None, group: int | None = None, blocks_per_cu: int = 0):
-        flags = (L.FLAG_PA if pa else 0) | (L.FLAG_GRAPH if graph else 0) | (int(variant) << L.VARIANT_SHIFT)
-        flags |= coop_flags(coop, group)
-        self._create(scenes, sobols, powers, caps, flags, precision, int(blocks_per_cu))

@@ -19,7 +19,7 @@
  *   ehss_solve               one replicate of solve_frame (ccs_cli.py:196-219):
  *                            external_source (ehss_state.py:42-52) -> index.query ->
  *                            engine.trace -> EHSSResult readout (ehss_state.py:84-103).
- *   ehss_batch_*, ehss_plan_*  no single reference counterpart: the per-frame
+ *   ehss_plan_*              no single reference counterpart: the per-frame
  *                            solve_frame loop (ccs_cli.py:305-307) over many
  *                            structures as one launch.
  */
@@ -50,7 +50,7 @@ extern "C" {
 /* ehss_solve flags. */
 #define EHSS_FLAG_PA 1u        /* also run the BVH any-hit PA on the same rays */
 #define EHSS_FLAG_KEEP_RAYS 2u /* keep per-ray records for ehss_scene_last_rays */
-#define EHSS_FLAG_GRAPH 4u     /* batch: capture the launch sequence in a hipGraph and replay it */
+#define EHSS_FLAG_GRAPH 4u     /* plan: capture the launch sequence in a hipGraph and replay it */
 #define EHSS_FLAG_TIMESTAMPS 8u /* record each ray's completion time (ehss_scene_last_ray_times) */
 #define EHSS_FLAG_LDS_PACKETS 16u /* packet kernels stage the scene in LDS first (measured: not faster than
                                      scalar loads through the constant cache; kept for measurement) */
@@ -91,6 +91,21 @@ int ehss_device_info(char* name, size_t name_size, int* cu_count, int64_t* total
 /* Device FP64 arithmetic check (no reference counterpart): quotient[i] = a[i] / b[i]
  * and root[i] = sqrt(a[i]) computed by a kernel, for comparison with host IEEE. */
 int ehss_selftest_fp64(int64_t n, const double* a, const double* b, double* quotient, double* root);
+
+/* Source-angle cos/sin (no reference counterpart).  The incidence source takes
+ * cos and sin of the Sobol angles x = fl(2pi * q 2^-30), q a 30-bit Sobol word
+ * (ehss_state.py:48, inside_out_pa.py:47), which the reference evaluates with
+ * NumPy, i.e. the host libm.  The device computes them correctly rounded and
+ * applies a compatibility table (trig_compat.bin beside the library, or the file
+ * named by $EHSS_TRIG_COMPAT) listing every q where the host libm rounds the
+ * other way; it is read at initialisation.
+ * ehss_trig_compat_info: *entries = angles listed, or -1 when no table was found
+ * (cos/sin then stay correctly rounded); libm: the libm the table was generated
+ * from; path: the file looked for.
+ * ehss_sobol_sincos: s[i], c[i] = sin, cos of the angle of word q[i] < 2^30, as
+ * the device source computes them. */
+int ehss_trig_compat_info(int64_t* entries, char* libm, size_t libm_size, char* path, size_t path_size);
+int ehss_sobol_sincos(int64_t n, const uint32_t* q, double* s, double* c);
 
 /* ---------------------------------------------------------------------------
  * Reference seam.  Exact signature and array contract of ehss_trace at
@@ -278,28 +293,6 @@ int ehss_pa_grid(int64_t n_atoms, const double* centered, const double* radii, i
  * with lower corner lower[2] and pixel size spacing. */
 int ehss_pa_grid_count(int64_t n_atoms, const double* xy, const double* radii, const double* lower, double spacing,
                        int grid, int64_t* count);
-
-/* ---------------------------------------------------------------------------
- * Batched solves: many (scene, seed, 2^power rays, cap) jobs in one persistent
- * launch with a ray-packet work queue; each job is reduced in its own fixed
- * order, so its totals equal those of ehss_solve on the same job.  Scenes must
- * outlive the batch.  Launch is asynchronous; query/wait collect results.
- * ------------------------------------------------------------------------- */
-typedef struct ehss_batch ehss_batch;
-
-/* blocks_per_cu: persistent grid size (256-thread blocks per CU, 0 = default). */
-int ehss_batch_create(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols, const int32_t* powers,
-                      const int32_t* caps, int precision, unsigned flags, int blocks_per_cu, ehss_batch** out);
-/* Replaces the scramble of one job (used by the next launch, graph or not). */
-int ehss_batch_set_sobol(ehss_batch* batch, int job, const ehss_sobol* sobol);
-int ehss_batch_launch(ehss_batch* batch);
-/* *done = 1 once the last launch has finished (never blocks). */
-int ehss_batch_query(ehss_batch* batch, int* done);
-/* Blocks until done; totals: n_jobs entries; histograms / ledgers (may be NULL):
- * concatenated (cap_j + 1)-bin arrays in job order.  kernel_ms is the device
- * time of the whole batch. */
-int ehss_batch_wait(ehss_batch* batch, ehss_totals* totals, int64_t* histograms, double* ledgers);
-int ehss_batch_destroy(ehss_batch* batch);
 
 /* ---------------------------------------------------------------------------
  * Prepared multi-structure plans: many (scene, scramble, 2^power rays, cap)

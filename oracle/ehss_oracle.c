@@ -20,7 +20,9 @@
  *   oracle_groups       == PA-tree leaf order from oracle_pa_build (two builders).
  *   oracle_static_tree  closed-form topology algebra, box nesting/containment.
  *   oracle_pa_query     == brute-force any-hit over every sphere.
- *   oracle_solve_cpu    CCS within 1e-12 of the NumPy-path oracle on the same u.
+ *   oracle_solve_cpu    == the NumPy-path oracle on the same u at C1 2^16 and C2 2^20
+ *                          rays: omega within 1e-12, hit and unresolved counts and
+ *                          counters exact (tests/test_oracle_source.py).
  */
 #include <math.h>
 #include <stdint.h>
@@ -528,13 +530,19 @@ static void sobol_point(const uint32_t* sv /*4x30*/, const uint32_t* shift, uint
     }
 }
 
+/* NumPy evaluates cos and sin separately.  Called directly, the compiler merges
+ * the pair into one sincos(), whose last bits differ from the separate calls on
+ * rare angles; calling through volatile pointers keeps them separate. */
+static double (*volatile libm_cos)(double) = cos;
+static double (*volatile libm_sin)(double) = sin;
+
 /* ehss_state.py:42-52 with inside_out_pa.py:45-58, per ray, in NumPy's operation order. */
 static void source_ray(const double* u, const double* center, double radius, double* o, double* d) {
     const double two_pi = 2 * M_PI;
     double z = 1 - 2 * u[0];
     double phi = two_pi * u[1];
     double rho = sqrt(smax(0., 1 - z * z));
-    d[0] = rho * cos(phi); d[1] = rho * sin(phi); d[2] = z;
+    d[0] = rho * libm_cos(phi); d[1] = rho * libm_sin(phi); d[2] = z;
     double ax[3] = {0., 0., 1.};
     if (fabs(d[2]) > .9) { ax[0] = 1.; ax[1] = 0.; ax[2] = 0.; }
     double t[3] = {ax[1] * d[2] - ax[2] * d[1], ax[2] * d[0] - ax[0] * d[2], ax[0] * d[1] - ax[1] * d[0]};
@@ -543,7 +551,7 @@ static void source_ray(const double* u, const double* center, double radius, dou
     double b[3] = {d[1] * t[2] - d[2] * t[1], d[2] * t[0] - d[0] * t[2], d[0] * t[1] - d[1] * t[0]};
     double r = radius * sqrt(u[2]);
     double ph = two_pi * u[3];
-    double cp = cos(ph), sp = sin(ph);
+    double cp = libm_cos(ph), sp = libm_sin(ph);
     for (int k = 0; k < 3; ++k) o[k] = (center[k] - radius * d[k]) + r * (cp * t[k] + sp * b[k]);
 }
 

@@ -11,8 +11,10 @@ reference so that, on one machine, their bits agree with the reference's.
 
 Pinning summary (see tests/test_oracle_*.py):
   sobol_scramble / sobol_points   == SciPy 1.15.3 qmc.Sobol golden vectors (bitwise).
-  external_source                 closed forms (unit norms, frame orthogonality, disk
-                                  radius, sum of weights) and the single-sphere pi r^2.
+  external_source                 per-ray closed forms that fix the u -> ray assignment
+                                  (d_z = 1 - 2 u0, azimuths 2 pi u1 and 2 pi u3, impact
+                                  radius R sqrt(u2)), planted swaps failing them, and
+                                  the single-sphere pi r^2 (tests/test_oracle_source.py).
   readout                         math.fsum of the same contributions; clip bounds.
   trace / static_tree / pa_*      through ehss_oracle.c (see its header).
 """
