@@ -88,6 +88,15 @@ int load_trig_compat() {
     return EHSS_OK;
 }
 
+// Work memory of the allocation-free solve (ehss_solve_noalloc): a static device
+// array, one copy per device, held until the process ends.  Host code never
+// passes `scratch` itself to a kernel (its host-side name is not a device
+// address): ensure_init looks up each device's address once with
+// hipGetSymbolAddress and noalloc_scratch() hands that out.
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+constexpr int kMaxDevices = 64;
+void* g_scratch[kMaxDevices] = {};
+
 __global__ void k_sobol_sincos(int64_t n, const uint32_t* q, ehss_trig::CompatTable t, double* s, double* c) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -151,6 +160,12 @@ int ensure_init() {
     if (g_device < 0) g_device = 0;
     EHSS_HIP(hipSetDevice(g_device));
     if (!g_trig_mem) EHSS_TRY(load_trig_compat());
+    EHSS_CHECK(g_device < kMaxDevices, "device %d out of range", g_device);
+    if (!g_scratch[g_device]) {
+        void* p = nullptr;
+        EHSS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[g_device] = p;
+    }
     EHSS_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     return EHSS_OK;
 }
@@ -158,6 +173,12 @@ int ensure_init() {
 hipStream_t lib_stream() { return g_stream; }
 
 ehss_trig::CompatTable trig_compat() { return g_trig; }
+
+void* noalloc_scratch() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    return g_scratch[dev];
+}
 
 namespace {
 

@@ -40,6 +40,10 @@ hipStream_t lib_stream();
 // Host-libm compatibility table of the Sobol angles on the device (null
 // pointers when no table file was found: cos/sin then stay correctly rounded).
 ehss_trig::CompatTable trig_compat();
+// Size of the allocation-free solve's static work array (ehss_solve_noalloc),
+// and its device address on the current device (null before ensure_init).
+constexpr size_t kNoallocScratchBytes = (size_t)EHSS_NOALLOC_SCRATCH_MB << 20;
+void* noalloc_scratch();
 
 // Owning device buffer.
 struct DevBuf {

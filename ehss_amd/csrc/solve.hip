@@ -160,6 +160,51 @@ __global__ void EHSS_DRAIN_KERNEL k_drain(SceneView S, SolveArgs A, int32_t repl
 // LDS bytes of a scene staged by the kLds kernels.
 size_t drain_lds_bytes(const SceneView& v) { return staged_scene_bytes(v); }
 
+// Allocation-free solve: the scramble and ray map arrive as kernel arguments
+// (captured by value under graph capture) and are written to the work array.
+__global__ void k_put_tables(ehss_sobol sob, RayMap map, ehss_sobol* d_sob, RayMap* d_map) {
+    const int i = threadIdx.x;
+    if (i < 4 * 32) d_sob->sv[i / 32][i % 32] = sob.sv[i / 32][i % 32];
+    if (i < 4) d_sob->shift[i] = sob.shift[i];
+    if (i < 32) d_map->col[i] = map.col[i];
+}
+
+// Allocation-free solve: totals, histogram and ledger from the device counters,
+// as solve_collect forms them on the host.
+__global__ void k_write_totals(const unsigned long long* c, const unsigned long long* hist,
+                               const unsigned long long* ledger, int64_t bins, int64_t n, double w, int pa,
+                               ehss_totals* totals, int64_t* hist_out, double* ledger_out) {
+    for (int64_t k = threadIdx.x; k < bins; k += blockDim.x) {
+        if (hist_out) hist_out[k] = (int64_t)hist[k];
+        if (ledger_out) ledger_out[k] = (double)ledger[k] / kLedgerScale * w;
+    }
+    if (threadIdx.x != 0) return;
+    const Partial* fin = reinterpret_cast<const Partial*>(c + 8);
+    ehss_totals T;
+    memset(&T, 0, sizeof T);
+    T.weight = w;
+    T.omega = fin->s + fin->c;
+    T.omega_err = fin->c;
+    T.rays = n;
+    T.escaped = fin->escaped;
+    T.unresolved = fin->unresolved;
+    T.pa_hits = pa ? fin->pa_hits : 0;
+    T.ehss_hits = fin->ehss_hits;
+    T.grazing_rays = fin->grazing;
+    T.pa = pa ? w * (double)fin->pa_hits : 0.;
+    T.pa_from_ehss = w * (double)fin->ehss_hits;
+    T.residual_mass = w * (double)fin->unresolved;
+    T.tail_bound = 2 * T.residual_mass;
+    T.box_tests = (int64_t)c[0];
+    T.sphere_tests = (int64_t)c[1];
+    T.fp64_retests = (int64_t)(c[2] + c[5]);
+    T.pa_box_tests = (int64_t)c[3];
+    T.pa_sphere_tests = (int64_t)c[4];
+    T.max_bounces = (int64_t)(int)c[6];
+    T.kernel_ms = 0.;
+    *totals = T;
+}
+
 __global__ void k_zero(unsigned long long* p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0ull;
@@ -468,15 +513,18 @@ bool make_cross_maps(const ehss_sobol* sobs, int reps, int m, CrossMap* out) {
 }
 // Persistent drain grid: enough waves to keep every SIMD busy, never more than
 // the rays that could be queued.
+// The CU count is read once per process (call_once: entry points may run on
+// several threads at once; the library is bound to one device).
 int64_t drain_blocks(int64_t n) {
-    static int cus = 0;
-    if (!cus) {
+    static std::once_flag once;
+    static int cus = 256;
+    std::call_once(once, [] {
         int dev = 0;
         hipDeviceProp_t prop;
-        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+            prop.multiProcessorCount > 0)
             cus = prop.multiProcessorCount;
-        if (cus <= 0) cus = 256;
-    }
+    });
     return std::max<int64_t>(1, std::min<int64_t>((int64_t)cus * 8, (n + 255) / 256));
 }
 
@@ -504,6 +552,19 @@ constexpr int kCrossMinReps = 16;  // replicates per launch from which they are 
 #endif
 constexpr size_t kDrainLdsLimit = (size_t)EHSS_DRAIN_LDS_KB * 1024;  // of the 160 KB per CU
 constexpr size_t kSolveLdsLimit = 48 * 1024;   // packet kernels: keeps several blocks per CU
+
+// Raises k_drain<1>'s dynamic LDS limit to kDrainLdsLimit (once per process;
+// the library is bound to one device).
+int set_drain_lds_limit() {
+    static std::once_flag once;
+    static hipError_t status = hipSuccess;
+    std::call_once(once, [] {
+        status = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_drain<1>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDrainLdsLimit);
+    });
+    EHSS_HIP(status);
+    return EHSS_OK;
+}
 
 // Ray index of record slot t (host copy of the device mapping: Gray index from
 // the ray map, then inverse Gray code).
@@ -681,15 +742,7 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
             stripes = sc->drain_stripes.as<unsigned long long>();
             EHSS_TRY(launch_zero(stripes, words, s));
         }
-        if (lds <= kDrainLdsLimit) {
-            static std::once_flag attr_once;
-            hipError_t attr = hipSuccess;
-            std::call_once(attr_once, [&] {
-                attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_drain<1>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kDrainLdsLimit);
-            });
-            EHSS_HIP(attr);
-        }
+        if (lds <= kDrainLdsLimit) EHSS_TRY(set_drain_lds_limit());
         // First drain: A.group lanes per ray; with a spill buffer, rays still
         // bouncing after stage1 queries move to a second drain with whole
         // waves per ray.
@@ -829,6 +882,142 @@ int ehss_solve(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, i
     EHSS_CHECK(totals, "null totals");
     EHSS_TRY(solve_enqueue(scene, 1, sobol, power, cap, precision, flags));
     return solve_collect(scene, totals, histogram, ledger);
+}
+
+// Bytes the allocation-free solve carves at its limits (2^EHSS_NOALLOC_MAX_POWER
+// rays, cap EHSS_NOALLOC_MAX_CAP): counters, histogram, ledger, partials, the
+// scramble and ray map, records, the drain queue; each piece 256-byte aligned.
+constexpr size_t noalloc_piece(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kNoallocMaxRays = (size_t)1 << EHSS_NOALLOC_MAX_POWER;
+constexpr size_t kNoallocMaxBytes =
+    noalloc_piece(8 * (kCounterWords + 2)) + 2 * noalloc_piece(8 * (EHSS_NOALLOC_MAX_CAP + 1)) +
+    noalloc_piece(sizeof(Partial) * ((kNoallocMaxRays + kReduceRays - 1) / kReduceRays)) +
+    noalloc_piece(sizeof(ehss_sobol)) + noalloc_piece(sizeof(RayMap)) + noalloc_piece(4 * kNoallocMaxRays) +
+    noalloc_piece(8 * kNoallocMaxRays) + noalloc_piece(sizeof(RayState) * kNoallocMaxRays);
+static_assert(kNoallocMaxBytes <= kNoallocScratchBytes, "the allocation-free work array is too small");
+
+int ehss_solve_noalloc(ehss_scene* sc, const ehss_sobol* sobol, int power, int cap, int precision, unsigned flags,
+                       void* stream, ehss_totals* totals, int64_t* histogram, double* ledger) {
+    EHSS_TRY(ensure_init());  // the calling thread may not have selected the device
+    EHSS_CHECK(sc && sobol && totals, "null scene/sobol/totals");
+    EHSS_CHECK(power >= 0 && power <= EHSS_NOALLOC_MAX_POWER, "power %d out of range 0..%d of the allocation-free solve",
+               power, EHSS_NOALLOC_MAX_POWER);
+    EHSS_CHECK(cap >= 1 && cap <= EHSS_NOALLOC_MAX_CAP, "cap %d out of range 1..%d of the allocation-free solve", cap,
+               EHSS_NOALLOC_MAX_CAP);
+    EHSS_CHECK(precision == EHSS_PRECISION_FP64 || precision == EHSS_PRECISION_FP32_FIXUP, "unknown precision %d",
+               precision);
+    EHSS_CHECK(!(flags & (EHSS_FLAG_KEEP_RAYS | EHSS_FLAG_TIMESTAMPS)),
+               "the allocation-free solve keeps no per-ray records (KEEP_RAYS / TIMESTAMPS)");
+    const int64_t n = 1LL << power;
+    int variant = (int)((flags >> EHSS_VARIANT_SHIFT) & 0xFu);
+    if (precision == EHSS_PRECISION_FP64) variant = 0;
+    else if (variant == 0) variant = kDefaultVariant;
+    EHSS_CHECK(variant >= 0 && variant <= 4, "unknown kernel variant %d", variant);
+    const int group = group_from_flags(flags, n);
+    const int stage1 = stage1_queries(flags, n);
+    EHSS_CHECK(!(variant == 4 && stage1 > 0 && group > 1),
+               "the allocation-free solve has no second drain queue (EHSS_STAGE1 with EHSS_GROUP_LOG2 > 0)");
+    unsigned char* base = static_cast<unsigned char*>(noalloc_scratch());
+    if (!base) return set_error(EHSS_ERR_HIP, "the allocation-free work array has no address on this device");
+
+    // Carve the work array (256-byte aligned pieces).
+    const int64_t bins = cap + 1;
+    const int64_t nblocks = (n + kReduceRays - 1) / kReduceRays;
+    const int64_t words = (int64_t)kCounterWords + 2;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char* p = base + off;
+        off += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    auto* ctr = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * words));
+    auto* hist = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * bins));
+    auto* led = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * bins));
+    auto* partials = reinterpret_cast<Partial*>(take(sizeof(Partial) * nblocks));
+    auto* d_sob = reinterpret_cast<ehss_sobol*>(take(sizeof(ehss_sobol)));
+    auto* d_map = reinterpret_cast<RayMap*>(take(sizeof(RayMap)));
+    auto* info = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * n));
+    auto* contrib = reinterpret_cast<double*>(take(sizeof(double) * n));
+    RayState* qbuf = variant == 4 ? reinterpret_cast<RayState*>(take(sizeof(RayState) * n)) : nullptr;
+    // The largest carve (2^EHSS_NOALLOC_MAX_POWER rays, cap EHSS_NOALLOC_MAX_CAP) fits: checked per call.
+    if (off > kNoallocScratchBytes)
+        return set_error(EHSS_ERR_INVALID, "allocation-free solve needs %zu bytes of its %zu-byte work array", off,
+                         kNoallocScratchBytes);
+
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : sc->stream;
+    EHSS_TRY(launch_zero(ctr, words, s));
+    EHSS_TRY(launch_zero(hist, bins, s));
+    EHSS_TRY(launch_zero(led, bins, s));
+    RayMap map;
+    make_ray_map(*sobol, power, map.col);
+    hipLaunchKernelGGL(k_put_tables, dim3(1), dim3(128), 0, s, *sobol, map, d_sob, d_map);
+
+    SolveArgs A;
+    A.n = n;
+    A.power = power;
+    A.cap = cap;
+    A.w = M_PI * sc->radius * sc->radius / (double)n;  // np.pi*radius*radius/n
+    A.cx = sc->center[0];
+    A.cy = sc->center[1];
+    A.cz = sc->center[2];
+    A.radius = sc->radius;
+    A.info = info;
+    A.contrib = contrib;
+    A.counters = ctr;
+    A.coop = coop_from_flags(flags);
+    A.group = group;
+    A.packets = packets_from_flags(flags);
+    A.pad_ = 0;
+    A.trig = trig_compat();
+    A.times = nullptr;
+    EHSS_CHECK(A.coop <= kWave, "tail threshold %d out of range 0..64", A.coop);
+    const bool pa = (flags & EHSS_FLAG_PA) != 0;
+    uint32_t* qcount = reinterpret_cast<uint32_t*>(ctr + kCounterWords);
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    const size_t stage_bytes = staged_scene_bytes(sc->view);
+    const bool lds_packets = (variant >= 2) && stage_bytes <= kSolveLdsLimit && (flags & EHSS_FLAG_LDS_PACKETS);
+    const size_t solve_lds = lds_packets ? stage_bytes : 0;
+#define EHSS_LAUNCH_L(M, P, K)                                                                                 \
+    hipLaunchKernelGGL((k_solve<M, P, K>), grid, block, solve_lds, s, sc->view, A, d_sob, d_map, qbuf, qcount)
+#define EHSS_LAUNCH(M)                          \
+    do {                                        \
+        if (lds_packets) {                      \
+            if (pa) EHSS_LAUNCH_L(M, 1, 1);     \
+            else EHSS_LAUNCH_L(M, 0, 1);        \
+        } else {                                \
+            if (pa) EHSS_LAUNCH_L(M, 1, 0);     \
+            else EHSS_LAUNCH_L(M, 0, 0);        \
+        }                                       \
+    } while (0)
+    switch (variant) {
+        case 0: EHSS_LAUNCH(0); break;
+        case 1: EHSS_LAUNCH(1); break;
+        case 2: EHSS_LAUNCH(2); break;
+        case 3: EHSS_LAUNCH(3); break;
+        default: EHSS_LAUNCH(4); break;
+    }
+#undef EHSS_LAUNCH
+#undef EHSS_LAUNCH_L
+    if (variant == 4) {
+        const size_t lds = drain_lds_bytes(sc->view);
+        const dim3 dgrid((unsigned)drain_blocks(n));
+        if (lds <= kDrainLdsLimit) {
+            EHSS_TRY(set_drain_lds_limit());
+            hipLaunchKernelGGL(k_drain<1>, dgrid, dim3(256), lds, s, sc->view, A, 1, qbuf, qcount, qcount + 1, group,
+                               0, nullptr, nullptr, nullptr);
+        } else {
+            hipLaunchKernelGGL(k_drain<0>, dgrid, dim3(256), 0, s, sc->view, A, 1, qbuf, qcount, qcount + 1, group, 0,
+                               nullptr, nullptr, nullptr);
+        }
+    }
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)nblocks, 1u), dim3(256), 0, s, A, nblocks, partials, hist, led);
+    hipLaunchKernelGGL(k_finalize, dim3(1u), dim3(256), 0, s, partials, nblocks, ctr,
+                       (const unsigned long long*)nullptr);
+    hipLaunchKernelGGL(k_write_totals, dim3(1), dim3(256), 0, s, (const unsigned long long*)ctr,
+                       (const unsigned long long*)hist, (const unsigned long long*)led, bins, n, A.w, (int)pa, totals,
+                       histogram, ledger);
+    EHSS_HIP(hipGetLastError());
+    return EHSS_OK;
 }
 
 int ehss_solve_many(ehss_scene* scene, int replicates, const ehss_sobol* sobols, int power, int cap, int precision,

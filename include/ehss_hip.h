@@ -234,6 +234,32 @@ int ehss_cross_map(const ehss_sobol* sobols, int reps, int power, uint32_t* out)
 int ehss_solve(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, int precision, unsigned flags,
                ehss_totals* totals, int64_t* histogram, double* ledger);
 
+/* Allocation-free solve.  No counterpart in the reference: it is for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  One replicate of ehss_solve -- same
+ * kernels, same results -- with every work buffer (records, drain queue,
+ * partials, histogram, counters, scramble tables) in one static device array of
+ * EHSS_NOALLOC_SCRATCH_MB MiB that the library holds on each device until the
+ * process ends.  It calls no hipMalloc/hipFree/hipHostMalloc and no allocating
+ * library routine, records no memset, and does not synchronise: the launches
+ * are enqueued on `stream` (a hipStream_t; NULL = the scene's stream) and the
+ * results are written by a kernel into caller memory the device can write
+ * (device memory, or mapped pinned host memory): totals (one ehss_totals,
+ * kernel_ms = 0), histogram (cap + 1 int64) and ledger (cap + 1 double), both
+ * may be NULL.  Limits: power <= EHSS_NOALLOC_MAX_POWER, cap <=
+ * EHSS_NOALLOC_MAX_CAP; flags as ehss_solve except EHSS_FLAG_KEEP_RAYS,
+ * EHSS_FLAG_TIMESTAMPS and a two-stage drain (EHSS_STAGE1(q > 0) with more than
+ * one lane per ray), which are refused.  Every call on a device shares that
+ * device's array and starts by clearing what must start at zero: calls that
+ * overlap on one device (on different streams, or from different threads) race
+ * and give wrong results.  The scene must not be rebuilt or destroyed before
+ * the stream has passed the call. */
+#define EHSS_NOALLOC_MAX_POWER 16
+#define EHSS_NOALLOC_MAX_CAP 4096
+#define EHSS_NOALLOC_SCRATCH_MB 9
+int ehss_solve_noalloc(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, int precision, unsigned flags,
+                       void* stream, ehss_totals* totals, int64_t* histogram, double* ledger);
+
 /* `replicates` independent solves of the scene in one launch (one scramble each,
  * the replicates of ccs_cli.py:186-227): totals gets one entry per replicate,
  * histograms / ledgers (may be NULL) cap + 1 bins per replicate, replicate-major.
