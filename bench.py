@@ -94,50 +94,53 @@ def rays_per_ccs(scene, scramble, replicate_seed, reps=32, target=0.01):
     return 20, rel
 
 
-def ccs_rate_sequential(scene, sobols, power, precision):
+def ccs_rate_sequential(scene, seeds, power, precision):
+    """One CCS estimate per solve; each scramble is made inside the clock."""
+    from ehss_amd.engine import scramble
     t = time.perf_counter()
-    for s in sobols:
-        scene.solve(s, power, CAP, precision=precision, pa=False)
-    return len(sobols) / (time.perf_counter() - t)
+    for sd in seeds:
+        scene.solve(scramble(sd), power, CAP, precision=precision, pa=False)
+    return len(seeds) / (time.perf_counter() - t)
 
 
-def ccs_rate_overlapped(scenes, sobols, power, precision, per_launch=1):
+def ccs_rate_overlapped(scenes, seeds, power, precision, per_launch=1):
     """CCS estimates with one launch of `per_launch` replicates in flight per
-    scene copy (each on its own stream), refilled as they finish.  The
-    scrambles of each launch are converted once, before the clock starts."""
-    from ehss_amd.engine import sobol_table
-    chunks = [sobol_table(sobols[k:k + per_launch]) for k in range(0, len(sobols), per_launch)]
-    it = iter(chunks)
+    scene copy (each on its own stream), refilled as they finish.  Each launch's
+    scrambles are made (ehss_scramble_many) when it is issued, inside the clock,
+    while the other launch runs on the device."""
+    from ehss_amd.engine import scramble_many
+    chunks = (seeds[k:k + per_launch] for k in range(0, len(seeds), per_launch))
     live = []
     count = 0
     t = time.perf_counter()
     for s in scenes:
-        nxt = next(it, None)
+        nxt = next(chunks, None)
         if nxt is None:
             break
-        s.solve_async(nxt, power, CAP, precision=precision, pa=False)
+        s.solve_async(scramble_many(nxt), power, CAP, precision=precision, pa=False)
         live.append(s)
     while live:
         for s in list(live):
             if s.solve_done():
                 count += len(s.solve_wait(as_array=True))
-                nxt = next(it, None)
+                nxt = next(chunks, None)
                 if nxt is None:
                     live.remove(s)
                 else:
-                    s.solve_async(nxt, power, CAP, precision=precision, pa=False)
+                    s.solve_async(scramble_many(nxt), power, CAP, precision=precision, pa=False)
         time.sleep(2e-5)
     return count / (time.perf_counter() - t)
 
 
-def ccs_rate_replicated(scene, sobols, power, precision, per_launch):
-    """CCS estimates as replicates of one launch (ehss_solve_many)."""
-    from ehss_amd.engine import sobol_table
-    chunks = [sobol_table(sobols[k:k + per_launch]) for k in range(0, len(sobols), per_launch)]
+def ccs_rate_replicated(scene, seeds, power, precision, per_launch):
+    """CCS estimates as replicates of one launch (ehss_solve_many); scrambles
+    made inside the clock."""
+    from ehss_amd.engine import scramble_many
     t = time.perf_counter()
     count = 0
-    for chunk in chunks:
-        count += len(scene.solve_many(chunk, power, CAP, precision=precision, pa=False, as_array=True))
+    for k in range(0, len(seeds), per_launch):
+        count += len(scene.solve_many(scramble_many(seeds[k:k + per_launch]), power, CAP, precision=precision,
+                                      pa=False, as_array=True))
     return count / (time.perf_counter() - t)
 
 
@@ -160,19 +163,18 @@ def batched18(molecules, scramble, replicate_seed, power=20, cap=64, launches=10
         return [scramble(replicate_seed(3 + j, k)) for j in range(jobs)]
 
     def plan_rate(plans, count):
-        """Launches of new scrambles (made before the clock starts), with
-        len(plans) launches in flight, one per plan (each plan has its own
-        stream and workspace; they share the resident scenes)."""
-        batches = [scrambles(k + 1) for k in range(count)]
+        """Launches of new scrambles (made inside the clock), with len(plans)
+        launches in flight, one per plan (each plan has its own stream and
+        workspace; they share the resident scenes)."""
         for p in plans:
             p.run()
         t0 = time.perf_counter()
         live = []
-        for k, batch in enumerate(batches):
+        for k in range(count):
             p = plans[k % len(plans)]
             if len(live) == len(plans):
                 live.pop(0).wait()
-            for j, sob in enumerate(batch):
+            for j, sob in enumerate(scrambles(k + 1)):
                 p.set_sobol(j, sob)
             p.launch()
             live.append(p)
@@ -186,8 +188,8 @@ def batched18(molecules, scramble, replicate_seed, power=20, cap=64, launches=10
     try:
         t0 = time.perf_counter()
         for k in range(2):
-            for s, sob in zip(scs, scrambles(k)):
-                s.solve(sob, power, cap, pa=True)
+            for j, s in enumerate(scs):
+                s.solve(scramble(replicate_seed(3 + j, k)), power, cap, pa=True)
         seq = 2 * rays / (time.perf_counter() - t0)
         with GPUPlan(scs, scrambles(0), [power] * jobs, [cap] * jobs, precision="fp64", graph=True) as plan:
             plain = plan_rate([plan], 2)
@@ -215,62 +217,145 @@ def batched18(molecules, scramble, replicate_seed, power=20, cap=64, launches=10
             s.close()
 
 
-def cpu_model() -> str:
+def cpu_host_info() -> dict:
+    """The host CPU as lscpu reports it (model name, family/model/stepping,
+    sockets, cores, clock), with /proc/cpuinfo's model name as a fallback."""
+    import subprocess
+    info = {}
     try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
-    except OSError:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Vendor ID": "vendor", "CPU family": "family", "Model": "model_number",
+                "Stepping": "stepping", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus", "CPU max MHz": "max_mhz",
+                "Hypervisor vendor": "hypervisor"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys and v.strip():
+                info.setdefault(keys[k.strip()], v.strip())
+    except (OSError, subprocess.SubprocessError):
         pass
-    return "unknown"
+    if "model" not in info:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        info["model"] = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            info["model"] = "unknown"
+    return info
 
 
-# Host facts of the CPU baseline (model, one-core rate), reported beside it.
-cpu_host = {}
+# Per-config CPU samples: (config, workload, seeds, 2^power rays per seed, cap).
+# C1 is the whole configuration; C2 and C3 are bounded samples of theirs.
+CPU_CONFIGS = (("C1", "c60", (0, 104729, 209458), 16, 64),
+               ("C2", "globule650", (SEED,), 21, 64),
+               ("C3", "ellipsoid2000", (2,), 20, 128))
+CPU_RUNS = 5
 
 
-def cpu_baseline(centers, radii, sample_power=21):
-    """Reference CPU kernel (src/native/ehss_cpu.cpp, built from the reference
-    sources into oracle/_ref) on a bounded sample of the same workload, all
-    host cores, and on one core; the project's C restatement of the whole path
-    when the reference build is absent."""
+def cpu_baseline(molecules):
+    """CPU baselines per config on the host cores, median of CPU_RUNS warm runs:
+    the reference's own ehss_trace (src/native/ehss_cpu.cpp built from the
+    reference sources into oracle/_ref) on the NumPy-path source of the same
+    points (trace only), and the project's C restatement of the whole path
+    (Sobol, source, PA, trees, trace, readout; kind "port").  The headline
+    cpu_baseline value is C2's reference trace rate."""
     from oracle import oracle as O
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    cpu_host.update(model=cpu_model(), cores_used=cores, host_cpus=os.cpu_count())
+    host = cpu_host_info()
+    host.update(cores_used=cores)
     ref = O.ref_lib()
+    per = {}
+
+    def median_time(fn):
+        fn()  # warm
+        ts = []
+        for _ in range(CPU_RUNS):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    for name, wl, seeds, power, cap in CPU_CONFIGS:
+        c, r, _ = molecules.workload(wl)
+        rays = len(seeds) << power
+        tree = O.static_tree(c, r, 4)
+        row = dict(workload=wl, rays=rays, seeds=list(seeds), cap=cap, runs=CPU_RUNS, threads=cores)
+        if ref is not None:
+            srcs = [O.external_source(c, r, power, sd) for sd in seeds]
+
+            def trace_all(threads=cores, m=None):
+                for src in srcs:
+                    O.trace(tree, c, r, src["origins"][:m], src["directions"][:m], cap=cap, record=False,
+                            lib_handle=ref, threads=threads)
+            row["reference_trace_rays_per_s"] = rays / median_time(trace_all)
+            if name == "C2":
+                m = 1 << (power - 4)
+                row["reference_trace_rays_per_s_one_core"] = m / median_time(lambda: trace_all(1, m))
+        row["whole_path_rays_per_s"] = rays / median_time(
+            lambda: [O.solve_cpu(c, r, sd, power, cap, threads=cores) for sd in seeds])
+        per[name] = row
+        say(f"cpu {name}: {row}")
+    c2 = per["C2"]
     if ref is not None:
-        src = O.external_source(centers, radii, sample_power, SEED)
-        tree = O.static_tree(centers, radii, 4)
-        times = []
-        for _ in range(3):
+        value, kind = c2["reference_trace_rays_per_s"], "reference"
+        sample = (f"reference ehss_trace (src/native/ehss_cpu.cpp, g++ -O2 -fopenmp) on the first 2^21 rays of "
+                  f"the globule650 net (seed {SEED}, cap 64), trace only (source, PA and readout excluded), "
+                  f"median of {CPU_RUNS} warm runs with {cores} OpenMP threads; per_config has C1 (the whole "
+                  f"config, 3 x 2^16 rays) and C3 (2^20 rays, cap 128) and the whole-path rates")
+    else:
+        value, kind = c2["whole_path_rays_per_s"], "port"
+        sample = f"oracle C restatement of the whole path on 2^21 rays, median of {CPU_RUNS}"
+    return dict(value=value, unit="rays/s", cores=cores, kind=kind, sample=sample, host=host, per_config=per)
+
+
+def seam_rates(scene, power=20, runs=3):
+    """PCIe-inclusive rates of the per-ray seams, on the first 2^power rays of
+    the workload's net (host arrays in, host arrays out): ehss_trace (the
+    reference's C signature: tree and rays uploaded, plain FP64 trace, outputs
+    downloaded, every call), ehss_trace_status in FP32+fixup, and
+    ehss_scene_trace against the resident scene."""
+    import ctypes as ct
+    from ehss_amd import _lib as L
+    from ehss_amd.engine import scramble
+    src = scene.source_rays(scramble(SEED), power)
+    tree = scene.export_tree()
+    c = np.ascontiguousarray(scene.centers, dtype=np.float64)
+    r = np.ascontiguousarray(scene.radii, dtype=np.float64)
+    n = 1 << power
+    nn = len(tree["end"])
+
+    def seam(precision):
+        p, d = src["origins"].copy(), src["directions"].copy()
+        lasts = np.full(n, -1, np.int64)
+        counts = np.zeros(n, np.int64)
+        esc, unr = np.zeros(n, np.uint8), np.zeros(n, np.uint8)
+        ctr = np.zeros(2, np.int64)
+        arrays = [p, d, lasts, counts, esc, unr, None, tree["lo"], tree["hi"], tree["end"], tree["leaf_group"],
+                  tree["offsets"], tree["ids"], c, r, ctr]
+        if precision is None:
+            obs = ct.c_int(0)
+            L.fn("ehss_trace")(n, nn, CAP, 16, 0, *(L.ptr(a) for a in arrays), ct.byref(obs))
+            if obs.value < 0:
+                raise RuntimeError(L.last_error())
+        else:
+            L.call("ehss_trace_status", n, nn, CAP, L.PRECISIONS[precision], 0, *(L.ptr(a) for a in arrays))
+
+    def med(fn):
+        fn()
+        ts = []
+        for _ in range(runs):
             t0 = time.perf_counter()
-            O.trace(tree, centers, radii, src["origins"], src["directions"], cap=CAP, record=False,
-                    lib_handle=ref, threads=cores)
-            times.append(time.perf_counter() - t0)
-        t = float(np.median(times))
-        # The same kernel on one core, on a sixteenth of the sample.
-        one = []
-        for _ in range(3):
-            t0 = time.perf_counter()
-            O.trace(tree, centers, radii, src["origins"][: 1 << (sample_power - 4)],
-                    src["directions"][: 1 << (sample_power - 4)], cap=CAP, record=False, lib_handle=ref, threads=1)
-            one.append(time.perf_counter() - t0)
-        cpu_host.update(rays_per_s_one_core=(1 << (sample_power - 4)) / float(np.median(one)),
-                        one_core_sample=f"the first 2^{sample_power - 4} rays of the same sample, 1 OpenMP thread, "
-                                        f"median of 3")
-        return dict(value=(1 << sample_power) / t, unit="rays/s", cores=cores, kind="reference",
-                    sample=f"reference ehss_trace (src/native/ehss_cpu.cpp, g++ -O2 -fopenmp) on 2^{sample_power} "
-                           f"rays of the same molecule, seed and cap; trace only (source, PA and readout "
-                           f"excluded); median of 3 warm runs with {cores} OpenMP threads")
-    times = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        O.solve_cpu(centers, radii, SEED, sample_power - 2, CAP, threads=cores)
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    return dict(value=(1 << (sample_power - 2)) / t, unit="rays/s", cores=cores, kind="port",
-                sample=f"oracle C restatement of the whole path on 2^{sample_power - 2} rays, median of 3")
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return n / float(np.median(ts))
+
+    return dict(rays=n, bytes_per_ray_each_way=2 * 24 + 8 + 8 + 2,
+                ehss_trace_fp64_rays_per_s=med(lambda: seam(None)),
+                ehss_trace_status_fp32_fixup_rays_per_s=med(lambda: seam("fp32-fixup")),
+                scene_trace_fp32_fixup_rays_per_s=med(lambda: scene.trace(src["origins"], src["directions"], cap=CAP)),
+                note="host arrays in and out: PCIe transfers, uploads and the final synchronisation included")
 
 
 def traffic_probe(args):
@@ -320,8 +405,6 @@ def main():
     # Warm-up launches are whole launches, at least one per stream, so that no
     # workspace grows inside the timed region.
     warm = max(args.warmup, 2 * STEPS_PER_LAUNCH)
-    seeds = [replicate_seed(SEED, k) for k in range(warm + args.steps)]
-    sobols = [scramble(s) for s in seeds]
     n = 1 << POWER
 
     # Warmup, then exactly K timed steps in launches of STEPS_PER_LAUNCH (the
@@ -333,8 +416,11 @@ def main():
     scene2 = GPUScene(centers, radii)
 
     def batches(first, count):
-        return [sobol_table(sobols[b:min(b + STEPS_PER_LAUNCH, first + count)])
-                for b in range(first, first + count, STEPS_PER_LAUNCH)]
+        """Each launch's scrambles (host PCG64 + LMS, ehss_scramble) and launch
+        table, built when the launch is issued: inside the timed region."""
+        for b in range(first, first + count, STEPS_PER_LAUNCH):
+            yield sobol_table([scramble(replicate_seed(SEED, k)) for k in range(b, min(b + STEPS_PER_LAUNCH,
+                                                                                          first + count))])
 
     def run_overlapped(tabs):
         pending, it, out = [], iter(tabs), None
@@ -354,9 +440,8 @@ def main():
         return out
 
     run_overlapped(batches(0, warm))
-    timed = batches(warm, args.steps)
     t0 = time.perf_counter()
-    res = run_overlapped(timed)[-1]
+    res = run_overlapped(batches(warm, args.steps))[-1]
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
     value = n * args.steps / elapsed
@@ -365,6 +450,8 @@ def main():
 
     # Kernel time per launch for the roofline: the same launches one at a time
     # on one stream, HIP events around the trace kernels.
+    timed = list(batches(warm, args.steps))
+    sobols = [scramble(replicate_seed(SEED, k)) for k in range(max(20, 2 * STEPS_PER_LAUNCH))]
     kernel_ms, launch_rays = [], []
     for tb in timed:
         out = scene.solve_many(tb, POWER, CAP, precision=args.precision, pa=True, as_array=True)
@@ -415,16 +502,16 @@ def main():
         say("CCS at 1% RMSE")
         power_ccs, rel = rays_per_ccs(scene, scramble, replicate_seed)
         per = CCS_PER_LAUNCH
-        ccs_sobols = [scramble(replicate_seed(7, k)) for k in range(4 * per)]
-        seq_fast = ccs_rate_sequential(scene, ccs_sobols[:400], power_ccs, "fp32-fixup")
-        seq_plain = ccs_rate_sequential(scene, ccs_sobols[:100], power_ccs, "fp64")
-        ccs_rate_replicated(scene, ccs_sobols[:per], power_ccs, "fp32-fixup", per)
-        rep_fast = ccs_rate_replicated(scene, ccs_sobols, power_ccs, "fp32-fixup", per)
-        rep_plain = ccs_rate_replicated(scene, ccs_sobols[:per], power_ccs, "fp64", per)
+        ccs_seeds = np.array([replicate_seed(7, k) for k in range(4 * per)], dtype=np.uint64)
+        seq_fast = ccs_rate_sequential(scene, ccs_seeds[:400], power_ccs, "fp32-fixup")
+        seq_plain = ccs_rate_sequential(scene, ccs_seeds[:100], power_ccs, "fp64")
+        ccs_rate_replicated(scene, ccs_seeds[:per], power_ccs, "fp32-fixup", per)
+        rep_fast = ccs_rate_replicated(scene, ccs_seeds, power_ccs, "fp32-fixup", per)
+        rep_plain = ccs_rate_replicated(scene, ccs_seeds[:per], power_ccs, "fp64", per)
         copies = [GPUScene(centers, radii) for _ in range(2)]
-        ccs_rate_overlapped(copies, ccs_sobols[:2 * per], power_ccs, "fp32-fixup", per)
-        ovl_fast = ccs_rate_overlapped(copies, ccs_sobols, power_ccs, "fp32-fixup", per)
-        ovl_plain = ccs_rate_overlapped(copies, ccs_sobols[:2 * per], power_ccs, "fp64", per)
+        ccs_rate_overlapped(copies, ccs_seeds[:2 * per], power_ccs, "fp32-fixup", per)
+        ovl_fast = ccs_rate_overlapped(copies, ccs_seeds, power_ccs, "fp32-fixup", per)
+        ovl_plain = ccs_rate_overlapped(copies, ccs_seeds[:2 * per], power_ccs, "fp64", per)
         for c in copies:
             c.close()
         extras["globule650"] = dict(
@@ -444,6 +531,8 @@ def main():
             batched_fp32_fixup_vs_plain_fp64_kernel=best_fast / seq_plain,
             batched_fp32_fixup_vs_batched_plain_fp64=best_fast / max(rep_plain, ovl_plain),
             rays_per_s_at_ccs_size=best_fast * (1 << power_ccs))
+        say("per-ray seams (PCIe-inclusive)")
+        extras["seam"] = seam_rates(scene)
         scene.close()
 
         say("ellipsoid2000, 2^22 rays, cap 128")
@@ -494,9 +583,9 @@ def main():
     else:
         scene.close()
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(centers, radii)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(molecules)
     if cpu is not None:
-        ratios = dict(rays_per_s_vs_cpu=value / cpu["value"], cpu_host=dict(cpu_host))
+        ratios = dict(rays_per_s_vs_cpu=value / cpu["value"])
         if "ccs_1pct" in extras:
             cc = extras["ccs_1pct"]
             cpu_ccs = cpu["value"] / cc["rays_per_ccs"]

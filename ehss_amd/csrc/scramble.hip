@@ -16,7 +16,11 @@
 // reference these bits are tested against.
 #include <string.h>
 
+#include <algorithm>
+#include <atomic>
 #include <mutex>
+#include <thread>
+#include <vector>
 
 #include "ehss_internal.h"
 
@@ -142,8 +146,9 @@ void direction_numbers(uint32_t v[kDim][kBits]) {
 
 using namespace ehss;
 
-extern "C" int ehss_scramble(uint64_t seed, ehss_sobol* out) {
-    EHSS_CHECK(out, "null output");
+namespace {
+
+void scramble_one(uint64_t seed, ehss_sobol* out) {
     static uint32_t v[kDim][kBits];
     static std::once_flag once;
     std::call_once(once, [] { direction_numbers(v); });
@@ -179,5 +184,29 @@ extern "C" int ehss_scramble(uint64_t seed, ehss_sobol* out) {
             out->sv[d][j] = w;
         }
     }
+}
+
+}  // namespace
+
+extern "C" int ehss_scramble(uint64_t seed, ehss_sobol* out) {
+    EHSS_CHECK(out, "null output");
+    scramble_one(seed, out);
+    return EHSS_OK;
+}
+
+// Many scrambles on up to 16 host threads (chunks of 16 seeds).
+extern "C" int ehss_scramble_many(int64_t n, const uint64_t* seeds, ehss_sobol* out) {
+    EHSS_CHECK(n >= 0 && (n == 0 || (seeds && out)), "invalid ehss_scramble_many arguments");
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int threads = (int)std::min<int64_t>({(int64_t)hw, 16, n / 32});
+    std::atomic<int64_t> next{0};
+    auto work = [&] {
+        for (int64_t i; (i = next.fetch_add(16)) < n;)
+            for (int64_t k = i; k < std::min(n, i + 16); ++k) scramble_one(seeds[k], &out[k]);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
     return EHSS_OK;
 }

@@ -84,6 +84,15 @@ def scramble(seed: int) -> L.Sobol:
     return s
 
 
+def scramble_many(seeds) -> ct.Array:
+    """scramble() of every seed, on the host's cores (ehss_scramble_many); the
+    result is a launch table (sobol_table) as it is."""
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+    out = (L.Sobol * len(seeds))()
+    L.call("ehss_scramble_many", len(seeds), L.ptr(seeds), ct.byref(out))
+    return out
+
+
 def scramble_py(seed: int) -> L.Sobol:
     """scramble() from the NumPy restatement (scramble_matrices)."""
     sv, shift = scramble_matrices(seed, 4)

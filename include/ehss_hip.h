@@ -212,6 +212,10 @@ typedef struct {
  * default_rng(seed) stream (SeedSequence + PCG64, Lemire bounded draws) as SciPy
  * does.  Replaces the host scramble of the reference's source per replicate. */
 int ehss_scramble(uint64_t seed, ehss_sobol* out);
+/* ehss_scramble for n seeds on the host's cores (up to 16 threads): out[i] is
+ * the net of seeds[i].  One CCS estimate needs one scramble, so batched
+ * estimates make many per launch. */
+int ehss_scramble_many(int64_t n, const uint64_t* seeds, ehss_sobol* out);
 
 /* Host-only: the wave-coherent ray map used by ehss_solve for 2^power rays
  * (cols[32]; no reference counterpart).  Thread t = 64 w + l traces the ray with

@@ -37,3 +37,17 @@ def test_seed_range():
         scramble(-1)
     with pytest.raises(ValueError):
         scramble(2 ** 64)
+
+
+def test_scramble_many_equals_one_at_a_time():
+    import ctypes as ct
+    from ehss_amd.engine import scramble_many
+    rng = np.random.default_rng(5)
+    seeds = np.concatenate([np.array([0, 1, 2**32 - 1, 2**63, 2**64 - 1], dtype=np.uint64),
+                            rng.integers(0, 2**63, 700, dtype=np.int64).astype(np.uint64)])
+    many = scramble_many(seeds)
+    assert len(many) == len(seeds)
+    for s, got in zip(seeds, many):
+        assert bytes(got) == bytes(scramble(int(s))), int(s)
+    assert len(scramble_many(np.zeros(0, np.uint64))) == 0
+    assert ct.sizeof(many) == len(seeds) * ct.sizeof(type(many[0]))
