@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -190,6 +191,9 @@ struct ehss_scene {
     ehss::SceneView view;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;
+    // prepared plans holding this scene's views (ehss_plan_create counts one per
+    // job): rebuild and destroy are refused while nonzero
+    std::atomic<int> plans{0};
     // pending asynchronous solve (at most one per scene)
     bool pending = false;
     int p_reps = 1;

@@ -110,6 +110,7 @@ using namespace ehss;
 struct ehss_plan {
     int n_jobs = 0, precision = 0, variant = 0, coop = 0, group = 1, stage1 = 0;
     bool pa = false, use_graph = false, captured = false, dirty = true;
+    bool holds = false;  // the scenes count this plan (ehss_scene::plans)
     std::vector<PlanJob> jobs;
     std::vector<ehss_scene*> scenes;
     DevBuf jobs_dev, info, contrib, partials, words, bins, queue, queue2;
@@ -129,6 +130,8 @@ constexpr size_t kPlanLdsLimit = 48 * 1024;     // packet kernels: keeps several
 void destroy_plan(ehss_plan* p) {
     if (!p) return;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
+    if (p->holds)
+        for (ehss_scene* sc : p->scenes) sc->plans.fetch_sub(1);
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->ev0) (void)hipEventDestroy(p->ev0);
@@ -296,6 +299,10 @@ int create_plan(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols,
         for (int j = 0; j < n_jobs; ++j) most = std::max(most, staged_scene_bytes(scenes[j]->view));
         if (most <= kPlanLdsLimit) p->lds_bytes = most;
     }
+    // The plan copied each scene's views (device pointers, counts, disk):
+    // the scenes refuse rebuild and destroy until the plan is destroyed.
+    for (ehss_scene* sc : p->scenes) sc->plans.fetch_add(1);
+    p->holds = true;
     *out = p;
     p = nullptr;
     return EHSS_OK;

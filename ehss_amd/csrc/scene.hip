@@ -239,6 +239,8 @@ int ehss_scene_rebuild(ehss_scene* scene, int64_t n, const double* centers, cons
     EHSS_TRY(ensure_init());  // the calling thread (a serving worker) may not have selected the device
     EHSS_CHECK(scene, "null scene");
     if (scene->pending) return set_error(EHSS_ERR_BUSY, "a solve is pending on this scene");
+    if (const int held = scene->plans.load())
+        return set_error(EHSS_ERR_BUSY, "the scene is held by %d plan job(s); destroy the plan(s) first", held);
     std::vector<int32_t> cg;
     int32_t custom_ng = 0;
     EHSS_TRY(check_structure(n, centers, radii, nullptr, leaf_size, cg, &custom_ng));
@@ -248,6 +250,8 @@ int ehss_scene_rebuild(ehss_scene* scene, int64_t n, const double* centers, cons
 
 int ehss_scene_destroy(ehss_scene* scene) {
     if (!scene) return EHSS_OK;
+    if (const int held = scene->plans.load())
+        return set_error(EHSS_ERR_BUSY, "the scene is held by %d plan job(s); destroy the plan(s) first", held);
     if (scene->stream) (void)hipStreamSynchronize(scene->stream);
     if (scene->ev0) (void)hipEventDestroy(scene->ev0);
     if (scene->ev1) (void)hipEventDestroy(scene->ev1);

@@ -229,7 +229,7 @@ class GPUScene:
     # -- lifetime -------------------------------------------------------------
     def close(self):
         if getattr(self, "_h", None):
-            L.fn("ehss_scene_destroy")(self._h)
+            L.check(L.fn("ehss_scene_destroy")(self._h))  # refused while a plan holds the scene
             self._h = None
 
     def __enter__(self):

@@ -156,10 +156,11 @@ int ehss_scene_create_grouped(int64_t n, const double* centers, const double* ra
  * (ehss_cpu_workload.py:35-46): the trees and filter arrays are rebuilt on the
  * scene's stream into the scene's existing device arrays, which only grow, so a
  * structure no larger than the largest one the scene has held allocates and
- * frees nothing.  Fails with EHSS_ERR_BUSY while a solve is pending; per-ray
- * records of the previous structure are dropped.  Same validation as
- * ehss_scene_create. */
+ * frees nothing.  Fails with EHSS_ERR_BUSY while a solve is pending or while a
+ * prepared plan (ehss_plan_create) holds the scene; per-ray records of the
+ * previous structure are dropped.  Same validation as ehss_scene_create. */
 int ehss_scene_rebuild(ehss_scene* scene, int64_t n, const double* centers, const double* radii, int leaf_size);
+/* Fails with EHSS_ERR_BUSY (and destroys nothing) while a plan holds the scene. */
 int ehss_scene_destroy(ehss_scene* scene);
 int ehss_scene_get_info(const ehss_scene* scene, ehss_scene_info* info);
 /* Copies the static tree back (arrays sized from ehss_scene_get_info):
@@ -333,7 +334,9 @@ int ehss_pa_grid_count(int64_t n_atoms, const double* xy, const double* radii, c
  * histogram and ledger equal those of ehss_solve on the same scene, scramble,
  * power, cap, precision and flags.  Replaces, for a set of structures, the
  * per-frame loop of the CLI (ccs_cli.py:305-307, one solve_frame per structure,
- * ccs_cli.py:170).  Scenes must outlive the plan and must have no solve pending
+ * ccs_cli.py:170).  The plan copies each job's scene views at creation, so it
+ * holds its scenes: ehss_scene_rebuild and ehss_scene_destroy fail with
+ * EHSS_ERR_BUSY until ehss_plan_destroy.  Scenes must have no solve pending
  * while a plan launch runs.
  * ------------------------------------------------------------------------- */
 typedef struct ehss_plan ehss_plan;

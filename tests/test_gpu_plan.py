@@ -187,3 +187,27 @@ def test_two_plans_in_flight_over_shared_scenes():
     finally:
         for s in sc:
             s.close()
+
+
+def test_plan_holds_its_scenes():
+    """A plan copies its scenes' views: rebuilding or destroying a held scene is
+    refused (EHSS_ERR_BUSY) until the plan is destroyed, then allowed."""
+    from ehss_amd import _lib as L
+    a, b = scenes.globule(300, 4), scenes.globule(650, 1)
+    sc = GPUScene(*a)
+    other = GPUScene(*b)
+    try:
+        plan = GPUPlan([sc, other, sc], [scramble(1), scramble(2), scramble(3)], [10] * 3, [64] * 3)
+        before = plan.run()
+        for fn in (lambda: sc.rebuild(*b), lambda: other.rebuild(*a)):
+            with pytest.raises(L.EHSSError, match="held by"):
+                fn()
+        with pytest.raises(L.EHSSError, match="held by 2 plan job"):
+            sc.close()
+        assert plan.run()[0]["omega"] == before[0]["omega"]  # the plan still works, unchanged
+        plan.close()
+        sc.rebuild(*b)
+        assert sc.info["atoms"] == len(b[1])
+    finally:
+        sc.close()
+        other.close()
