@@ -149,3 +149,33 @@ def test_default_adaptive_pa_single_sphere(tmp_path):
     assert a["summary"]["pa_angstrom2"]["mean"] == pytest.approx(truth, rel=0.01)
     assert a["summary"]["ehss_angstrom2"]["mean"] == pytest.approx(truth, rel=2e-6)
     assert a["summary"] == solve_frame(s, args, False)["summary"]
+
+
+def test_main_exits_2_on_invalid_input(tmp_path, capsys):
+    from ehss_amd.ccs_cli import main
+    with pytest.raises(SystemExit) as exc:
+        main([str(tmp_path / "missing*.xyz"), "--output", str(tmp_path / "out")])
+    assert exc.value.code == 2 and "No XYZ files match" in capsys.readouterr().err
+    (tmp_path / "out").mkdir()
+    (tmp_path / "a.xyz").write_text("1\n\nC 0 0 0\n")
+    with pytest.raises(SystemExit) as exc:
+        main([str(tmp_path / "a.xyz"), "--output", str(tmp_path / "out")])
+    assert exc.value.code == 2 and "already exists" in capsys.readouterr().err
+
+
+def test_csv_row_leaves_unrequested_methods_blank():
+    from ehss_amd.ccs_cli import CSV_COLUMNS, _csv_row
+    rec = dict(input_file="m.xyz", frame=2, selected_atoms=7, timings=dict(all_replicates_with_setup_s=0.5),
+               summary=dict(ehss_angstrom2=dict(mean=10.0, sd=None), ehss_tail_bound_mean_angstrom2=0.0,
+                            ehss_unresolved_rays_total=0))
+    row = _csv_row(rec)
+    assert tuple(row) == CSV_COLUMNS
+    assert row["pa_mean_A2"] is None and row["pa_sd_A2"] is None and row["ehss_mean_A2"] == 10.0
+    assert row["atoms"] == 7 and row["frame"] == 2
+
+
+def test_signed_zero_coordinates_are_coincident(tmp_path):
+    p = tmp_path / "z.xyz"
+    p.write_text("3\n\nC 0 0 0\nO 1 1 1\nN -0.0 0 -0\n")
+    with pytest.raises(ValueError, match="coincident"):
+        prepare_frame(read_xyz(p)[0], radius_table(), 1.4)
