@@ -24,14 +24,15 @@ hipStream_t g_stream = nullptr;
 // Compatibility table (ehss_trig.h): one device allocation, kept until the
 // process ends.
 void* g_trig_mem = nullptr;
-ehss_trig::CompatTable g_trig{nullptr, nullptr};
+ehss_trig::CompatTable g_trig{nullptr, nullptr, 0.0};
 int64_t g_trig_entries = -1;
 std::string g_trig_libm, g_trig_path;
 
 struct TrigFileHeader {
     char magic[8];
     uint32_t key_bits, shift, buckets, entries, not_sure, libm_ulp2;
-    char libm[32];
+    double lookup_from;
+    char libm[24];
 };
 static_assert(sizeof(TrigFileHeader) == 64, "trig table header is 64 bytes");
 
@@ -56,9 +57,9 @@ int load_trig_compat() {
     TrigFileHeader h;
     std::vector<uint32_t> off;
     std::vector<uint16_t> ent;
-    bool ok = fread(&h, sizeof h, 1, f) == 1 && memcmp(h.magic, "EHSSTRG1", 8) == 0 && h.key_bits == 30 &&
+    bool ok = fread(&h, sizeof h, 1, f) == 1 && memcmp(h.magic, "EHSSTRG2", 8) == 0 && h.key_bits == 30 &&
               h.shift == (uint32_t)ehss_trig::kCompatShift && h.buckets == ehss_trig::kCompatBuckets &&
-              h.libm_ulp2 == 0;
+              h.libm_ulp2 == 0 && h.lookup_from > 0.0 && h.lookup_from <= 0.5;
     if (ok) {
         off.resize((size_t)h.buckets + 1);
         ent.resize(h.entries);
@@ -82,6 +83,7 @@ int load_trig_compat() {
                            hipMemcpyHostToDevice));
     g_trig.offsets = (const uint32_t*)g_trig_mem;
     g_trig.entries = (const uint16_t*)((char*)g_trig_mem + off_bytes);
+    g_trig.lookup_from = h.lookup_from;
     g_trig_entries = h.entries;
     h.libm[sizeof h.libm - 1] = 0;
     g_trig_libm = h.libm;

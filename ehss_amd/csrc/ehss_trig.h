@@ -20,7 +20,11 @@
 //     rounded on about 0.15% of them.  A table, generated at build time by
 //     evaluating the host libm on all 2^30 angles (trig_host.cpp), lists every q
 //     whose libm result differs from step 1, with the direction of the one-ulp
-//     step.  ehss_sincos_sobol applies it.
+//     step.  ehss_sincos_sobol applies it.  The libm can only round the other
+//     way when the exact value lies very close to a rounding midpoint, so the
+//     table also records the threshold `lookup_from`: every listed angle has
+//     |lo| >= lookup_from * ulp(hi) for its double-double result, and only
+//     angles meeting that test (about 3 % of them) are looked up.
 //
 // The arithmetic uses only IEEE +, -, *, fma, rint and exact scalings, so the
 // host build (g++ -ffp-contract=off) and the device build give the same bits.
@@ -150,7 +154,18 @@ constexpr uint32_t kCompatBuckets = 1u << (30 - kCompatShift);
 struct CompatTable {
     const uint32_t* offsets;  // kCompatBuckets + 1
     const uint16_t* entries;
+    double lookup_from;       // look up only when |lo| >= lookup_from * ulp(hi) for cos or sin
 };
+
+// Spacing of the doubles at |h| (frexp's 2^(e - 53)); exact.
+EHSS_TRIG_HD double ulp_of(double h) {
+    int e;
+    frexp(h, &e);
+    return ldexp(1.0, e - 53);
+}
+
+// |lo| / ulp(hi) >= from, evaluated exactly (ulp_of is a power of two).
+EHSS_TRIG_HD bool near_midpoint(DD v, double from) { return fabs(v.lo) >= from * ulp_of(v.hi); }
 
 EHSS_TRIG_HD double ulp_step(double v, bool up) {
     return nextafter(v, up ? INFINITY : -INFINITY);
@@ -173,8 +188,10 @@ EHSS_TRIG_HD uint32_t compat_flags(CompatTable t, uint32_t q) {
 // cos and sin of the Sobol angle x = fl(2pi * q 2^-30) as the host libm
 // returns them (correctly rounded when the table is absent).
 EHSS_TRIG_HD void sincos_sobol(double x, uint32_t q, CompatTable t, double* s, double* c) {
-    int sure;
-    sincos_cr(x, s, c, &sure);
+    const SinCos v = sincos_dd(x);
+    *s = v.s.hi;
+    *c = v.c.hi;
+    if (!t.offsets || !(near_midpoint(v.c, t.lookup_from) || near_midpoint(v.s, t.lookup_from))) return;
     const uint32_t f = compat_flags(t, q);
     if (f & 1u) *c = ulp_step(*c, (f & 2u) != 0);
     if (f & 4u) *s = ulp_step(*s, (f & 8u) != 0);

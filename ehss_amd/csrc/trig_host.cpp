@@ -27,14 +27,15 @@ using namespace ehss_trig;
 namespace {
 
 struct FileHeader {
-    char magic[8];        // "EHSSTRG1"
+    char magic[8];        // "EHSSTRG2"
     uint32_t key_bits;    // 30
     uint32_t shift;       // kCompatShift
     uint32_t buckets;     // kCompatBuckets
     uint32_t entries;
     uint32_t not_sure;    // angles whose rounding test failed (listed in the table when they differ)
     uint32_t libm_ulp2;   // angles where libm is two or more ulps away (must be 0)
-    char libm[32];        // gnu_get_libc_version()
+    double lookup_from;   // CompatTable::lookup_from
+    char libm[24];        // "glibc " + gnu_get_libc_version()
 };
 static_assert(sizeof(FileHeader) == 64, "64-byte header");
 
@@ -60,9 +61,9 @@ void ehss_trig_host_sincos(int64_t n, const double* x, double* s, double* c, int
 }
 
 // Host build of sincos_sobol on words q (table arrays may be null).
-void ehss_trig_host_sobol(int64_t n, const uint32_t* q, const uint32_t* offsets, const uint16_t* entries, double* s,
-                          double* c) {
-    const CompatTable t{offsets, entries};
+void ehss_trig_host_sobol(int64_t n, const uint32_t* q, const uint32_t* offsets, const uint16_t* entries,
+                          double lookup_from, double* s, double* c) {
+    const CompatTable t{offsets, entries, lookup_from};
     for (int64_t i = 0; i < n; ++i) {
         const double x = kTwoPi * ((double)q[i] * kScale);
         sincos_sobol(x, q[i], t, &s[i], &c[i]);
@@ -70,7 +71,8 @@ void ehss_trig_host_sobol(int64_t n, const uint32_t* q, const uint32_t* offsets,
 }
 
 // Writes the compatibility table to `path`.  stats (8 int64): entries, cos
-// differences, sin differences, not-sure angles, libm >= 2 ulps away, buckets.
+// differences, sin differences, not-sure angles, libm >= 2 ulps away, buckets,
+// angles the device looks up (near_midpoint for cos or sin).
 // Returns 0 on success, -1 on an I/O error, -2 if libm is ever two or more ulps
 // from the correctly rounded value (the table format cannot express that).
 int ehss_trig_compat_generate(const char* path, int threads, int64_t* stats) {
@@ -79,18 +81,20 @@ int ehss_trig_compat_generate(const char* path, int threads, int64_t* stats) {
     std::vector<std::vector<uint16_t>> out(chunks);
     std::vector<uint32_t> counts(kCompatBuckets, 0);
     int64_t ncos = 0, nsin = 0, unsure = 0, far = 0;
+    // Largest |lo| / ulp(hi) deficit below one half over the listed angles:
+    // every listed angle has |lo| / ulp(hi) >= 0.5 - gap (exact arithmetic).
+    double gap = 0.0;
     if (threads < 1) threads = 1;
-#pragma omp parallel for num_threads(threads) schedule(dynamic, 4) reduction(+ : ncos, nsin, unsure, far)
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 4) reduction(+ : ncos, nsin, unsure, far) \
+    reduction(max : gap)
     for (int ch = 0; ch < chunks; ++ch) {
-        std::vector<uint16_t>& v = out[ch];
         for (uint32_t b = (uint32_t)ch * kChunk; b < (uint32_t)(ch + 1) * kChunk; ++b) {
             for (uint32_t k = 0; k < (1u << kCompatShift); ++k) {
                 const uint32_t q = (b << kCompatShift) | k;
                 const double x = kTwoPi * ((double)q * kScale);
-                double s, c;
-                int ok;
-                sincos_cr(x, &s, &c, &ok);
-                unsure += !ok;
+                const SinCos v = sincos_dd(x);
+                const double s = v.s.hi, c = v.c.hi;
+                unsure += !(rounding_sure(v.s) && rounding_sure(v.c));
                 const double lc = libm_cos(x), ls = libm_sin(x);
                 uint32_t f = 0;
                 if (lc != c) {
@@ -105,8 +109,10 @@ int ehss_trig_compat_generate(const char* path, int threads, int64_t* stats) {
                     else ++far;
                     ++nsin;
                 }
+                if (f & 1u) gap = fmax(gap, 0.5 - fabs(v.c.lo) / ulp_of(v.c.hi));
+                if (f & 4u) gap = fmax(gap, 0.5 - fabs(v.s.lo) / ulp_of(v.s.hi));
                 if (f) {
-                    v.push_back((uint16_t)((k << 4) | f));
+                    out[ch].push_back((uint16_t)((k << 4) | f));
                     ++counts[b];
                 }
             }
@@ -115,6 +121,14 @@ int ehss_trig_compat_generate(const char* path, int threads, int64_t* stats) {
     std::vector<uint32_t> offsets(kCompatBuckets + 1, 0);
     for (uint32_t b = 0; b < kCompatBuckets; ++b) offsets[b + 1] = offsets[b] + counts[b];
     const uint32_t total = offsets[kCompatBuckets];
+    const double lookup_from = 0.5 - gap;
+    // How many angles the device looks up with this threshold.
+    int64_t looked = 0;
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(+ : looked)
+    for (int64_t q = 0; q < (1LL << 30); ++q) {
+        const SinCos v = sincos_dd(kTwoPi * ((double)q * kScale));
+        looked += near_midpoint(v.c, lookup_from) || near_midpoint(v.s, lookup_from);
+    }
     if (stats) {
         stats[0] = total;
         stats[1] = ncos;
@@ -122,17 +136,19 @@ int ehss_trig_compat_generate(const char* path, int threads, int64_t* stats) {
         stats[3] = unsure;
         stats[4] = far;
         stats[5] = kCompatBuckets;
+        stats[6] = looked;
     }
     if (far) return -2;
     FileHeader h;
     memset(&h, 0, sizeof h);
-    memcpy(h.magic, "EHSSTRG1", 8);
+    memcpy(h.magic, "EHSSTRG2", 8);
     h.key_bits = 30;
     h.shift = kCompatShift;
     h.buckets = kCompatBuckets;
     h.entries = total;
     h.not_sure = (uint32_t)unsure;
     h.libm_ulp2 = 0;
+    h.lookup_from = lookup_from;
     snprintf(h.libm, sizeof h.libm, "glibc %s", gnu_get_libc_version());
     FILE* f = fopen(path, "wb");
     if (!f) return -1;

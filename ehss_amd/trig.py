@@ -32,7 +32,8 @@ def host_lib() -> ct.CDLL:
             raise RuntimeError(f"{HOST_LIB} is missing: build it with `python -m ehss_amd.build`")
         L = ct.CDLL(str(HOST_LIB))
         L.ehss_trig_host_sincos.argtypes = [ct.c_int64] + [ct.c_void_p] * 4
-        L.ehss_trig_host_sobol.argtypes = [ct.c_int64] + [ct.c_void_p] * 5
+        L.ehss_trig_host_sobol.argtypes = [ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_double,
+                                           ct.c_void_p, ct.c_void_p]
         L.ehss_trig_libm_version.argtypes = [ct.c_char_p, ct.c_size_t]
         _lib = L
     return _lib
@@ -49,6 +50,7 @@ class CompatTable:
     shift: int
     libm: str
     not_sure: int
+    lookup_from: float  # look up only when |lo| >= lookup_from * ulp(hi) for cos or sin
 
     def words(self) -> tuple[np.ndarray, np.ndarray]:
         """Every listed word q and its flags (1 cos differs, 2 cos up, 4 sin differs, 8 sin up)."""
@@ -60,18 +62,19 @@ class CompatTable:
 
 def load_table(path: Path | str = TABLE) -> CompatTable:
     raw = Path(path).read_bytes()
-    if raw[:8] != b"EHSSTRG1":
+    if raw[:8] != b"EHSSTRG2":
         raise ValueError(f"{path}: not a trig compatibility table")
     key_bits, shift, buckets, entries, not_sure, far = np.frombuffer(raw[8:32], np.uint32)
-    if key_bits != 30 or far != 0:
+    lookup_from = float(np.frombuffer(raw[32:40], np.float64)[0])
+    if key_bits != 30 or far != 0 or not 0 < lookup_from <= 0.5:
         raise ValueError(f"{path}: unsupported table")
-    libm = raw[32:64].split(b"\0")[0].decode()
+    libm = raw[40:64].split(b"\0")[0].decode()
     off_end = HEADER_BYTES + 4 * (int(buckets) + 1)
     offsets = np.frombuffer(raw[HEADER_BYTES:off_end], np.uint32).copy()
     ent = np.frombuffer(raw[off_end:], np.uint16).copy()
     if len(ent) != entries or offsets[-1] != entries:
         raise ValueError(f"{path}: truncated table")
-    return CompatTable(offsets, ent, int(shift), libm, int(not_sure))
+    return CompatTable(offsets, ent, int(shift), libm, int(not_sure), lookup_from)
 
 
 def host_libm() -> str:
@@ -99,5 +102,6 @@ def sobol_sincos(q: np.ndarray, table: CompatTable | None) -> tuple[np.ndarray, 
     s, c = np.empty(len(q)), np.empty(len(q))
     off = None if table is None else table.offsets
     ent = None if table is None else table.entries
-    host_lib().ehss_trig_host_sobol(len(q), _p(q), _p(off), _p(ent), _p(s), _p(c))
+    start = 0.5 if table is None else table.lookup_from
+    host_lib().ehss_trig_host_sobol(len(q), _p(q), _p(off), _p(ent), start, _p(s), _p(c))
     return s, c

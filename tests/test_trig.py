@@ -58,6 +58,7 @@ def test_quadrant_words_give_tiny_exact_residues():
 
 def test_table_matches_host_libm_on_every_listed_word(table):
     assert table.libm == T.host_libm()
+    assert 0.45 < table.lookup_from < 0.5  # the libm errs only near rounding midpoints
     q, flags = table.words()
     assert len(q) > 0 and np.all(np.diff(q.astype(np.int64)) > 0)
     x = T.angles(q)
@@ -89,12 +90,12 @@ def test_planted_table_mistakes_fail(table):
     pick = np.arange(0, len(q), max(1, len(q) // 2000))
     x = T.angles(q[pick])
     # a flipped direction bit
-    bad = T.CompatTable(table.offsets, table.entries.copy(), table.shift, table.libm, 0)
+    bad = T.CompatTable(table.offsets, table.entries.copy(), table.shift, table.libm, 0, table.lookup_from)
     bad.entries[pick] ^= np.uint16(2 | 8)
     s, c = T.sobol_sincos(q[pick], bad)
     assert np.sum(c != np.cos(x)) + np.sum(s != np.sin(x)) > len(pick) // 2
     # dropped entries (flags cleared)
-    bad = T.CompatTable(table.offsets, table.entries.copy(), table.shift, table.libm, 0)
+    bad = T.CompatTable(table.offsets, table.entries.copy(), table.shift, table.libm, 0, table.lookup_from)
     bad.entries[pick] &= np.uint16(0xFFF0)
     s, c = T.sobol_sincos(q[pick], bad)
     assert np.sum(c != np.cos(x)) + np.sum(s != np.sin(x)) >= len(pick)
