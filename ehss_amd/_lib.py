@@ -29,6 +29,7 @@ FLAG_GRAPH = 4
 FLAG_TIMESTAMPS = 8
 FLAG_LDS_PACKETS = 16
 FLAG_PER_REPLICATE = 32
+FLAG_CONTINUABLE = 64
 VARIANT_SHIFT = 8
 COOP_SHIFT = 12
 GROUP_SHIFT = 19
@@ -125,6 +126,7 @@ _SIGS = {
     "ehss_source_rays": (_INT, [_P, ct.POINTER(Sobol), _INT, _P, _P, _P]),
     "ehss_trig_compat_info": (_INT, [_P, _P, ct.c_size_t, _P, ct.c_size_t]),
     "ehss_scramble_many": (_INT, [_I64, _P, _P]),
+    "ehss_solve_extend": (_INT, [_P, _INT, _P, _P, _P]),
     "ehss_solve_noalloc": (_INT, [_P, ct.POINTER(Sobol), _INT, _INT, _INT, ct.c_uint, _P, _P, _P, _P]),
     "ehss_sobol_sincos": (_INT, [_I64, _P, _P, _P]),
     "ehss_pa_grid": (_INT, [_I64, _P, _P, _INT, _P, _INT, _P, _P]),

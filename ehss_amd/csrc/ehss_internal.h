@@ -207,6 +207,12 @@ struct ehss_scene {
     ehss::DevBuf ray_info, ray_contrib, partials, hist, ledger, counters, queue, queue2, ray_times, tables,
         drain_stripes, custom_groups;
     ehss::BuildWork build_work;  // tree-build work arrays, reused by ehss_scene_rebuild
+    // continuation of the last solve (EHSS_FLAG_CONTINUABLE): two queues of
+    // saved unresolved rays (ping-pong) and their words {count, head} x 2
+    ehss::DevBuf cont_q[2], cont_words;
+    bool cont_valid = false;
+    int cont_which = 0, cont_cap = 0, cont_reps = 0, cont_power = 0;
+    unsigned cont_flags = 0;
     int64_t last_rays = 0, last_times = 0;
     uint32_t last_map[32] = {0};  // ray map of the last solve (slot -> ray index)
     int last_power = 0;

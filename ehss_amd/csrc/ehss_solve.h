@@ -60,6 +60,8 @@
 
 namespace ehss {
 
+struct RayState;
+
 struct SolveArgs {
     int64_t n;
     int32_t power;
@@ -75,6 +77,12 @@ struct SolveArgs {
     int32_t packets;               // variant 4: packet queries before queueing (1..15)
     int32_t pad_;
     ehss_trig::CompatTable trig;   // host-libm compatibility table of the Sobol angles (may be null)
+    // Continuation (EHSS_FLAG_CONTINUABLE): the state of every ray that ends
+    // unresolved is also appended here, for ehss_solve_extend (null: off).
+    RayState* cont;
+    uint32_t* cont_count;
+    int32_t rep;                   // replicate of these arguments (queue entries carry it)
+    int32_t pad2_;
 };
 
 // Words per replicate in the solve workspace: [0..5] kernel counters, [6] max
@@ -286,26 +294,8 @@ __device__ __forceinline__ void ray_process(const SceneView& S, int32_t cap, Ray
     }
 }
 
-// The ray's record (ehss_state.py:84-87): bounces and flags, w (1 - u.v) if escaped,
-// stored at the ray's slot.  Slots follow thread order, a fixed bijection with
-// ray indices (ehss_ray_map), so the fixed-order reduction over slots does not
-// depend on scheduling; host readback permutes to ray-index order.
-__device__ __forceinline__ void ray_record(const SolveArgs& A, RayCtx& R) {
-    double contrib = 0.;
-    if (R.escaped) {
-        R.flags |= EHSS_RAY_ESCAPED;
-        // clip(1 - sum(incoming * outgoing), 0, 2) * w
-        double angle = 1 - ((R.inc.x * R.d.x + R.inc.y * R.d.y) + R.inc.z * R.d.z);
-        angle = angle < 0. ? 0. : (angle > 2. ? 2. : angle);
-        contrib = A.w * angle;
-    }
-    if (R.count > 0 || (R.flags & EHSS_RAY_UNRESOLVED)) R.flags |= EHSS_RAY_EHSS_HIT;
-    A.info[R.i] = (uint32_t)R.count | (R.flags << kFlagShift);
-    A.contrib[R.i] = contrib;
-    if (A.times) A.times[R.i] = (long long)wall_clock64();
-}
-
-__device__ __forceinline__ void ray_save(const RayCtx& R, int32_t job, RayState* q) {
+// Queue entry of a ray (its state between queries).
+__device__ __forceinline__ void ray_save(const RayCtx& R, int32_t job, RayState* q, uint32_t flags) {
     RayState st;
     st.o[0] = R.o.x;
     st.o[1] = R.o.y;
@@ -319,12 +309,39 @@ __device__ __forceinline__ void ray_save(const RayCtx& R, int32_t job, RayState*
     st.ep = R.pe.ep;
     st.ed = R.pe.ed;
     st.i = R.i;
-    st.flags = R.flags;
+    st.flags = flags;
     st.last = R.last;
     st.count = R.count;
     st.job = job;
     st.pad = 0;
     *q = st;
+}
+
+__device__ __forceinline__ void ray_save(const RayCtx& R, int32_t job, RayState* q) { ray_save(R, job, q, R.flags); }
+
+// The ray's record (ehss_state.py:84-87): bounces and flags, w (1 - u.v) if escaped,
+// stored at the ray's slot.  Slots follow thread order, a fixed bijection with
+// ray indices (ehss_ray_map), so the fixed-order reduction over slots does not
+// depend on scheduling; host readback permutes to ray-index order.
+__device__ __forceinline__ void ray_record(const SolveArgs& A, RayCtx& R) {
+    // A continuable solve keeps the state of an unresolved ray as it stood
+    // before its last query (count == cap): continuing from there with a
+    // larger cap repeats that query and then follows the path a full retrace
+    // at the larger cap would follow, bit for bit.
+    if (A.cont && (R.flags & EHSS_RAY_UNRESOLVED))
+        ray_save(R, A.rep, A.cont + atomicAdd(A.cont_count, 1u), R.flags & ~(uint32_t)EHSS_RAY_UNRESOLVED);
+    double contrib = 0.;
+    if (R.escaped) {
+        R.flags |= EHSS_RAY_ESCAPED;
+        // clip(1 - sum(incoming * outgoing), 0, 2) * w
+        double angle = 1 - ((R.inc.x * R.d.x + R.inc.y * R.d.y) + R.inc.z * R.d.z);
+        angle = angle < 0. ? 0. : (angle > 2. ? 2. : angle);
+        contrib = A.w * angle;
+    }
+    if (R.count > 0 || (R.flags & EHSS_RAY_UNRESOLVED)) R.flags |= EHSS_RAY_EHSS_HIT;
+    A.info[R.i] = (uint32_t)R.count | (R.flags << kFlagShift);
+    A.contrib[R.i] = contrib;
+    if (A.times) A.times[R.i] = (long long)wall_clock64();
 }
 
 __device__ __forceinline__ int32_t ray_job(const RayState* q) { return q->job; }

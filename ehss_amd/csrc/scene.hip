@@ -110,6 +110,7 @@ int check_structure(int64_t n, const double* centers, const double* radii, const
 int build_into(ehss_scene* sc, int64_t n, const double* centers, const double* radii, const std::vector<int32_t>& cg,
                int32_t custom_ng, int leaf_size) {
     hipStream_t s = sc->stream;
+    sc->cont_valid = false;  // the saved rays belong to the previous structure
     sc->n = (int32_t)n;
     sc->leaf_size = leaf_size;
     enclosing_disk(n, centers, radii, sc->center, &sc->radius);

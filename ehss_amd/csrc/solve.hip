@@ -23,6 +23,7 @@ namespace {
 // histogram and ledger arrays (replicates are laid out one after another).
 __device__ __forceinline__ SolveArgs replicate_args(const SolveArgs& A, int32_t r) {
     SolveArgs B = A;
+    B.rep = r;
     B.info = A.info + (int64_t)r * A.n;
     B.contrib = A.contrib + (int64_t)r * A.n;
     B.counters = A.counters + (int64_t)r * kCounterWords;
@@ -203,6 +204,16 @@ __global__ void k_write_totals(const unsigned long long* c, const unsigned long 
     T.max_bounces = (int64_t)(int)c[6];
     T.kernel_ms = 0.;
     *totals = T;
+}
+
+// Continuation words {count, head} of queue `in` and `out`: the input keeps its
+// count and restarts its head, the output starts empty.
+__global__ void k_cont_reset(uint32_t* w, int in) {
+    if (threadIdx.x == 0) {
+        w[2 * in + 1] = 0u;
+        w[2 * (1 - in)] = 0u;
+        w[2 * (1 - in) + 1] = 0u;
+    }
 }
 
 __global__ void k_zero(unsigned long long* p, int64_t n) {
@@ -672,6 +683,19 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     A.packets = packets_from_flags(flags);
     A.pad_ = 0;
     A.trig = trig_compat();
+    A.cont = nullptr;
+    A.cont_count = nullptr;
+    A.rep = 0;
+    A.pad2_ = 0;
+    sc->cont_valid = false;  // this launch overwrites the records
+    if (flags & EHSS_FLAG_CONTINUABLE) {
+        EHSS_TRY(sc->cont_q[0].reserve(sizeof(RayState) * total));
+        EHSS_TRY(sc->cont_words.reserve(4 * sizeof(uint32_t)));
+        EHSS_TRY(launch_zero(sc->cont_words.as<unsigned long long>(), 2, s));
+        A.cont = sc->cont_q[0].as<RayState>();
+        A.cont_count = sc->cont_words.as<uint32_t>();
+        sc->cont_which = 0;
+    }
     A.times = nullptr;
     if (flags & EHSS_FLAG_TIMESTAMPS) {
         EHSS_TRY(sc->ray_times.reserve(sizeof(long long) * total));
@@ -786,6 +810,106 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     return EHSS_OK;
 }
 
+// Continues the scene's last continuable solve at new_cap (ehss_solve_extend):
+// the saved unresolved rays are drained with a whole wave per ray (they are
+// the long, trapped paths), rays unresolved again are saved into the other
+// queue, and every replicate's records are reduced again.  Enqueued like a
+// solve; solve_collect waits for it.
+int extend_enqueue(ehss_scene* sc, int new_cap) {
+    EHSS_TRY(ensure_init());
+    EHSS_CHECK(sc, "null scene");
+    if (sc->pending) return set_error(EHSS_ERR_BUSY, "a solve is already pending on this scene");
+    EHSS_CHECK(sc->cont_valid,
+               "no continuable solve on this scene (EHSS_FLAG_CONTINUABLE, and no other solve or rebuild since)");
+    EHSS_CHECK(new_cap > sc->cont_cap && new_cap <= (int)kBounceMask - 1, "new cap %d must be in %d..%u", new_cap,
+               sc->cont_cap + 1, kBounceMask - 1);
+    hipStream_t s = sc->stream;
+    const int reps = sc->cont_reps, power = sc->cont_power;
+    const int64_t n = 1LL << power;
+    const int64_t total = n * reps;
+    const int64_t nblocks = (n + kReduceRays - 1) / kReduceRays;
+    const int64_t bins = new_cap + 1;
+    const int64_t words = (int64_t)kCounterWords * reps + 2;
+    const int in = sc->cont_which, out = 1 - in;
+    EHSS_TRY(sc->cont_q[out].reserve(sizeof(RayState) * total));
+    EHSS_TRY(sc->hist.reserve(sizeof(unsigned long long) * bins * reps));
+    EHSS_TRY(sc->ledger.reserve(sizeof(unsigned long long) * bins * reps));
+    unsigned long long* ctr = sc->counters.as<unsigned long long>();
+    EHSS_TRY(launch_zero(ctr, words, s));
+    EHSS_TRY(launch_zero(sc->hist.as<unsigned long long>(), bins * reps, s));
+    EHSS_TRY(launch_zero(sc->ledger.as<unsigned long long>(), bins * reps, s));
+    uint32_t* cw = sc->cont_words.as<uint32_t>();
+    hipLaunchKernelGGL(k_cont_reset, dim3(1), dim3(64), 0, s, cw, in);
+
+    SolveArgs A;
+    A.n = n;
+    A.power = power;
+    A.cap = new_cap;
+    A.w = M_PI * sc->radius * sc->radius / (double)n;  // np.pi*radius*radius/n
+    A.cx = sc->center[0];
+    A.cy = sc->center[1];
+    A.cz = sc->center[2];
+    A.radius = sc->radius;
+    A.info = sc->ray_info.as<uint32_t>();
+    A.contrib = sc->ray_contrib.as<double>();
+    A.counters = ctr;
+    A.coop = coop_from_flags(sc->cont_flags);
+    A.group = kWave;
+    A.packets = packets_from_flags(sc->cont_flags);
+    A.pad_ = 0;
+    A.trig = trig_compat();
+    A.cont = sc->cont_q[out].as<RayState>();
+    A.cont_count = cw + 2 * out;
+    A.rep = 0;
+    A.pad2_ = 0;
+    A.times = nullptr;
+    unsigned long long* stripes = nullptr;
+    if (reps > 1) {
+        const int64_t sw = (int64_t)reps * kDrainStripes * 4;
+        EHSS_TRY(sc->drain_stripes.reserve(sizeof(unsigned long long) * sw));
+        stripes = sc->drain_stripes.as<unsigned long long>();
+        EHSS_TRY(launch_zero(stripes, sw, s));
+    }
+    EHSS_HIP(hipEventRecord(sc->ev0, s));
+    const size_t lds = drain_lds_bytes(sc->view);
+    const dim3 dgrid((unsigned)drain_blocks(total));
+    const RayState* q = sc->cont_q[in].as<RayState>();
+    if (lds <= kDrainLdsLimit) {
+        EHSS_TRY(set_drain_lds_limit());
+        hipLaunchKernelGGL(k_drain<1>, dgrid, dim3(256), lds, s, sc->view, A, reps, q, cw + 2 * in, cw + 2 * in + 1,
+                           kWave, 0, nullptr, nullptr, stripes);
+    } else {
+        hipLaunchKernelGGL(k_drain<0>, dgrid, dim3(256), 0, s, sc->view, A, reps, q, cw + 2 * in, cw + 2 * in + 1,
+                           kWave, 0, nullptr, nullptr, stripes);
+    }
+    EHSS_HIP(hipGetLastError());
+    EHSS_HIP(hipEventRecord(sc->ev1, s));
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)nblocks, (unsigned)reps), dim3(256), 0, s, A, nblocks,
+                       sc->partials.as<Partial>(), sc->hist.as<unsigned long long>(),
+                       sc->ledger.as<unsigned long long>());
+    hipLaunchKernelGGL(k_finalize, dim3((unsigned)reps), dim3(256), 0, s, sc->partials.as<Partial>(), nblocks, ctr,
+                       (const unsigned long long*)stripes);
+    EHSS_HIP(hipGetLastError());
+    const int64_t host_words = (int64_t)kCounterWords * reps + 2 * bins * reps;
+    EHSS_TRY(sc->host_words.reserve(sizeof(unsigned long long) * host_words));
+    unsigned long long* hw = sc->host_words.as<unsigned long long>();
+    EHSS_HIP(hipMemcpyAsync(hw, ctr, sizeof(unsigned long long) * kCounterWords * reps, hipMemcpyDeviceToHost, s));
+    EHSS_HIP(hipMemcpyAsync(hw + (int64_t)kCounterWords * reps, sc->hist.p, sizeof(unsigned long long) * bins * reps,
+                            hipMemcpyDeviceToHost, s));
+    EHSS_HIP(hipMemcpyAsync(hw + (int64_t)kCounterWords * reps + bins * reps, sc->ledger.p,
+                            sizeof(unsigned long long) * bins * reps, hipMemcpyDeviceToHost, s));
+    EHSS_HIP(hipEventRecord(sc->ev_done, s));
+    sc->pending = true;
+    sc->cont_which = out;
+    sc->p_power = power;
+    sc->p_reps = reps;
+    sc->p_rays = n;
+    sc->p_cap = new_cap;
+    sc->p_flags = sc->cont_flags;
+    sc->p_weight = A.w;
+    return EHSS_OK;
+}
+
 int solve_query(ehss_scene* sc, int* done) {
     EHSS_CHECK(sc && done, "null scene/done");
     if (!sc->pending) return set_error(EHSS_ERR_INVALID, "no solve pending on this scene");
@@ -847,6 +971,13 @@ int solve_collect(ehss_scene* sc, ehss_totals* totals, int64_t* histogram, doubl
         if (ledger_out)
             for (int64_t k = 0; k < bins; ++k) ledger_out[bins * r + k] = (double)h_led[k] / kLedgerScale * w;
     }
+    if (sc->p_flags & EHSS_FLAG_CONTINUABLE) {
+        sc->cont_valid = true;
+        sc->cont_cap = sc->p_cap;
+        sc->cont_reps = reps;
+        sc->cont_power = sc->p_power;
+        sc->cont_flags = sc->p_flags;
+    }
     sc->last_rays = (sc->p_flags & EHSS_FLAG_KEEP_RAYS) ? n : 0;
     sc->last_times = (sc->p_flags & EHSS_FLAG_TIMESTAMPS) ? n : 0;
     sc->last_weight = w;
@@ -906,8 +1037,8 @@ int ehss_solve_noalloc(ehss_scene* sc, const ehss_sobol* sobol, int power, int c
                EHSS_NOALLOC_MAX_CAP);
     EHSS_CHECK(precision == EHSS_PRECISION_FP64 || precision == EHSS_PRECISION_FP32_FIXUP, "unknown precision %d",
                precision);
-    EHSS_CHECK(!(flags & (EHSS_FLAG_KEEP_RAYS | EHSS_FLAG_TIMESTAMPS)),
-               "the allocation-free solve keeps no per-ray records (KEEP_RAYS / TIMESTAMPS)");
+    EHSS_CHECK(!(flags & (EHSS_FLAG_KEEP_RAYS | EHSS_FLAG_TIMESTAMPS | EHSS_FLAG_CONTINUABLE)),
+               "the allocation-free solve keeps no per-ray records (KEEP_RAYS / TIMESTAMPS / CONTINUABLE)");
     const int64_t n = 1LL << power;
     int variant = (int)((flags >> EHSS_VARIANT_SHIFT) & 0xFu);
     if (precision == EHSS_PRECISION_FP64) variant = 0;
@@ -969,6 +1100,10 @@ int ehss_solve_noalloc(ehss_scene* sc, const ehss_sobol* sobol, int power, int c
     A.packets = packets_from_flags(flags);
     A.pad_ = 0;
     A.trig = trig_compat();
+    A.cont = nullptr;
+    A.cont_count = nullptr;
+    A.rep = 0;
+    A.pad2_ = 0;
     A.times = nullptr;
     EHSS_CHECK(A.coop <= kWave, "tail threshold %d out of range 0..64", A.coop);
     const bool pa = (flags & EHSS_FLAG_PA) != 0;
@@ -1024,6 +1159,12 @@ int ehss_solve_many(ehss_scene* scene, int replicates, const ehss_sobol* sobols,
                     unsigned flags, ehss_totals* totals, int64_t* histograms, double* ledgers) {
     EHSS_CHECK(totals, "null totals");
     EHSS_TRY(solve_enqueue(scene, replicates, sobols, power, cap, precision, flags));
+    return solve_collect(scene, totals, histograms, ledgers);
+}
+
+int ehss_solve_extend(ehss_scene* scene, int new_cap, ehss_totals* totals, int64_t* histograms, double* ledgers) {
+    EHSS_CHECK(totals, "null totals");
+    EHSS_TRY(extend_enqueue(scene, new_cap));
     return solve_collect(scene, totals, histograms, ledgers);
 }
 

@@ -304,7 +304,7 @@ class GPUScene:
     def solve(self, sobol: L.Sobol, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
               keep_rays: bool = False, histogram: bool = False, variant: int = 0, coop: int | None = None,
               timestamps: bool = False, group: int | None = None, stage1: int | None = None,
-              lds: bool = False, packets: int | None = None) -> dict:
+              lds: bool = False, packets: int | None = None, continuable: bool = False) -> dict:
         """One fused device solve of 2^power rays.  `variant` selects the FP32
         filter kernel (0 default, 1 per-lane, 2 packet first query, 3 packet
         throughout, 4 packet first queries then queue + persistent drain);
@@ -318,7 +318,7 @@ class GPUScene:
         ledger = np.zeros(cap + 1, np.float64) if histogram else None
         flags = (L.FLAG_PA if pa else 0) | (L.FLAG_KEEP_RAYS if keep_rays else 0) | (int(variant) << L.VARIANT_SHIFT)
         flags |= coop_flags(coop, group, stage1, packets) | (L.FLAG_TIMESTAMPS if timestamps else 0)
-        flags |= L.FLAG_LDS_PACKETS if lds else 0
+        flags |= (L.FLAG_LDS_PACKETS if lds else 0) | (L.FLAG_CONTINUABLE if continuable else 0)
         L.call("ehss_solve", self.handle, ct.byref(sobol), int(power), int(cap), precision_code(precision),
                flags, ct.byref(totals), L.ptr(hist), L.ptr(ledger))
         out = totals.as_dict()
@@ -330,7 +330,8 @@ class GPUScene:
     def solve_many(self, sobols, power: int, cap: int = 64, precision="fp32-fixup", pa: bool = True,
                    histogram: bool = False, variant: int = 0, coop: int | None = None, group: int | None = None,
                    keep_rays: bool = False, stage1: int | None = None, lds: bool = False,
-                   per_replicate: bool = False, packets: int | None = None, as_array: bool = False):
+                   per_replicate: bool = False, packets: int | None = None, as_array: bool = False,
+                   continuable: bool = False):
         """len(sobols) independent solves in one launch; one result dict per
         scramble, each equal to solve() with that scramble.  per_replicate:
         never schedule the replicates across the lanes of a wave.  as_array: the
@@ -347,12 +348,23 @@ class GPUScene:
         flags = (L.FLAG_PA if pa else 0) | (L.FLAG_KEEP_RAYS if keep_rays else 0) | \
             (int(variant) << L.VARIANT_SHIFT) | coop_flags(coop, group, stage1, packets) | \
             (L.FLAG_LDS_PACKETS if lds else 0)
-        flags |= L.FLAG_PER_REPLICATE if per_replicate else 0
+        flags |= (L.FLAG_PER_REPLICATE if per_replicate else 0) | (L.FLAG_CONTINUABLE if continuable else 0)
         L.call("ehss_solve_many", self.handle, reps, table, int(power), int(cap), precision_code(precision), flags,
                totals, L.ptr(hist), L.ptr(ledger))
         if as_array:
             return np.frombuffer(totals, dtype=L.TOTALS_DTYPE, count=reps).copy()
         return self._split(totals, reps, cap, hist, ledger)
+
+    def extend(self, new_cap: int, replicates: int = 1, histogram: bool = False) -> list[dict]:
+        """Continues the last solve (made with continuable=True, or a previous
+        extend) at a larger cap: only the rays that ended unresolved are traced
+        further.  One result per replicate of that solve, bit-identical to
+        solving the same rays again at new_cap (work counters aside)."""
+        totals = (L.Totals * replicates)()
+        hist = np.zeros(replicates * (new_cap + 1), np.int64) if histogram else None
+        ledger = np.zeros(replicates * (new_cap + 1), np.float64) if histogram else None
+        L.call("ehss_solve_extend", self.handle, int(new_cap), totals, L.ptr(hist), L.ptr(ledger))
+        return self._split(totals, replicates, new_cap, hist, ledger)
 
     @staticmethod
     def _split(totals, reps, cap, hist, ledger) -> list[dict]:

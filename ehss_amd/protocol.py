@@ -4,14 +4,14 @@
   s + 20014 (mod 2^32), each of 2^power points, pooled with equal weights, so
   the pooled estimate is the mean of the three block estimates
   (manuscript/source/supporting_information.md:21, manuscript.md:33).
-* The blocks are traced at collision cap 512 and retraced at 1,024, 2,048 and
+* The blocks are traced at collision cap 512 and extended to 1,024, 2,048 and
   4,096 while the sampled unresolved-path bound exceeds 0.05 % of the computed
-  CCS; every attempt is kept (manuscript.md:61).  Retracing uses the same rays,
-  and the trace is deterministic, so every path that resolved below the old
-  cap is unchanged and the unresolved ones continue: the result equals an exact
-  continuation from the saved state.
+  CCS; every attempt is kept (manuscript.md:61).  The reference retraces the
+  same rays at each cap.  Here only the unresolved rays are traced further,
+  from their saved state (ehss_solve_extend): the trace is deterministic, so
+  the result is bit-identical to the retrace.
 
-The three blocks of an attempt run as one multi-replicate launch.
+The three blocks run as one multi-replicate launch, extended together.
 """
 from __future__ import annotations
 
@@ -66,8 +66,11 @@ def publication_estimate(scene, seed: int, power: int, caps=CAPS, tail_fraction:
     sobols = [scramble(s) for s in block_seeds(seed)]
     attempts = []
     accepted = False
-    for cap in caps:
-        res = scene.solve_many(sobols, power, cap, precision=precision, pa=pa)
+    for k, cap in enumerate(caps):
+        if k == 0:
+            res = scene.solve_many(sobols, power, cap, precision=precision, pa=pa, continuable=True)
+        else:
+            res = scene.extend(cap, replicates=len(sobols))
         om = [r["omega"] for r in res]
         att = Attempt(cap=cap, omega=float(np.mean(om)), pa=float(np.mean([r["pa"] for r in res])),
                       tail_bound=float(np.mean([r["tail_bound"] for r in res])),

@@ -55,6 +55,8 @@ extern "C" {
 #define EHSS_FLAG_LDS_PACKETS 16u /* packet kernels stage the scene in LDS first (measured: not faster than
                                      scalar loads through the constant cache; kept for measurement) */
 #define EHSS_FLAG_PER_REPLICATE 32u /* solve_many: never schedule replicates across the lanes of a wave */
+#define EHSS_FLAG_CONTINUABLE 64u /* keep the records and the state of every ray that ends unresolved, so that
+                                     ehss_solve_extend can continue exactly those rays at a larger cap */
 /* Kernel variant of the FP32 filter path in flag bits 8..11 (0 = default):
  * 1 one ray per lane, 2 wave-uniform first query then per-lane bounces,
  * 3 wave-uniform traversal for every bounce, 4 wave-uniform first queries, then
@@ -264,6 +266,19 @@ int ehss_solve(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, i
 #define EHSS_NOALLOC_SCRATCH_MB 9
 int ehss_solve_noalloc(ehss_scene* scene, const ehss_sobol* sobol, int power, int cap, int precision, unsigned flags,
                        void* stream, ehss_totals* totals, int64_t* histogram, double* ledger);
+
+/* Continues the scene's last solve (ehss_solve / ehss_solve_many with
+ * EHSS_FLAG_CONTINUABLE, or a previous extension) at a larger collision cap:
+ * only the rays that ended unresolved are traced further, from their saved
+ * state, and every replicate is then reduced again in its fixed order.  The
+ * results -- totals, cap + 1 histogram and ledger bins per replicate -- are
+ * bit-identical to a full retrace of the same rays at new_cap (the work
+ * counters are those of the continuation).  This is the cap ladder of the
+ * publication protocol (manuscript.md:61: the same rays retraced at 1,024,
+ * 2,048 and 4,096) without retracing the resolved paths.  Synchronous;
+ * new_cap > the previous cap.  Invalidated by any other solve on the scene or a
+ * rebuild. */
+int ehss_solve_extend(ehss_scene* scene, int new_cap, ehss_totals* totals, int64_t* histograms, double* ledgers);
 
 /* `replicates` independent solves of the scene in one launch (one scramble each,
  * the replicates of ccs_cli.py:186-227): totals gets one entry per replicate,

@@ -13,10 +13,20 @@ class FakeScene:
         self.tail_at_512 = tail_at_512
         self.calls = []
 
-    def solve_many(self, sobols, power, cap, precision="fp32-fixup", pa=True):
+    def solve_many(self, sobols, power, cap, precision="fp32-fixup", pa=True, continuable=False):
+        assert continuable  # the protocol extends its first attempt
         self.calls.append((tuple(sobols), power, cap))
+        self.sobols = sobols
+        return self._results(cap)
+
+    def extend(self, new_cap, replicates=1):
+        assert replicates == len(self.sobols) and new_cap > self.calls[-1][2]
+        self.calls.append((tuple(self.sobols), self.calls[-1][1], new_cap))
+        return self._results(new_cap)
+
+    def _results(self, cap):
         tail = self.tail_at_512 * 512 / cap
-        return [dict(omega=1000.0 + s % 7, pa=900.0, tail_bound=tail, unresolved=int(tail * 10)) for s in sobols]
+        return [dict(omega=1000.0 + s % 7, pa=900.0, tail_bound=tail, unresolved=int(tail * 10)) for s in self.sobols]
 
 
 def test_block_seeds_stride_and_wrap():
@@ -29,7 +39,7 @@ def test_cap_extension_stops_when_bound_is_small():
     res = protocol.publication_estimate(sc, 11, 12, scramble=lambda s: s)
     assert [a.cap for a in res.attempts] == [512, 1024, 2048]
     assert res.accepted and res.final.tail_bound <= 5e-4 * res.final.omega
-    assert all(c[0] == (11, 10018, 20025) and c[1] == 12 for c in sc.calls)  # same rays retraced
+    assert all(c[0] == (11, 10018, 20025) and c[1] == 12 for c in sc.calls)  # same rays, extended
     assert res.omega == pytest.approx(np.mean([1000 + 11 % 7, 1000 + 10018 % 7, 1000 + 20025 % 7]))
 
 
