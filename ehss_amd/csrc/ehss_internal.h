@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include <atomic>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -191,6 +192,10 @@ struct ehss_scene {
     ehss::SceneView view;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;
+    // Serialises the entry points that use this scene's buffers: calls on one
+    // scene from several threads wait for each other instead of racing (a
+    // rebuild may reallocate arrays another call's kernels still read).
+    mutable std::mutex lock;
     // prepared plans holding this scene's views (ehss_plan_create counts one per
     // job): rebuild and destroy are refused while nonzero
     std::atomic<int> plans{0};

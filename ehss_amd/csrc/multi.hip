@@ -199,6 +199,7 @@ int create_plan(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols,
     int64_t rays = 0;
     for (int j = 0; j < n_jobs; ++j) {
         EHSS_CHECK(scenes[j], "null scene for job %d", j);
+        std::lock_guard<std::mutex> guard(scenes[j]->lock);
         EHSS_CHECK(!scenes[j]->pending, "a solve is pending on the scene of job %d", j);
         EHSS_CHECK(powers[j] >= 0 && powers[j] <= 26, "power of job %d out of range 0..26", j);
         EHSS_CHECK(caps[j] >= 1 && caps[j] <= (int)kBounceMask - 1, "cap of job %d out of range", j);
@@ -255,6 +256,7 @@ int create_plan(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols,
     p->jobs.resize(n_jobs);
     for (int j = 0; j < n_jobs; ++j) {
         ehss_scene* sc = scenes[j];
+        std::lock_guard<std::mutex> guard(sc->lock);
         PlanJob& J = p->jobs[j];
         memset(&J, 0, sizeof J);
         J.view = sc->view;

@@ -239,6 +239,7 @@ int ehss_scene_create_grouped(int64_t n, const double* centers, const double* ra
 int ehss_scene_rebuild(ehss_scene* scene, int64_t n, const double* centers, const double* radii, int leaf_size) {
     EHSS_TRY(ensure_init());  // the calling thread (a serving worker) may not have selected the device
     EHSS_CHECK(scene, "null scene");
+    std::lock_guard<std::mutex> guard(scene->lock);
     if (scene->pending) return set_error(EHSS_ERR_BUSY, "a solve is pending on this scene");
     if (const int held = scene->plans.load())
         return set_error(EHSS_ERR_BUSY, "the scene is held by %d plan job(s); destroy the plan(s) first", held);
@@ -251,8 +252,11 @@ int ehss_scene_rebuild(ehss_scene* scene, int64_t n, const double* centers, cons
 
 int ehss_scene_destroy(ehss_scene* scene) {
     if (!scene) return EHSS_OK;
-    if (const int held = scene->plans.load())
-        return set_error(EHSS_ERR_BUSY, "the scene is held by %d plan job(s); destroy the plan(s) first", held);
+    {
+        std::lock_guard<std::mutex> guard(scene->lock);  // waits for a call in progress
+        if (const int held = scene->plans.load())
+            return set_error(EHSS_ERR_BUSY, "the scene is held by %d plan job(s); destroy the plan(s) first", held);
+    }
     if (scene->stream) (void)hipStreamSynchronize(scene->stream);
     if (scene->ev0) (void)hipEventDestroy(scene->ev0);
     if (scene->ev1) (void)hipEventDestroy(scene->ev1);
@@ -264,6 +268,7 @@ int ehss_scene_destroy(ehss_scene* scene) {
 
 int ehss_scene_get_info(const ehss_scene* scene, ehss_scene_info* info) {
     EHSS_CHECK(scene && info, "null scene/info");
+    std::lock_guard<std::mutex> guard(scene->lock);
     info->atoms = scene->n;
     info->groups = scene->ng;
     info->nodes = scene->nn;
@@ -280,6 +285,7 @@ int ehss_scene_get_info(const ehss_scene* scene, ehss_scene_info* info) {
 int ehss_scene_export_tree(const ehss_scene* scene, double* lo, double* hi, int64_t* end, int64_t* leaf_group,
                            int64_t* offsets, int64_t* ids, int64_t* groups) {
     EHSS_CHECK(scene, "null scene");
+    std::lock_guard<std::mutex> guard(scene->lock);
     hipStream_t s = scene->stream;
     std::vector<Node64> nodes((size_t)scene->nn);
     EHSS_HIP(hipMemcpyAsync(nodes.data(), scene->nodes.p, sizeof(Node64) * scene->nn, hipMemcpyDeviceToHost, s));
@@ -301,6 +307,7 @@ int ehss_scene_export_tree(const ehss_scene* scene, double* lo, double* hi, int6
 int ehss_scene_export_pa_tree(const ehss_scene* scene, double* lo, double* hi, int64_t* end, int64_t* first,
                               int64_t* count, int64_t* ids) {
     EHSS_CHECK(scene, "null scene");
+    std::lock_guard<std::mutex> guard(scene->lock);
     hipStream_t s = scene->stream;
     std::vector<NodePA> nodes((size_t)scene->pa_nn);
     EHSS_HIP(hipMemcpyAsync(nodes.data(), scene->pa.p, sizeof(NodePA) * scene->pa_nn, hipMemcpyDeviceToHost, s));
@@ -323,6 +330,7 @@ int ehss_scene_trace(ehss_scene* scene, int64_t nr, int cap, int precision, int 
                      int32_t* history, int64_t* counters) {
     EHSS_TRY(ensure_init());
     EHSS_CHECK(scene, "null scene");
+    std::lock_guard<std::mutex> guard(scene->lock);
     EHSS_CHECK(nr == 0 || (p && d && lasts && counts && escaped && unresolved), "null ray buffer");
     for (int64_t r = 0; r < nr; ++r)
         EHSS_CHECK(lasts[r] < scene->n, "Invalid initial physical identity (ray %lld)", (long long)r);
@@ -358,6 +366,7 @@ int ehss_scene_pa_query(ehss_scene* scene, int64_t nr, int precision, const doub
                         const double* directions, const int64_t* owners, uint8_t* hits, int64_t* counters) {
     EHSS_TRY(ensure_init());
     EHSS_CHECK(scene, "null scene");
+    std::lock_guard<std::mutex> guard(scene->lock);
     EHSS_CHECK(nr >= 0 && (nr == 0 || (origins && directions && hits)), "null ray buffer");
     EHSS_CHECK(precision == EHSS_PRECISION_FP64 || precision == EHSS_PRECISION_FP32_FIXUP, "unknown precision");
     hipStream_t s = scene->stream;

@@ -606,6 +606,7 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
                   unsigned flags) {
     EHSS_TRY(ensure_init());  // the calling thread may not have selected the device
     EHSS_CHECK(sc && sobs, "null scene/sobol");
+    std::lock_guard<std::mutex> guard(sc->lock);
     if (sc->pending) return set_error(EHSS_ERR_BUSY, "a solve is already pending on this scene");
     EHSS_CHECK(reps >= 1 && reps <= kMaxReplicates, "replicate count %d out of range 1..%d", reps, kMaxReplicates);
     EHSS_CHECK(power >= 0 && power <= 30, "power %d out of range 0..30", power);
@@ -818,6 +819,7 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
 int extend_enqueue(ehss_scene* sc, int new_cap) {
     EHSS_TRY(ensure_init());
     EHSS_CHECK(sc, "null scene");
+    std::lock_guard<std::mutex> guard(sc->lock);
     if (sc->pending) return set_error(EHSS_ERR_BUSY, "a solve is already pending on this scene");
     EHSS_CHECK(sc->cont_valid,
                "no continuable solve on this scene (EHSS_FLAG_CONTINUABLE, and no other solve or rebuild since)");
@@ -912,6 +914,7 @@ int extend_enqueue(ehss_scene* sc, int new_cap) {
 
 int solve_query(ehss_scene* sc, int* done) {
     EHSS_CHECK(sc && done, "null scene/done");
+    std::lock_guard<std::mutex> guard(sc->lock);
     if (!sc->pending) return set_error(EHSS_ERR_INVALID, "no solve pending on this scene");
     const hipError_t e = hipEventQuery(sc->ev_done);
     if (e == hipErrorNotReady) {
@@ -927,6 +930,7 @@ int solve_query(ehss_scene* sc, int* done) {
 // not NULL, cap + 1 histogram and ledger bins per replicate, replicate-major).
 int solve_collect(ehss_scene* sc, ehss_totals* totals, int64_t* histogram, double* ledger_out) {
     EHSS_CHECK(sc && totals, "null scene/totals");
+    std::lock_guard<std::mutex> guard(sc->lock);
     if (!sc->pending) return set_error(EHSS_ERR_INVALID, "no solve pending on this scene");
     sc->pending = false;
     EHSS_HIP(hipEventSynchronize(sc->ev_done));
@@ -1031,6 +1035,7 @@ int ehss_solve_noalloc(ehss_scene* sc, const ehss_sobol* sobol, int power, int c
                        void* stream, ehss_totals* totals, int64_t* histogram, double* ledger) {
     EHSS_TRY(ensure_init());  // the calling thread may not have selected the device
     EHSS_CHECK(sc && sobol && totals, "null scene/sobol/totals");
+    std::lock_guard<std::mutex> guard(sc->lock);
     EHSS_CHECK(power >= 0 && power <= EHSS_NOALLOC_MAX_POWER, "power %d out of range 0..%d of the allocation-free solve",
                power, EHSS_NOALLOC_MAX_POWER);
     EHSS_CHECK(cap >= 1 && cap <= EHSS_NOALLOC_MAX_CAP, "cap %d out of range 1..%d of the allocation-free solve", cap,
@@ -1185,6 +1190,7 @@ int ehss_solve_wait(ehss_scene* scene, ehss_totals* totals, int64_t* histogram, 
 
 int ehss_scene_last_ray_times(ehss_scene* scene, int64_t n, int64_t* ticks, double* tick_hz) {
     EHSS_CHECK(scene && ticks, "null scene/ticks");
+    std::lock_guard<std::mutex> guard(scene->lock);
     if (scene->pending) return set_error(EHSS_ERR_BUSY, "a solve is pending on this scene");
     EHSS_CHECK(scene->last_times > 0, "no time stamps kept: solve with EHSS_FLAG_TIMESTAMPS first");
     EHSS_CHECK(n == scene->last_times, "record count %lld != %lld rays of the last solve", (long long)n,
@@ -1205,6 +1211,7 @@ int ehss_scene_last_ray_times(ehss_scene* scene, int64_t n, int64_t* ticks, doub
 
 int ehss_scene_last_rays(ehss_scene* scene, int64_t n, uint32_t* flags, double* contrib, int32_t* bounces) {
     EHSS_CHECK(scene, "null scene");
+    std::lock_guard<std::mutex> guard(scene->lock);
     if (scene->pending) return set_error(EHSS_ERR_BUSY, "a solve is pending on this scene");
     EHSS_CHECK(scene->last_rays > 0, "no per-ray records kept: solve with EHSS_FLAG_KEEP_RAYS first");
     EHSS_CHECK(n == scene->last_rays, "record count %lld != %lld rays of the last solve", (long long)n,
@@ -1230,6 +1237,7 @@ int ehss_source_rays(ehss_scene* scene, const ehss_sobol* sobol, int power, doub
     EHSS_TRY(ensure_init());
     EHSS_CHECK(scene && sobol, "null scene/sobol");
     EHSS_CHECK(power >= 0 && power <= 26, "power %d out of range 0..26", power);
+    std::lock_guard<std::mutex> guard(scene->lock);
     hipStream_t s = scene->stream;
     const int64_t n = 1LL << power;
     DevBuf bu, bo, bd;
