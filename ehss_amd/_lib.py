@@ -133,6 +133,7 @@ _SIGS = {
     "ehss_pa_grid_count": (_INT, [_I64, _P, _P, _P, ct.c_double, _INT, _P]),
     "ehss_plan_create": (_INT, [_INT, _P, _P, _P, _P, _INT, ct.c_uint, ct.POINTER(_P)]),
     "ehss_plan_set_sobol": (_INT, [_P, _INT, ct.POINTER(Sobol)]),
+    "ehss_plan_retarget": (_INT, [_P, _INT, _P, _P, _P, _P]),
     "ehss_plan_launch": (_INT, [_P]),
     "ehss_plan_query": (_INT, [_P, ct.POINTER(_INT)]),
     "ehss_plan_wait": (_INT, [_P, _P, _P, _P]),

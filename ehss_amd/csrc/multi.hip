@@ -109,6 +109,7 @@ using namespace ehss;
 
 struct ehss_plan {
     int n_jobs = 0, precision = 0, variant = 0, coop = 0, group = 1, stage1 = 0;
+    unsigned flags = 0;
     bool pa = false, use_graph = false, captured = false, dirty = true;
     bool holds = false;  // the scenes count this plan (ehss_scene::plans)
     std::vector<PlanJob> jobs;
@@ -185,17 +186,14 @@ int enqueue_plan(ehss_plan* p) {
     return EHSS_OK;
 }
 
-int create_plan(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols, const int32_t* powers,
-                const int32_t* caps, int precision, unsigned flags, ehss_plan** out) {
-    EHSS_TRY(ensure_init());
-    EHSS_CHECK(out, "null output handle");
-    *out = nullptr;
-    EHSS_CHECK(n_jobs >= 1 && n_jobs <= 65535 && scenes && sobols && powers && caps, "invalid plan description");
-    EHSS_CHECK(precision == EHSS_PRECISION_FP64 || precision == EHSS_PRECISION_FP32_FIXUP, "unknown precision");
-    int variant = (int)((flags >> EHSS_VARIANT_SHIFT) & 0xFu);
-    if (precision == EHSS_PRECISION_FP64) variant = 0;
-    else if (variant == 0) variant = 4;
-    EHSS_CHECK(variant >= 0 && variant <= 4, "unknown kernel variant %d", variant);
+// Points the plan at a job list: validates it, lays out the records, queues,
+// partials, words and bins (grow-only buffers), fills the host job table and
+// moves the plan's holds from its previous scenes to these.  n_jobs == 0
+// empties the plan (holds released; launching is refused until it is filled).
+int fill_plan(ehss_plan* p, int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols, const int32_t* powers,
+              const int32_t* caps) {
+    EHSS_CHECK(n_jobs >= 0 && n_jobs <= 65535 && (n_jobs == 0 || (scenes && sobols && powers && caps)),
+               "invalid plan description");
     int64_t rays = 0;
     for (int j = 0; j < n_jobs; ++j) {
         EHSS_CHECK(scenes[j], "null scene for job %d", j);
@@ -206,22 +204,20 @@ int create_plan(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols,
         rays += 1LL << powers[j];
     }
     EHSS_CHECK(rays <= (1LL << 30), "%lld rays exceed 2^30 per plan", (long long)rays);
-    ehss_plan* p = new ehss_plan();
-    struct Guard {
-        ehss_plan*& q;
-        ~Guard() { destroy_plan(q); }
-    } guard{p};
+    // The previous job list's holds go first (the same scene may appear again).
+    if (p->holds)
+        for (ehss_scene* sc : p->scenes) sc->plans.fetch_sub(1);
+    p->holds = false;
+    p->scenes.assign(scenes, scenes + n_jobs);
+    p->jobs.clear();
     p->n_jobs = n_jobs;
-    p->precision = precision;
-    p->variant = variant;
-    p->pa = (flags & EHSS_FLAG_PA) != 0;
-    p->use_graph = (flags & EHSS_FLAG_GRAPH) != 0;
-    p->coop = coop_from_flags(flags);
+    p->rays = rays;
+    p->dirty = true;
+    p->max_blocks = p->max_red = p->n_bins = p->n_words = 0;
+    if (n_jobs == 0) return EHSS_OK;
+    const unsigned flags = p->flags;
     p->group = group_from_flags(flags, rays);
     p->stage1 = stage1_queries(flags, rays);
-    EHSS_CHECK(p->coop <= kWave, "tail threshold %d out of range 0..64", p->coop);
-    p->rays = rays;
-    p->scenes.assign(scenes, scenes + n_jobs);
     std::vector<int64_t> ray_off(n_jobs), red_off(n_jobs), bin_off(n_jobs);
     int64_t red = 0, bins = 0, off = 0;
     for (int j = 0; j < n_jobs; ++j) {
@@ -239,18 +235,16 @@ int create_plan(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols,
     EHSS_CHECK(p->max_blocks <= (1LL << 31) - 1, "too many blocks per job");
     p->n_bins = bins;
     p->n_words = (int64_t)kPlanWords * n_jobs;
-    EHSS_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
-    EHSS_HIP(hipEventCreate(&p->ev0));
-    EHSS_HIP(hipEventCreate(&p->ev1));
-    EHSS_TRY(p->info.alloc(sizeof(uint32_t) * rays));
-    EHSS_TRY(p->contrib.alloc(sizeof(double) * rays));
-    EHSS_TRY(p->partials.alloc(sizeof(Partial) * red));
-    EHSS_TRY(p->words.alloc(sizeof(unsigned long long) * p->n_words));
-    EHSS_TRY(p->bins.alloc(sizeof(unsigned long long) * 2 * bins));
-    if (variant == 4) {
-        EHSS_TRY(p->queue.alloc(sizeof(RayState) * rays));
-        if (p->stage1 > 0 && p->group > 1) EHSS_TRY(p->queue2.alloc(sizeof(RayState) * rays));
+    EHSS_TRY(p->info.reserve(sizeof(uint32_t) * rays));
+    EHSS_TRY(p->contrib.reserve(sizeof(double) * rays));
+    EHSS_TRY(p->partials.reserve(sizeof(Partial) * red));
+    EHSS_TRY(p->words.reserve(sizeof(unsigned long long) * p->n_words));
+    EHSS_TRY(p->bins.reserve(sizeof(unsigned long long) * 2 * bins));
+    if (p->variant == 4) {
+        EHSS_TRY(p->queue.reserve(sizeof(RayState) * rays));
+        if (p->stage1 > 0 && p->group > 1) EHSS_TRY(p->queue2.reserve(sizeof(RayState) * rays));
     }
+    EHSS_TRY(p->jobs_dev.reserve(sizeof(PlanJob) * n_jobs));
     unsigned long long* W = p->words.as<unsigned long long>();
     unsigned long long* H = p->bins.as<unsigned long long>();
     p->jobs.resize(n_jobs);
@@ -284,34 +278,77 @@ int create_plan(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols,
         J.hist = H + bin_off[j];
         J.ledger = H + bins + bin_off[j];
         J.qwords = reinterpret_cast<uint32_t*>(W + (int64_t)kPlanWords * j + kCounterWords);
-        if (variant == 4) {
+        if (p->variant == 4) {
             J.queue = p->queue.as<RayState>() + ray_off[j];
-            if (p->queue2.p) J.spill = p->queue2.as<RayState>() + ray_off[j];
+            if (p->stage1 > 0 && p->group > 1) J.spill = p->queue2.as<RayState>() + ray_off[j];
         }
     }
-    EHSS_TRY(p->jobs_dev.alloc(sizeof(PlanJob) * n_jobs));
-    // Drain blocks per job: as for a single solve of the largest job.
     int64_t most_rays = 0;
     for (int j = 0; j < n_jobs; ++j) most_rays = std::max<int64_t>(most_rays, 1LL << powers[j]);
     p->drain_grid = drain_blocks(most_rays);
-    // LDS-staged scenes for the packet kernels, when every job's scene fits
-    // (as for a single solve: the flag has no effect otherwise).
-    if ((flags & EHSS_FLAG_LDS_PACKETS) && variant >= 2) {
+    p->lds_bytes = 0;
+    if ((flags & EHSS_FLAG_LDS_PACKETS) && p->variant >= 2) {
         size_t most = 0;
         for (int j = 0; j < n_jobs; ++j) most = std::max(most, staged_scene_bytes(scenes[j]->view));
         if (most <= kPlanLdsLimit) p->lds_bytes = most;
     }
     // The plan copied each scene's views (device pointers, counts, disk):
-    // the scenes refuse rebuild and destroy until the plan is destroyed.
+    // the scenes refuse rebuild and destroy until the plan lets them go.
     for (ehss_scene* sc : p->scenes) sc->plans.fetch_add(1);
     p->holds = true;
+    return EHSS_OK;
+}
+
+int create_plan(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols, const int32_t* powers,
+                const int32_t* caps, int precision, unsigned flags, ehss_plan** out) {
+    EHSS_TRY(ensure_init());
+    EHSS_CHECK(out, "null output handle");
+    *out = nullptr;
+    EHSS_CHECK(n_jobs >= 1, "a plan needs at least one job");
+    EHSS_CHECK(precision == EHSS_PRECISION_FP64 || precision == EHSS_PRECISION_FP32_FIXUP, "unknown precision");
+    int variant = (int)((flags >> EHSS_VARIANT_SHIFT) & 0xFu);
+    if (precision == EHSS_PRECISION_FP64) variant = 0;
+    else if (variant == 0) variant = 4;
+    EHSS_CHECK(variant >= 0 && variant <= 4, "unknown kernel variant %d", variant);
+    EHSS_CHECK(coop_from_flags(flags) <= kWave, "tail threshold %d out of range 0..64", coop_from_flags(flags));
+    ehss_plan* p = new ehss_plan();
+    struct Guard {
+        ehss_plan*& q;
+        ~Guard() { destroy_plan(q); }
+    } guard{p};
+    p->flags = flags;
+    p->precision = precision;
+    p->variant = variant;
+    p->pa = (flags & EHSS_FLAG_PA) != 0;
+    p->use_graph = (flags & EHSS_FLAG_GRAPH) != 0;
+    p->coop = coop_from_flags(flags);
+    EHSS_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    EHSS_HIP(hipEventCreate(&p->ev0));
+    EHSS_HIP(hipEventCreate(&p->ev1));
+    EHSS_TRY(fill_plan(p, n_jobs, scenes, sobols, powers, caps));
     *out = p;
     p = nullptr;
     return EHSS_OK;
 }
 
+// Re-points a plan at another job list, reusing its stream and (grow-only)
+// buffers; waits for the plan's previous launch, drops a captured graph (the
+// next launch captures the new sequence).
+int retarget_plan(ehss_plan* p, int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols,
+                  const int32_t* powers, const int32_t* caps) {
+    EHSS_CHECK(p, "null plan");
+    EHSS_HIP(hipStreamSynchronize(p->stream));
+    if (p->exec) EHSS_HIP(hipGraphExecDestroy(p->exec));
+    if (p->graph) EHSS_HIP(hipGraphDestroy(p->graph));
+    p->exec = nullptr;
+    p->graph = nullptr;
+    p->captured = false;
+    return fill_plan(p, n_jobs, scenes, sobols, powers, caps);
+}
+
 int launch_plan(ehss_plan* p) {
     EHSS_CHECK(p, "null plan");
+    EHSS_CHECK(p->n_jobs > 0 && p->holds, "the plan has no jobs (retarget it first)");
     hipStream_t s = p->stream;
     if (p->dirty) {
         EHSS_HIP(hipMemcpyAsync(p->jobs_dev.p, p->jobs.data(), sizeof(PlanJob) * p->n_jobs, hipMemcpyHostToDevice,
@@ -396,6 +433,12 @@ extern "C" {
 int ehss_plan_create(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols, const int32_t* powers,
                      const int32_t* caps, int precision, unsigned flags, ehss_plan** out) {
     return create_plan(n_jobs, scenes, sobols, powers, caps, precision, flags, out);
+}
+
+int ehss_plan_retarget(ehss_plan* plan, int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols,
+                       const int32_t* powers, const int32_t* caps) {
+    EHSS_TRY(ensure_init());
+    return retarget_plan(plan, n_jobs, scenes, sobols, powers, caps);
 }
 
 int ehss_plan_set_sobol(ehss_plan* plan, int job, const ehss_sobol* sobol) {

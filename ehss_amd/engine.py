@@ -472,6 +472,27 @@ class _MultiSolve:
     def set_sobol(self, job: int, sobol: L.Sobol) -> None:
         L.call(f"ehss_{self._prefix}_set_sobol", self.handle, int(job), ct.byref(sobol))
 
+    def retarget(self, scenes, sobols, powers, caps) -> None:
+        """Point the plan at another job list (ehss_plan_retarget): its buffers
+        and stream are reused; the previous scenes are released.  An empty list
+        only releases them."""
+        n = len(scenes)
+        if not (len(sobols) == len(powers) == len(caps) == n):
+            raise ValueError("one scene, scramble, power and cap per job required")
+        handles = (ct.c_void_p * max(n, 1))(*[s.handle for s in scenes])
+        sob = sobol_table(sobols) if n else (L.Sobol * 1)()
+        pw = np.asarray(powers, np.int32)
+        cp = np.asarray(caps, np.int32)
+        L.call(f"ehss_{self._prefix}_retarget", self.handle, n, handles, sob, L.ptr(pw), L.ptr(cp))
+        self.scenes = list(scenes)
+        self.caps = [int(c) for c in caps]
+        self.powers = [int(p) for p in powers]
+        self.n_jobs = n
+
+    def release(self) -> None:
+        """Let the scenes go (rebuild and destroy are refused while held)."""
+        self.retarget([], [], [], [])
+
     def launch(self) -> None:
         L.call(f"ehss_{self._prefix}_launch", self.handle)
 

@@ -6,10 +6,12 @@ Each request is a structure (centres, radii).  The loop
   footprint (scene trees + solve workspace) fits the budget left -- the scene
   and its trees are built on the device at admission and stay resident until
   the request completes;
-* keeps every admitted request's next solve in flight on the request's own
-  stream (ehss_solve_async), so solves of different structures and sizes
-  overlap on the device and a new request joins as soon as it is admitted
-  (continuous batching, not a queue of whole jobs);
+* keeps every admitted request's next solve in flight: either on the
+  request's own stream (ehss_solve_async), so solves of different structures
+  and sizes overlap on the device, or -- batched mode -- gathered with the next
+  replicates of every other ready request into one multi-structure launch (a
+  retargeted plan), a few batches in flight; either way a new request joins as
+  soon as it is admitted (continuous batching, not a queue of whole jobs);
 * chooses the ray count per request adaptively: independent scrambled-Sobol
   replicates of 2^power rays (seeds replicate_seed(seed, k), the reference's
   per-replicate rule) until the relative standard error of the mean CCS is at
@@ -121,13 +123,23 @@ class Server:
     it also has rebuild(centers, radii) and device_bytes(), finished scenes are
     reused.  scramble(seed) and replicate_seed(seed, k) give the per-replicate
     nets.  rebuild_workers: threads that rebuild reused scenes (0: in the loop
-    itself)."""
+    itself).
+
+    batch_plans > 0 switches to batched launches: the next replicates of every
+    request that is ready are gathered into ONE multi-structure launch (a
+    prepared plan, retargeted for each batch: ehss_plan_retarget), with up to
+    batch_plans batches in flight.  One request's solve alone is a few dozen
+    workgroups; a batch of many requests fills the device.  A job's totals in a
+    plan are bit-identical to the same solve alone, so results do not depend on
+    the mode or on the mix.  plan_factory(scenes, sobols, powers, caps) makes a
+    plan (default: GPUPlan with the server's precision and PA setting)."""
 
     def __init__(self, budget_bytes: int, max_inflight: int = 16, power: int = 12, cap: int = 64,
                  target_rel: float = 0.01, min_reps: int = 4, max_reps: int = 64, precision: str = "fp32-fixup",
                  pa: bool = True, scene_factory=None, scramble=None, replicate_seed=None, poll_s: float = 5e-5,
-                 rebuild_workers: int = 12):
-        if budget_bytes <= 0 or max_inflight < 1 or min_reps < 2 or max_reps < min_reps or rebuild_workers < 0:
+                 rebuild_workers: int = 12, batch_plans: int = 0, plan_factory=None, max_batch_jobs: int = 4096):
+        if budget_bytes <= 0 or max_inflight < 1 or min_reps < 2 or max_reps < min_reps or rebuild_workers < 0 \
+                or batch_plans < 0 or max_batch_jobs < max_reps:
             raise ValueError("invalid server limits")
         self.rebuild_workers = int(rebuild_workers)
         if scene_factory is None or scramble is None or replicate_seed is None:
@@ -135,17 +147,25 @@ class Server:
             scene_factory = scene_factory or GPUScene
             scramble = scramble or sc
             replicate_seed = replicate_seed or rs
+        if batch_plans and plan_factory is None:
+            from .engine import GPUPlan
+
+            def plan_factory(scenes, sobols, powers, caps):
+                return GPUPlan(scenes, sobols, powers, caps, precision=precision, pa=pa)
         self.budget = int(budget_bytes)
         self.max_inflight = int(max_inflight)
         self.power, self.cap = int(power), int(cap)
         self.target_rel, self.min_reps, self.max_reps = float(target_rel), int(min_reps), int(max_reps)
         self.precision, self.pa = precision, bool(pa)
         self.scene_factory, self.scramble, self.replicate_seed = scene_factory, scramble, replicate_seed
+        self.batch_plans, self.plan_factory, self.max_batch_jobs = int(batch_plans), plan_factory, int(max_batch_jobs)
         self.poll_s = poll_s
         self.peak_bytes = 0
         self.peak_inflight = 0
         self.scenes_built = 0
         self.scenes_reused = 0
+        self.batches = 0
+        self.peak_batch_jobs = 0
 
     def footprint(self, req: Request) -> int:
         return footprint_bytes(len(req.radii), self.power, self.cap, self.max_reps)
@@ -162,10 +182,12 @@ class Server:
             need = math.ceil(1.2 * (v.std(ddof=1) / (self.target_rel * abs(v.mean()))) ** 2)
         return int(max(1, min(need - k, self.max_reps - k)))
 
-    def _launch(self, live: _Live) -> None:
+    def _seeds(self, live) -> list:
         k = len(live.res.omegas)
-        count = self._next_count(live.res)
-        sobols = [self.scramble(self.replicate_seed(live.req.seed, k + j)) for j in range(count)]
+        return [self.replicate_seed(live.req.seed, k + j) for j in range(self._next_count(live.res))]
+
+    def _launch(self, live: _Live) -> None:
+        sobols = [self.scramble(s) for s in self._seeds(live)]
         live.scene.solve_async(sobols, self.power, self.cap, precision=self.precision, pa=self.pa)
         live.res.launches += 1
 
@@ -186,6 +208,11 @@ class Server:
         used = 0         # footprints of live requests plus the bytes of idle scenes
         t0 = clock()
         pool = ThreadPoolExecutor(self.rebuild_workers) if self.rebuild_workers > 0 else None
+        batched = self.batch_plans > 0
+        ready: collections.deque = collections.deque()  # batched: requests whose next replicates wait
+        plans_free: list = []
+        plans_made = 0
+        in_flight: list = []  # batched: (plan, [(live item, replicate count)])
 
         def take_scene(req, fp):
             """A scene holding req: an idle one rebuilt in place (on a worker
@@ -216,6 +243,61 @@ class Server:
                 used += held
             else:
                 scene.close()
+
+        def start(item):
+            """The request's next replicates: launched now (per-request mode) or
+            queued for the next batch."""
+            if batched:
+                ready.append(item)
+            else:
+                self._launch(item)
+
+        def flush_batches():
+            """Batched mode: every ready request's next replicates, as jobs of one
+            plan launch, while a plan is free."""
+            nonlocal plans_made
+            while ready and (plans_free or plans_made < self.batch_plans):
+                scenes, sobols, assign = [], [], []
+                while ready and len(scenes) + self._next_count(ready[0].res) <= self.max_batch_jobs:
+                    item = ready.popleft()
+                    seeds = self._seeds(item)
+                    scenes += [item.scene] * len(seeds)
+                    sobols += [self.scramble(sd) for sd in seeds]
+                    assign.append((item, len(seeds)))
+                    item.res.launches += 1
+                powers, caps = [self.power] * len(scenes), [self.cap] * len(scenes)
+                if plans_free:
+                    plan = plans_free.pop()
+                    plan.retarget(scenes, sobols, powers, caps)
+                else:
+                    plan = self.plan_factory(scenes, sobols, powers, caps)
+                    plans_made += 1
+                plan.launch()
+                in_flight.append((plan, assign))
+                self.batches += 1
+                self.peak_batch_jobs = max(self.peak_batch_jobs, len(scenes))
+
+        def finished():
+            """(live item, list of replicate outputs) of every completed launch."""
+            out = []
+            if batched:
+                for entry in list(in_flight):
+                    plan, assign = entry
+                    if not plan.done():
+                        continue
+                    in_flight.remove(entry)
+                    outs = plan.wait()
+                    plan.release()  # the scenes may be rebuilt for other requests now
+                    plans_free.append(plan)
+                    k = 0
+                    for item, count in assign:
+                        out.append((item, outs[k:k + count]))
+                        k += count
+            else:
+                for item in list(live.values()):
+                    if item.rebuild is None and item.scene.solve_done():
+                        out.append((item, item.scene.solve_wait()))
+            return out
 
         try:
             while pending or waiting or live:
@@ -256,50 +338,52 @@ class Server:
                     used += acct
                     live[req.rid] = item
                     if rebuild is None:
-                        self._launch(item)
+                        start(item)
                 self.peak_bytes = max(self.peak_bytes, used)
                 self.peak_inflight = max(self.peak_inflight, len(live))
-                # Progress: collect every finished solve, relaunch or complete.
                 progressed = False
+                # Rebuilds that finished: the request's first launch follows.
                 for rid in list(live):
                     item = live[rid]
-                    if item.rebuild is not None:
-                        if not item.rebuild.done():
-                            continue
-                        progressed = True
-                        pending_rebuild, item.rebuild = item.rebuild, None
-                        try:
-                            pending_rebuild.result()
-                        except Exception as exc:  # invalid structure: report; the scene stays idle
-                            item.res.error, item.res.finished = str(exc), clock() - t0
-                            used -= item.footprint
-                            del live[rid]
-                            idle.append((item.held, item.scene))
-                            used += item.held
-                            done.append(item.res)
-                            continue
-                        self.scenes_reused += 1
-                        self._launch(item)
-                        continue
-                    if not item.scene.solve_done():
+                    if item.rebuild is None or not item.rebuild.done():
                         continue
                     progressed = True
+                    pending_rebuild, item.rebuild = item.rebuild, None
+                    try:
+                        pending_rebuild.result()
+                    except Exception as exc:  # invalid structure: report; the scene stays idle
+                        item.res.error, item.res.finished = str(exc), clock() - t0
+                        used -= item.footprint
+                        del live[rid]
+                        idle.append((item.held, item.scene))
+                        used += item.held
+                        done.append(item.res)
+                        continue
+                    self.scenes_reused += 1
+                    start(item)
+                if batched:
+                    flush_batches()
+                # Completed launches: relaunch or complete each request.
+                for item, outs in finished():
+                    progressed = True
                     res = item.res
-                    for out in item.scene.solve_wait():
-                        res.omegas.append(out["omega"])
-                        res.pa.append(out["pa"])
-                        res.unresolved += int(out["unresolved"])
-                        res.rays += int(out["rays"])
+                    for o in outs:
+                        res.omegas.append(o["omega"])
+                        res.pa.append(o["pa"])
+                        res.unresolved += int(o["unresolved"])
+                        res.rays += int(o["rays"])
                     if self._converged(res):
                         res.ccs = float(np.mean(res.omegas))
                         res.rel_se = _rel_se(res.omegas)
                         res.finished = clock() - t0
                         used -= item.footprint
-                        del live[rid]
+                        del live[res.rid]
                         release_scene(item.scene)
                         done.append(res)
                     else:
-                        self._launch(item)
+                        start(item)
+                if batched:
+                    flush_batches()
                 if not progressed:
                     if not live and not waiting and pending:
                         sleep(max(0.0, min(pending[0].arrival - (clock() - t0), 0.01)))
@@ -308,6 +392,12 @@ class Server:
         finally:
             if pool is not None:
                 pool.shutdown(wait=True)
+            for plan, _ in in_flight:
+                plan.wait()
+                plan.release()
+                plans_free.append(plan)
+            for plan in plans_free:
+                plan.close()
         for _, scene in idle:
             scene.close()
         return done

@@ -361,6 +361,14 @@ typedef struct ehss_plan ehss_plan;
  * 65535 jobs per plan. */
 int ehss_plan_create(int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols, const int32_t* powers,
                      const int32_t* caps, int precision, unsigned flags, ehss_plan** out);
+/* Re-points the plan at another job list (same precision and flags), reusing
+ * its stream and grow-only device buffers: waits for the plan's last launch,
+ * releases the scenes it held, holds the new ones, and drops a captured graph
+ * (the next launch captures again).  n_jobs == 0 only releases the scenes.
+ * For continuous batching: a server retargets a few plans to the replicates of
+ * the requests ready at each launch. */
+int ehss_plan_retarget(ehss_plan* plan, int n_jobs, ehss_scene* const* scenes, const ehss_sobol* sobols,
+                       const int32_t* powers, const int32_t* caps);
 /* Replaces the scramble of one job (used by the next launch, graph or not). */
 int ehss_plan_set_sobol(ehss_plan* plan, int job, const ehss_sobol* sobol);
 /* Asynchronous: enqueues the plan on its stream and returns. */

@@ -211,3 +211,36 @@ def test_plan_holds_its_scenes():
     finally:
         sc.close()
         other.close()
+
+
+def test_retargeted_plan_equals_solves_alone_and_releases_scenes():
+    """ehss_plan_retarget: the same plan object over other job lists (more and
+    fewer jobs, other scenes) gives each job its solve-alone totals; an empty
+    list releases the scenes for rebuild."""
+    from ehss_amd import _lib as L
+    structs = [scenes.globule(300, 4), scenes.globule(650, 1), scenes.SCENES["c60"]()]
+    scs = [GPUScene(c, r) for c, r in structs]
+    try:
+        plan = GPUPlan([scs[0]], [scramble(1)], [12], [64])
+        plan.run()
+        for k, jobs in enumerate(([(0, 12, 64), (1, 14, 32), (2, 10, 64)], [(1, 11, 64)],
+                                  [(2, 12, 16), (2, 12, 64), (0, 13, 64), (1, 9, 8)])):
+            say("retarget", k, jobs)
+            sob = [scramble(100 * k + j) for j in range(len(jobs))]
+            plan.retarget([scs[i] for i, _, _ in jobs], sob, [p for _, p, _ in jobs], [c for _, _, c in jobs])
+            got = plan.run(histogram=True)
+            for (i, p, c), sb, g in zip(jobs, sob, got):
+                alone = scs[i].solve(sb, p, c, histogram=True)
+                for key in PHYSICAL:
+                    assert alone[key] == g[key], (k, i, key)
+                np.testing.assert_array_equal(alone["histogram"], g["histogram"])
+        with pytest.raises(L.EHSSError, match="held by"):
+            scs[1].rebuild(*structs[0])
+        plan.release()
+        scs[1].rebuild(*structs[0])
+        with pytest.raises(L.EHSSError, match="no jobs"):
+            plan.launch()
+        plan.close()
+    finally:
+        for s in scs:
+            s.close()

@@ -171,3 +171,18 @@ def test_publication_protocol_on_device():
             assert att.omega == pytest.approx(np.mean(alone), rel=1e-15)
         tails = [a.tail_bound for a in res.attempts]
         assert all(b <= a for a, b in zip(tails, tails[1:]))
+
+
+def test_batched_serving_equals_per_request_launches():
+    """Batched mode (replicates of every ready request in one retargeted plan
+    launch) gives every request the same replicate values as per-request
+    launches."""
+    reqs = serve.poisson_stream(_bases(), rate_hz=2000.0, count=24, seed=8)
+    kw = dict(max_inflight=8, power=11, cap=64, target_rel=0.02, min_reps=4, max_reps=24)
+    per = {r.rid: r for r in serve.Server(1 << 34, **kw).run(reqs)}
+    srv = serve.Server(1 << 34, batch_plans=2, **kw)
+    bat = {r.rid: r for r in srv.run(reqs)}
+    say("batched", serve.latency_summary(bat.values()), "batches", srv.batches, "peak jobs", srv.peak_batch_jobs)
+    assert srv.batches < sum(r.launches for r in bat.values()) and srv.scenes_reused > 0
+    for rid, r in bat.items():
+        assert r.error is None and r.omegas == per[rid].omegas, rid

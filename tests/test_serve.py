@@ -23,6 +23,7 @@ class Clock:
 class FakeScene:
     """Solve time proportional to atoms; Omega = pi R^2 (1 + 0.03 z(seed))."""
     live = 0
+    held = 0  # plans holding this scene (batched mode); closing a held scene is an error
 
     def __init__(self, clock, centers, radii):
         if len(radii) == 0:
@@ -51,7 +52,7 @@ class FakeScene:
         return out
 
     def close(self):
-        assert not self.closed
+        assert not self.closed and self.held == 0
         self.closed = True
         FakeScene.live -= 1
 
@@ -232,3 +233,71 @@ def test_a_failed_launch_is_raised_not_reported_as_a_bad_structure():
         with pytest.raises(RuntimeError, match="device failure"):
             srv.run(stream(count=3, rate=10.0), clock=clock, sleep=clock.sleep)
     FakeScene.live = before  # the aborted runs leave their scenes to the caller
+
+
+class FakePlan:
+    """A multi-structure launch on the simulated device: jobs of several scenes
+    run together in less time than one launch each; it holds its scenes."""
+
+    made = 0
+
+    def __init__(self, clock, scenes, sobols, powers, caps):
+        self.clock, self.scenes, self.busy_until = clock, [], None
+        FakePlan.made += 1
+        self.retarget(scenes, sobols, powers, caps)
+
+    def retarget(self, scenes, sobols, powers, caps):
+        assert self.busy_until is None, "retarget waits for the last launch"
+        for s in self.scenes:
+            s.held -= 1
+        self.scenes, self.seeds, self.powers = list(scenes), list(sobols), list(powers)
+        for s in self.scenes:
+            assert not s.closed
+            s.held += 1
+
+    def launch(self):
+        assert self.scenes and self.busy_until is None
+        self.busy_until = self.clock() + 2e-7 * sum(s.n for s in set(self.scenes)) + 1e-6 * len(self.scenes)
+
+    def done(self):
+        return self.busy_until is not None and self.clock() >= self.busy_until
+
+    def wait(self):
+        assert self.done()
+        self.busy_until = None
+        return [dict(omega=s.base * (1 + 0.03 * np.random.default_rng(sd).normal()), pa=s.base, unresolved=0,
+                     rays=1 << pw) for s, sd, pw in zip(self.scenes, self.seeds, self.powers)]
+
+    def release(self):
+        self.retarget([], [], [], [])
+
+    def close(self):
+        assert not self.scenes
+
+
+def test_batched_mode_gives_the_same_results_as_per_request_launches():
+    reqs = stream(count=60, rate=5000.0)
+    kw = dict(max_inflight=12, power=10, target_rel=0.01, min_reps=4, max_reps=200)
+    clock = Clock()
+    FakeScene.held = 0
+    alone = {r.rid: r for r in make_server(clock, **kw).run(reqs, clock=clock, sleep=clock.sleep)}
+    clock = Clock()
+    srv = make_server(clock, batch_plans=2, plan_factory=lambda *a: FakePlan(clock, *a), **kw)
+    mixed = {r.rid: r for r in srv.run(reqs, clock=clock, sleep=clock.sleep)}
+    assert len(mixed) == len(reqs) and FakeScene.live == 0
+    assert srv.batches < sum(r.launches for r in mixed.values())  # requests shared launches
+    assert srv.peak_batch_jobs > srv.min_reps
+    for rid, r in mixed.items():
+        assert r.error is None and r.omegas == alone[rid].omegas and r.ccs == alone[rid].ccs
+
+
+def test_batched_mode_limits_jobs_per_launch():
+    reqs = stream(count=30, rate=1e5)
+    clock = Clock()
+    FakeScene.held = 0
+    srv = make_server(clock, max_inflight=30, power=10, target_rel=0.01, min_reps=4, max_reps=8, batch_plans=1,
+                      plan_factory=lambda *a: FakePlan(clock, *a), max_batch_jobs=16)
+    res = srv.run(reqs, clock=clock, sleep=clock.sleep)
+    assert len(res) == 30 and srv.peak_batch_jobs <= 16 and FakeScene.live == 0
+    with pytest.raises(ValueError):
+        make_server(clock, max_reps=64, batch_plans=1, max_batch_jobs=32)
