@@ -66,7 +66,11 @@ class Sobol(ct.Structure):
 class SceneInfo(ct.Structure):
     _fields_ = [("atoms", ct.c_int64), ("groups", ct.c_int64), ("nodes", ct.c_int64), ("pa_nodes", ct.c_int64),
                 ("leaf_size", ct.c_int32), ("max_depth", ct.c_int32), ("center", ct.c_double * 3),
-                ("radius", ct.c_double), ("scale", ct.c_double), ("device_bytes", ct.c_int64)]
+                ("radius", ct.c_double), ("scale", ct.c_double), ("device_bytes", ct.c_int64),
+                ("build_path", ct.c_int32), ("reserved", ct.c_int32)]
+
+
+BUILD_GENERAL, BUILD_SHAPED = 0, 1
 
 
 class Totals(ct.Structure):
@@ -109,6 +113,7 @@ _SIGS = {
     "ehss_scramble": (_INT, [ct.c_uint64, ct.POINTER(Sobol)]),
     "ehss_scene_rebuild": (_INT, [_P, ct.c_int64, _P, _P, _INT]),
     "ehss_scene_get_info": (_INT, [_P, ct.POINTER(SceneInfo)]),
+    "ehss_tree_counts": (_INT, [_I64, _INT, _P]),
     "ehss_scene_export_tree": (_INT, [_P] + [_P] * 7),
     "ehss_scene_export_pa_tree": (_INT, [_P] + [_P] * 6),
     "ehss_scene_trace": (_INT, [_P, _I64, _INT, _INT, _INT] + [_P] * 8),

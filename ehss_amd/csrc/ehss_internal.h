@@ -161,6 +161,40 @@ int build_scene_device(int32_t n, int32_t leaf_size, const double* d_centers, co
                        int32_t* d_pa_ids, const int32_t* d_custom_groups, int32_t custom_ng, BuiltScene* out,
                        BuildWork& work, hipStream_t s);
 
+// Shape of the median-split segment tree (ehss_static.py:16-50) of n atoms
+// when no segment has zero key span: a segment of size s at depth d splits iff
+// s > leaf and d < kMaxSplitDepth, into sizes s / 2 and s - s / 2.  Every
+// segment at depth d then has size q[d] = n >> d or q[d] + 1 (halving q or
+// q + 1 gives q / 2 or q / 2 + 1), so leaves[d][t], the number of leaves under
+// a segment of size q[d] + t at depth d, gives the size of every subtree.
+constexpr int kShapeDepths = 32;
+constexpr int kMaxSplitDepth = 24;
+struct TreeShape {
+    int32_t n, leaf, groups, max_depth;
+    int32_t q[kShapeDepths];
+    int32_t leaves[kShapeDepths][2];
+};
+TreeShape make_tree_shape(int32_t n, int32_t leaf);
+// True when two atoms have the same key triple ((c - r) + (c + r)) / 2: only
+// then can a segment of two or more atoms have zero span on every axis.
+bool has_duplicate_keys(int64_t n, const double* centers, const double* radii);
+// Largest leaf size the shaped build takes (its leaves are ranked in one thread).
+constexpr int32_t kShapedMaxLeaf = 32;
+// The build for structures without duplicate key triples: every count is
+// known from `shape`, so it enqueues its kernels on s and returns without
+// synchronising.  Same outputs as build_scene_device without caller groups.
+int build_scene_shaped(const TreeShape& shape, const double* d_centers, const double* d_radii, double rmax,
+                       double4* d_sph, int32_t* d_groups, int32_t* d_offsets, int32_t* d_eids, Node64* d_nodes,
+                       NodePA* d_pa, int32_t* d_pa_ids, BuildWork& work, hipStream_t s);
+
+// Roots of the cooperative-query subtrees of the static tree over ng groups
+// (its topology is implicit in ng: ehss_cpu_geometry.py:19-38): starting from
+// the root, every internal node is replaced by its two children while the list
+// stays within kFrontier entries.
+std::vector<int32_t> frontier_of(int32_t ng);
+// Writes a frontier list (at most kFrontier entries) to device memory on s.
+int launch_put_frontier(const std::vector<int32_t>& f, int32_t* dst, hipStream_t s);
+
 // Impact disk of ehss_state.py:46-47 computed on the host in NumPy's order:
 // center = sequential sum / n, radius = max(sqrt((dx^2 + dy^2) + dz^2) + r) * (1 + 1e-9).
 void enclosing_disk(int64_t n, const double* centers, const double* radii, double center[3], double* radius);
@@ -192,6 +226,10 @@ struct ehss_scene {
     ehss::SceneView view;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;
+    // recorded after each (re)build: launches on other streams wait for it
+    hipEvent_t ev_built = nullptr;
+    int build_path = 0;     // EHSS_BUILD_SHAPED or EHSS_BUILD_GENERAL
+    int64_t cap_atoms = 0;  // largest structure the grow-only arrays were sized for
     // Serialises the entry points that use this scene's buffers: calls on one
     // scene from several threads wait for each other instead of racing (a
     // rebuild may reallocate arrays another call's kernels still read).

@@ -251,6 +251,9 @@ int fill_plan(ehss_plan* p, int n_jobs, ehss_scene* const* scenes, const ehss_so
     for (int j = 0; j < n_jobs; ++j) {
         ehss_scene* sc = scenes[j];
         std::lock_guard<std::mutex> guard(sc->lock);
+        // The plan's launches read the scene's arrays: they follow its last
+        // (re)build, which cannot change while the plan holds the scene.
+        if (j == 0 || scenes[j - 1] != sc) EHSS_HIP(hipStreamWaitEvent(p->stream, sc->ev_built, 0));
         PlanJob& J = p->jobs[j];
         memset(&J, 0, sizeof J);
         J.view = sc->view;

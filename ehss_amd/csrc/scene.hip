@@ -41,33 +41,13 @@ using namespace ehss;
 
 namespace {
 
-// Roots of the subtrees one wave splits a cooperative query over: starting from
-// the root, every internal node is replaced by its two children (preorder: the
-// left child is node + 1, the right child starts where the left subtree ends)
-// while the list stays within kFrontier entries (eight per lane).
-int build_frontier(ehss_scene* sc, hipStream_t s) {
-    std::vector<Node64> nodes((size_t)sc->nn);
-    EHSS_HIP(hipMemcpyAsync(nodes.data(), sc->nodes.p, sizeof(Node64) * sc->nn, hipMemcpyDeviceToHost, s));
-    EHSS_HIP(hipStreamSynchronize(s));
-    std::vector<int32_t> f{0}, next;
-    for (;;) {
-        next.clear();
-        bool split = false;
-        for (const int32_t v : f) {
-            if (nodes[v].leaf >= 0) {
-                next.push_back(v);
-            } else {
-                next.push_back(v + 1);
-                next.push_back(nodes[v + 1].end);
-                split = true;
-            }
-        }
-        if (!split || next.size() > (size_t)kFrontier) break;
-        f.swap(next);
-    }
-    EHSS_TRY(sc->frontier.reserve(sizeof(int32_t) * f.size()));
-    EHSS_HIP(hipMemcpyAsync(sc->frontier.p, f.data(), sizeof(int32_t) * f.size(), hipMemcpyHostToDevice, s));
-    EHSS_HIP(hipStreamSynchronize(s));  // f is a local
+// Roots of the subtrees one wave splits a cooperative query over (at most
+// kFrontier): a function of the group count alone, since the static tree's
+// topology is implicit in it.
+int put_frontier(ehss_scene* sc, hipStream_t s) {
+    const std::vector<int32_t> f = frontier_of(sc->ng);
+    EHSS_TRY(sc->frontier.reserve(sizeof(int32_t) * kFrontier));
+    EHSS_TRY(launch_put_frontier(f, sc->frontier.as<int32_t>(), s));
     sc->view.frontier = sc->frontier.as<int32_t>();
     sc->view.n_frontier = (int32_t)f.size();
     return EHSS_OK;
@@ -121,6 +101,9 @@ int build_into(ehss_scene* sc, int64_t n, const double* centers, const double* r
         scale = std::max(scale, e);
     }
     sc->scale = scale;
+    // A grow-only array that grows is freed and reallocated: let the previous
+    // build's kernels (not synchronised) finish with it first.
+    if (n > sc->cap_atoms) EHSS_HIP(hipStreamSynchronize(s));
     EHSS_TRY(sc->centers.reserve(sizeof(double) * 3 * n));
     EHSS_TRY(sc->radii.reserve(sizeof(double) * n));
     EHSS_TRY(sc->sph.reserve(sizeof(double4) * n));
@@ -136,16 +119,38 @@ int build_into(ehss_scene* sc, int64_t n, const double* centers, const double* r
         EHSS_TRY(sc->custom_groups.reserve(sizeof(int32_t) * n));
         EHSS_HIP(hipMemcpyAsync(sc->custom_groups.p, cg.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
     }
-    BuiltScene built;
-    EHSS_TRY(build_scene_device((int32_t)n, leaf_size, sc->centers.as<double>(), sc->radii.as<double>(),
-                                sc->sph.as<double4>(), sc->groups.as<int32_t>(), sc->offsets.as<int32_t>(),
-                                sc->eids.as<int32_t>(), sc->nodes.as<Node64>(), sc->pa.as<NodePA>(),
-                                sc->pa_ids.as<int32_t>(), custom_ng ? sc->custom_groups.as<int32_t>() : nullptr,
-                                custom_ng, &built, sc->build_work, s));
-    sc->ng = built.n_groups;
-    sc->nn = built.n_nodes;
-    sc->pa_nn = built.pa_nodes;
-    sc->max_depth = built.max_depth;
+    // Without caller groups or two atoms sharing a key triple, the tree's shape
+    // is known here and the build runs without synchronising (build_shaped.hip);
+    // otherwise the level-by-level build reads its counts back.  EHSS_BUILD_PATH
+    // = general forces the latter (for comparisons).
+    const char* forced = getenv("EHSS_BUILD_PATH");
+    const bool shaped = !custom_ng && leaf_size <= kShapedMaxLeaf && !(forced && !strcmp(forced, "general")) &&
+                        !has_duplicate_keys(n, centers, radii);
+    if (shaped) {
+        const TreeShape shape = make_tree_shape((int32_t)n, leaf_size);
+        double rmax = 0;
+        for (int64_t i = 0; i < n; ++i) rmax = std::max(rmax, radii[i]);
+        EHSS_TRY(build_scene_shaped(shape, sc->centers.as<double>(), sc->radii.as<double>(), rmax,
+                                    sc->sph.as<double4>(), sc->groups.as<int32_t>(), sc->offsets.as<int32_t>(),
+                                    sc->eids.as<int32_t>(), sc->nodes.as<Node64>(), sc->pa.as<NodePA>(),
+                                    sc->pa_ids.as<int32_t>(), sc->build_work, s));
+        sc->ng = shape.groups;
+        sc->nn = 2 * shape.groups - 1;
+        sc->pa_nn = 2 * shape.groups - 1;
+        sc->max_depth = shape.max_depth;
+    } else {
+        BuiltScene built;
+        EHSS_TRY(build_scene_device((int32_t)n, leaf_size, sc->centers.as<double>(), sc->radii.as<double>(),
+                                    sc->sph.as<double4>(), sc->groups.as<int32_t>(), sc->offsets.as<int32_t>(),
+                                    sc->eids.as<int32_t>(), sc->nodes.as<Node64>(), sc->pa.as<NodePA>(),
+                                    sc->pa_ids.as<int32_t>(), custom_ng ? sc->custom_groups.as<int32_t>() : nullptr,
+                                    custom_ng, &built, sc->build_work, s));
+        sc->ng = built.n_groups;
+        sc->nn = built.n_nodes;
+        sc->pa_nn = built.pa_nodes;
+        sc->max_depth = built.max_depth;
+    }
+    sc->build_path = shaped ? EHSS_BUILD_SHAPED : EHSS_BUILD_GENERAL;
     SceneView& v = sc->view;
     v = SceneView();
     v.n_atoms = sc->n;
@@ -173,8 +178,10 @@ int build_into(ehss_scene* sc, int64_t n, const double* centers, const double* r
     v.sph32 = sc->sph32.as<float4>();
     v.sphs = sc->sphs.as<double4>();
     v.leaves32 = sc->leaves32.as<Node32>();
-    EHSS_TRY(build_frontier(sc, s));
-    EHSS_HIP(hipStreamSynchronize(s));
+    EHSS_TRY(put_frontier(sc, s));
+    // Work on other streams that reads the scene (plans, caller streams) waits for this.
+    EHSS_HIP(hipEventRecord(sc->ev_built, s));
+    sc->cap_atoms = std::max(sc->cap_atoms, n);
     // Per-ray records of an earlier structure no longer apply.
     sc->last_rays = 0;
     sc->last_times = 0;
@@ -197,6 +204,7 @@ int create_scene(int64_t n, const double* centers, const double* radii, const in
                 if (p->ev0) (void)hipEventDestroy(p->ev0);
                 if (p->ev1) (void)hipEventDestroy(p->ev1);
                 if (p->ev_done) (void)hipEventDestroy(p->ev_done);
+                if (p->ev_built) (void)hipEventDestroy(p->ev_built);
                 if (p->stream) (void)hipStreamDestroy(p->stream);
                 delete p;
             }
@@ -206,6 +214,7 @@ int create_scene(int64_t n, const double* centers, const double* radii, const in
     EHSS_HIP(hipEventCreate(&sc->ev0));
     EHSS_HIP(hipEventCreate(&sc->ev1));
     EHSS_HIP(hipEventCreateWithFlags(&sc->ev_done, hipEventDisableTiming));
+    EHSS_HIP(hipEventCreateWithFlags(&sc->ev_built, hipEventDisableTiming));
     EHSS_TRY(build_into(sc, n, centers, radii, cg, custom_ng, leaf_size));
     *out = sc;
     sc = nullptr;  // release the guard
@@ -261,6 +270,7 @@ int ehss_scene_destroy(ehss_scene* scene) {
     if (scene->ev0) (void)hipEventDestroy(scene->ev0);
     if (scene->ev1) (void)hipEventDestroy(scene->ev1);
     if (scene->ev_done) (void)hipEventDestroy(scene->ev_done);
+    if (scene->ev_built) (void)hipEventDestroy(scene->ev_built);
     if (scene->stream) (void)hipStreamDestroy(scene->stream);
     delete scene;
     return EHSS_OK;
@@ -279,6 +289,8 @@ int ehss_scene_get_info(const ehss_scene* scene, ehss_scene_info* info) {
     info->radius = scene->radius;
     info->scale = scene->scale;
     info->device_bytes = scene->device_bytes();
+    info->build_path = scene->build_path;
+    info->reserved = 0;
     return EHSS_OK;
 }
 

@@ -1081,6 +1081,7 @@ int ehss_solve_noalloc(ehss_scene* sc, const ehss_sobol* sobol, int power, int c
                          kNoallocScratchBytes);
 
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : sc->stream;
+    if (s != sc->stream) EHSS_HIP(hipStreamWaitEvent(s, sc->ev_built, 0));  // the scene's last (re)build
     EHSS_TRY(launch_zero(ctr, words, s));
     EHSS_TRY(launch_zero(hist, bins, s));
     EHSS_TRY(launch_zero(led, bins, s));

@@ -144,7 +144,24 @@ typedef struct {
     double radius;    /* max(|c_i - center| + r_i) * (1 + 1e-9) (ehss_state.py:47) */
     double scale;     /* FP32 filter scale: bound on |coordinate - center| */
     int64_t device_bytes;
+    int32_t build_path; /* EHSS_BUILD_SHAPED or EHSS_BUILD_GENERAL: how the last (re)build ran */
+    int32_t reserved;
 } ehss_scene_info;
+
+/* Tree builds.  Without caller groups, with leaf_size <= 32 and no two atoms
+ * sharing their key triple ((c - r) + (c + r)) / 2, no segment of the median
+ * split can have zero span, so the tree's shape is a function of (n, leaf_size):
+ * the build is enqueued on the scene's stream with every count known and
+ * returns without synchronising (EHSS_BUILD_SHAPED).  Otherwise the level-by-
+ * level build reads its counts back (EHSS_BUILD_GENERAL).  Both give the same
+ * trees.  Setting the environment variable EHSS_BUILD_PATH=general forces the
+ * latter. */
+#define EHSS_BUILD_GENERAL 0
+#define EHSS_BUILD_SHAPED 1
+/* Group, static-tree node and PA node counts and the depth of the deepest
+ * segment of the median-split tree of n atoms with leaf_size, when no segment
+ * has zero span (the shaped case above): out[0..3]. */
+int ehss_tree_counts(int64_t n, int leaf_size, int64_t* out);
 
 /* centers: n x 3 (row-major), radii: n; leaf_size >= 1.  Builds on the device. */
 int ehss_scene_create(int64_t n, const double* centers, const double* radii, int leaf_size, ehss_scene** out);

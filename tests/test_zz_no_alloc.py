@@ -70,3 +70,28 @@ def test_noalloc_refuses_its_limits():
                         (dict(power=10, cap=64, flags=L.FLAG_KEEP_RAYS), "no per-ray records")):
             with pytest.raises(L.EHSSError, match=msg):
                 _noalloc(sc, sob, **kw)
+
+
+def test_noalloc_on_a_caller_stream_right_after_rebuild():
+    """The caller's stream waits for the scene's last (re)build, which returns
+    before its kernels have run."""
+    c, r = scenes.globule(3000, 12)
+    with GPUScene(c, r) as fresh:
+        want = fresh.solve(scramble(4), 12, 64, pa=True, histogram=True)
+    with GPUScene(*scenes.globule(150, 13)) as sc:
+        tot = torch.zeros(ct.sizeof(L.Totals), dtype=torch.uint8, device="cuda")
+        hist = torch.zeros(65, dtype=torch.int64, device="cuda")
+        led = torch.zeros(65, dtype=torch.float64, device="cuda")
+        stream = torch.cuda.Stream()
+        torch.cuda.synchronize()
+        say("noalloc after rebuild")
+        sc.rebuild(c, r)
+        L.call("ehss_solve_noalloc", sc.handle, ct.byref(scramble(4)), 12, 64, L.PRECISIONS["fp32-fixup"],
+               int(L.FLAG_PA), ct.c_void_p(stream.cuda_stream), ct.c_void_p(tot.data_ptr()),
+               ct.c_void_p(hist.data_ptr()), ct.c_void_p(led.data_ptr()))
+        stream.synchronize()
+        got = L.Totals.from_buffer_copy(tot.cpu().numpy().tobytes()).as_dict()
+        got["histogram"] = hist.cpu().numpy()
+    for k in ("omega", "pa", "escaped", "unresolved", "max_bounces"):
+        assert got[k] == want[k], k
+    np.testing.assert_array_equal(got["histogram"], want["histogram"])
