@@ -108,6 +108,15 @@ class _Live:
     held: int = 0           # device bytes the reused scene held when idle
 
 
+def _best_fit(idle, need: int) -> int:
+    """Index of the idle (bytes, scene) to reuse for a request of `need` bytes:
+    the smallest that holds it, else the largest (the one that grows least)."""
+    fits = [k for k, (held, _) in enumerate(idle) if held >= need]
+    if fits:
+        return min(fits, key=lambda k: idle[k][0])
+    return max(range(len(idle)), key=lambda k: idle[k][0])
+
+
 def _rel_se(values) -> float:
     v = np.asarray(values, dtype=np.float64)
     if len(v) < 2 or v.mean() == 0:
@@ -132,15 +141,24 @@ class Server:
     workgroups; a batch of many requests fills the device.  A job's totals in a
     plan are bit-identical to the same solve alone, so results do not depend on
     the mode or on the mix.  plan_factory(scenes, sobols, powers, caps) makes a
-    plan (default: GPUPlan with the server's precision and PA setting)."""
+    plan (default: GPUPlan with the server's precision and PA setting).
+
+    warm_atoms > 0 makes the scene pool before the stream starts: max_inflight
+    scenes built for a lattice of warm_atoms atoms and solved once with
+    max_reps replicates, so that their grow-only arrays already fit any request
+    up to that size.  A finished request's scene goes back to the pool, and a new
+    request takes the smallest idle scene whose arrays fit it (else the largest):
+    a scene that grows frees an array, and a free synchronises the device."""
 
     def __init__(self, budget_bytes: int, max_inflight: int = 16, power: int = 12, cap: int = 64,
                  target_rel: float = 0.01, min_reps: int = 4, max_reps: int = 64, precision: str = "fp32-fixup",
                  pa: bool = True, scene_factory=None, scramble=None, replicate_seed=None, poll_s: float = 5e-5,
-                 rebuild_workers: int = 12, batch_plans: int = 0, plan_factory=None, max_batch_jobs: int = 4096):
+                 rebuild_workers: int = 12, batch_plans: int = 0, plan_factory=None, max_batch_jobs: int = 4096,
+                 warm_atoms: int = 0):
         if budget_bytes <= 0 or max_inflight < 1 or min_reps < 2 or max_reps < min_reps or rebuild_workers < 0 \
-                or batch_plans < 0 or max_batch_jobs < max_reps:
+                or batch_plans < 0 or max_batch_jobs < max_reps or warm_atoms < 0:
             raise ValueError("invalid server limits")
+        self.warm_atoms = int(warm_atoms)
         self.rebuild_workers = int(rebuild_workers)
         if scene_factory is None or scramble is None or replicate_seed is None:
             from .engine import GPUScene, replicate_seed as rs, scramble as sc
@@ -191,6 +209,25 @@ class Server:
         live.scene.solve_async(sobols, self.power, self.cap, precision=self.precision, pa=self.pa)
         live.res.launches += 1
 
+    def _warm_scenes(self, sleep) -> list:
+        """max_inflight scenes grown for warm_atoms atoms and max_reps replicates
+        (a cubic lattice, spacing 3 A, radius 1.5 A: no two atoms alike)."""
+        side = math.ceil(self.warm_atoms ** (1 / 3))
+        axis = np.arange(side, dtype=np.float64) * 3.0
+        grid = np.stack(np.meshgrid(axis, axis, axis, indexing="ij"), -1).reshape(-1, 3)[:self.warm_atoms]
+        radii = np.full(len(grid), 1.5)
+        sobols = [self.scramble(self.replicate_seed(0, k)) for k in range(self.max_reps)]
+        scenes = []
+        for _ in range(self.max_inflight):
+            scene = self.scene_factory(grid, radii)
+            scene.solve_async(sobols, self.power, self.cap, precision=self.precision, pa=self.pa)
+            scenes.append(scene)
+        for scene in scenes:
+            while not scene.solve_done():
+                sleep(self.poll_s)
+            scene.solve_wait()
+        return scenes
+
     def _converged(self, res: Result) -> bool:
         k = len(res.omegas)
         if k >= self.max_reps:
@@ -206,6 +243,12 @@ class Server:
         done: list[Result] = []
         idle: list = []  # (device bytes, scene) of finished requests, kept for reuse
         used = 0         # footprints of live requests plus the bytes of idle scenes
+        if self.warm_atoms:
+            for scene in self._warm_scenes(sleep):
+                held = int(scene.device_bytes())
+                idle.append((held, scene))
+                used += held
+            self.scenes_built += len(idle)
         t0 = clock()
         pool = ThreadPoolExecutor(self.rebuild_workers) if self.rebuild_workers > 0 else None
         batched = self.batch_plans > 0
@@ -323,6 +366,7 @@ class Server:
                     waiting.popleft()
                     res = Result(req.rid, len(req.radii), arrival=req.arrival, admitted=clock() - t0)
                     if idle:
+                        idle.append(idle.pop(_best_fit(idle, fp)))  # the scene reused next goes last
                         used -= idle[-1][0]  # the scene about to be reused is accounted as live
                         held_before = idle[-1][0]
                     else:

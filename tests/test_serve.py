@@ -301,3 +301,37 @@ def test_batched_mode_limits_jobs_per_launch():
     assert len(res) == 30 and srv.peak_batch_jobs <= 16 and FakeScene.live == 0
     with pytest.raises(ValueError):
         make_server(clock, max_reps=64, batch_plans=1, max_batch_jobs=32)
+
+
+def test_warm_pool_serves_every_request_without_growing_a_scene():
+    """warm_atoms: the pool is made before the stream (its clock starts after);
+    every request reuses a pool scene already large enough, with the results
+    of a cold server."""
+    reqs = stream(count=40)
+    clock = Clock()
+    cold = {r.rid: r for r in make_server(clock, max_inflight=4, power=10).run(reqs, clock=clock, sleep=clock.sleep)}
+    grown = []
+
+    class Tracked(ReusableScene):
+        def rebuild(self, centers, radii):
+            if len(radii) > self.n:
+                grown.append(len(radii))
+            super().rebuild(centers, radii)
+
+    clock = Clock()
+    srv = serve.Server(1 << 40, scene_factory=lambda c, r: Tracked(clock, c, r), scramble=lambda s: s,
+                       replicate_seed=lambda s, k: (s + 104729 * k) % 2 ** 32, max_inflight=4, power=10,
+                       warm_atoms=6000)
+    warm = {r.rid: r for r in srv.run(reqs, clock=clock, sleep=clock.sleep)}
+    assert srv.scenes_built == 4 and srv.scenes_reused == len(reqs) and not grown and FakeScene.live == 0
+    for rid, r in cold.items():
+        assert warm[rid].omegas == r.omegas and warm[rid].ccs == r.ccs
+    with pytest.raises(ValueError):
+        serve.Server(1 << 30, warm_atoms=-1, scene_factory=object, scramble=object, replicate_seed=object)
+
+
+def test_best_fit_reuse():
+    idle = [(500, "a"), (100, "b"), (300, "c"), (900, "d")]
+    assert idle[serve._best_fit(idle, 250)][1] == "c"   # the smallest that holds it
+    assert idle[serve._best_fit(idle, 100)][1] == "b"
+    assert idle[serve._best_fit(idle, 1000)][1] == "d"  # none does: the largest grows least
