@@ -36,11 +36,6 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
-# Hardware queues per process (read once, when HIP initialises): the serving
-# stream keeps up to 24 requests on their own streams, and HIP's default of 4
-# queues serialises their small launches four at a time.  12 measured best for
-# the serving mix (32 queues thrash).  A value set by the caller is kept.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "12")
 
 METRIC = "EHSS traced rays/s + CCS/s at 1% RMSE on 1 MI355X; p50/p99 serve latency"
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
@@ -391,7 +386,17 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the side measurements")
     ap.add_argument("--traffic-probe", action="store_true")
+    ap.add_argument("--hw-queues", type=int, default=12,
+                    help="GPU_MAX_HW_QUEUES for this process (1..16)")
     args = ap.parse_args()
+    if not 1 <= args.hw_queues <= 16:
+        ap.error("--hw-queues must be within 1..16")
+    # Hardware queues per process, read once when HIP initialises (nothing has
+    # yet): the serving stream keeps up to 24 requests on their own streams, and
+    # HIP's default of 4 queues runs their small launches four at a time.  12
+    # measured best for the serving mix; 32 thrash.  Single-stream measurements
+    # do not depend on it.
+    os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the EHSS/PA path runs on a single MI355X here and its "
               "multi-GPU sharding is a design note only; run with --gpus 1.", file=sys.stderr)
