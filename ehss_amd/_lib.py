@@ -63,6 +63,19 @@ class Sobol(ct.Structure):
     _fields_ = [("sv", (ct.c_uint32 * 32) * 4), ("shift", ct.c_uint32 * 4)]
 
 
+class Sobol5(ct.Structure):
+    _fields_ = [("sv", (ct.c_uint32 * 32) * 5), ("shift", ct.c_uint32 * 5)]
+
+
+SOURCE_BOX, SOURCE_ELLIPSOID, SOURCE_REVERSE_PA = 1, 2, 3
+
+
+class Source(ct.Structure):
+    _fields_ = [("kind", ct.c_int32), ("pad", ct.c_int32), ("center", ct.c_double * 3),
+                ("rotation", ct.c_double * 9), ("axes", ct.c_double * 3), ("area", ct.c_double),
+                ("cdf", ct.c_void_p)]
+
+
 class SceneInfo(ct.Structure):
     _fields_ = [("atoms", ct.c_int64), ("groups", ct.c_int64), ("nodes", ct.c_int64), ("pa_nodes", ct.c_int64),
                 ("leaf_size", ct.c_int32), ("max_depth", ct.c_int32), ("center", ct.c_double * 3),
@@ -129,6 +142,8 @@ _SIGS = {
     "ehss_ray_map": (_INT, [ct.POINTER(Sobol), _INT, _P]),
     "ehss_cross_map": (_INT, [_P, _INT, _INT, _P]),
     "ehss_source_rays": (_INT, [_P, ct.POINTER(Sobol), _INT, _P, _P, _P]),
+    "ehss_solve_source": (_INT, [_P, ct.POINTER(Source), ct.POINTER(Sobol5), _INT, _INT, _P, _P, _P, _P]),
+    "ehss_source_states": (_INT, [_P, ct.POINTER(Source), ct.POINTER(Sobol5), _INT] + [_P] * 7),
     "ehss_trig_compat_info": (_INT, [_P, _P, ct.c_size_t, _P, ct.c_size_t]),
     "ehss_scramble_many": (_INT, [_I64, _P, _P]),
     "ehss_solve_extend": (_INT, [_P, _INT, _P, _P, _P]),

@@ -30,8 +30,8 @@ TRIG_LIB = PKG / "libehss_trighost.so"
 TRIG_TABLE = PKG / "trig_compat.bin"
 OBJ = PKG / "_build"
 
-SOURCES = ["capi.hip", "trace.hip", "build.hip", "build_shaped.hip", "scene.hip", "solve.hip", "multi.hip",
-           "pagrid.hip", "scramble.hip"]
+SOURCES = ["capi.hip", "trace.hip", "build.hip", "build_shaped.hip", "scene.hip", "solve.hip", "sources.hip",
+           "multi.hip", "pagrid.hip", "scramble.hip"]
 TRIG_SOURCES = ["trig_host.cpp", "ehss_trig.h", "ehss_trig_table.h"]
 
 FLAGS = [

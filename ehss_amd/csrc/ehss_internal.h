@@ -249,6 +249,7 @@ struct ehss_scene {
     // solve workspace (grow-only)
     ehss::DevBuf ray_info, ray_contrib, partials, hist, ledger, counters, queue, queue2, ray_times, tables,
         drain_stripes, custom_groups;
+    ehss::DevBuf src_weights, src_cdf;  // alternative-source solves: per-ray weights, searchsorted table
     ehss::BuildWork build_work;  // tree-build work arrays, reused by ehss_scene_rebuild
     // continuation of the last solve (EHSS_FLAG_CONTINUABLE): two queues of
     // saved unresolved rays (ping-pong) and their words {count, head} x 2

@@ -913,4 +913,13 @@ __device__ __forceinline__ void finalize_block(const Partial* partials, int64_t 
     }
 }
 
+// Traces the ray states a kernel queued (one replicate; words[0] = their count,
+// written on the device, words[1..3] = head, second-stage count and head, all
+// zero on entry) to their ends on stream s with the fused solve's drain
+// schedule for `capacity` rays and `flags` (A.group lanes per ray for the
+// first-stage queries, then a whole wave per ray), recording each ray at its
+// slot through A.  Defined in solve.hip.
+int drain_queued(ehss_scene* sc, const SolveArgs& A, const RayState* queue, uint32_t* words, int64_t capacity,
+                 unsigned flags, hipStream_t s);
+
 }  // namespace ehss

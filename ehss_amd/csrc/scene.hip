@@ -31,7 +31,7 @@ int64_t ehss_scene::device_bytes() const {
     const ehss::DevBuf* bufs[] = {&centers, &radii, &sph,   &groups, &offsets, &eids,    &nodes,   &nodes32,
                                   &sph32,   &pa,    &pa_ids, &ray_info, &ray_contrib, &partials, &hist,  &ledger,
                                   &counters, &queue, &frontier, &sphs, &leaves32, &ray_times, &tables, &queue2,
-                                  &drain_stripes};
+                                  &drain_stripes, &src_weights, &src_cdf};
     int64_t total = 0;
     for (const ehss::DevBuf* b : bufs) total += (int64_t)b->bytes;
     return total;

@@ -992,6 +992,36 @@ int solve_collect(ehss_scene* sc, ehss_totals* totals, int64_t* histogram, doubl
 
 }  // namespace
 
+namespace ehss {
+
+int drain_queued(ehss_scene* sc, const SolveArgs& A, const RayState* queue, uint32_t* words, int64_t capacity,
+                 unsigned flags, hipStream_t s) {
+    const int stage1 = stage1_queries(flags, capacity);
+    RayState* spill = nullptr;
+    if (stage1 > 0 && A.group > 1) {
+        EHSS_TRY(sc->queue2.reserve(sizeof(RayState) * capacity));
+        spill = sc->queue2.as<RayState>();
+    }
+    const size_t lds = drain_lds_bytes(sc->view);
+    const dim3 dgrid((unsigned)drain_blocks(capacity));
+    if (lds <= kDrainLdsLimit) EHSS_TRY(set_drain_lds_limit());
+    auto drain = [&](const RayState* in, const uint32_t* count, uint32_t* head, int group, int max_q, RayState* out,
+                     uint32_t* out_count) {
+        if (lds <= kDrainLdsLimit)
+            hipLaunchKernelGGL(k_drain<1>, dgrid, dim3(256), lds, s, sc->view, A, 1, in, count, head, group, max_q,
+                               out, out_count, nullptr);
+        else
+            hipLaunchKernelGGL(k_drain<0>, dgrid, dim3(256), 0, s, sc->view, A, 1, in, count, head, group, max_q,
+                               out, out_count, nullptr);
+    };
+    drain(queue, words, words + 1, A.group, spill ? stage1 : 0, spill, words + 2);
+    if (spill) drain(spill, words + 2, words + 3, kWave, 0, nullptr, nullptr);
+    EHSS_HIP(hipGetLastError());
+    return EHSS_OK;
+}
+
+}  // namespace ehss
+
 extern "C" {
 
 int ehss_cross_map(const ehss_sobol* sobols, int reps, int power, uint32_t* out) {

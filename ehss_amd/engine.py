@@ -439,6 +439,81 @@ class GPUScene:
         L.call("ehss_source_rays", self.handle, ct.byref(sobol), int(power), L.ptr(u), L.ptr(o), L.ptr(d))
         return dict(u=u, origins=o, directions=d)
 
+    # -- alternative incidence sources, generated and traced on the device --------
+    def solve_source(self, source: L.Source, sobol: L.Sobol5, power: int, cap: int = 64,
+                     histogram: bool = False, keep_rays: bool = False) -> dict:
+        """One device solve of 2^power states of an alternative source
+        (ehss_solve_source); see source_params()."""
+        totals = L.Totals()
+        n = 1 << int(power)
+        hist = np.zeros(cap + 1, np.int64) if histogram else None
+        info = np.empty(n, np.uint32) if keep_rays else None
+        contrib = np.empty(n, np.float64) if keep_rays else None
+        L.call("ehss_solve_source", self.handle, ct.byref(source), ct.byref(sobol), int(power), int(cap),
+               ct.byref(totals), L.ptr(hist), L.ptr(info), L.ptr(contrib))
+        out = totals.as_dict()
+        if histogram:
+            out["histogram"] = hist
+        if keep_rays:  # per ray, in ray order
+            flags = info >> np.uint32(20)
+            out["rays_out"] = dict(bounces=(info & np.uint32(0xFFFFF)).astype(np.int64),
+                                   escaped=(flags & L.RAY_ESCAPED) != 0,
+                                   unresolved=(flags & L.RAY_UNRESOLVED) != 0,
+                                   grazing=(flags & L.RAY_GRAZING) != 0, contributions=contrib)
+        return out
+
+    def source_states(self, source: L.Source, sobol: L.Sobol5, power: int) -> dict:
+        """The states the device generates for an alternative source (ehss_source_states)."""
+        n = 1 << int(power)
+        o, d, inc = np.empty((n, 3)), np.empty((n, 3)), np.empty((n, 3))
+        w, own, cnt = np.empty(n), np.empty(n, np.int64), np.empty(n, np.int64)
+        blocked = np.empty(n, np.uint8)
+        L.call("ehss_source_states", self.handle, ct.byref(source), ct.byref(sobol), int(power), L.ptr(o),
+               L.ptr(d), L.ptr(inc), L.ptr(w), L.ptr(own), L.ptr(cnt), L.ptr(blocked))
+        return dict(origins=o, directions=d, incoming=inc, weights=w, owners=own, bounces=cnt,
+                    blocked=blocked.astype(bool))
+
+
+def scramble5(seed: int, dim: int = 5) -> L.Sobol5:
+    """ehss_sobol5 of scipy.stats.qmc.Sobol(dim, scramble=True, seed) for dim 4 or 5
+    (dimensions past `dim` are zero)."""
+    if dim not in (4, 5):
+        raise ValueError("dim must be 4 or 5")
+    sv, shift = scramble_matrices(seed, dim)
+    s = L.Sobol5()
+    for k in range(dim):
+        for j in range(BITS):
+            s.sv[k][j] = int(sv[k, j])
+        s.shift[k] = int(shift[k])
+    return s
+
+
+def source_params(kind: str, center=None, rotation=None, axes=None, area: float = 0.0, cdf=None) -> L.Source:
+    """ehss_source for kind 'box' or 'ellipsoid' (a CompactBoundary's center,
+    rotation, axes) or 'reverse-pa' (Spheres.area and the searchsorted table
+    cumsum(r^2) / sum(r^2), inside_out_pa.py:79).  The table array must stay
+    alive while the source is used."""
+    kinds = {"box": L.SOURCE_BOX, "ellipsoid": L.SOURCE_ELLIPSOID, "reverse-pa": L.SOURCE_REVERSE_PA}
+    if kind not in kinds:
+        raise ValueError("source kind must be box, ellipsoid or reverse-pa")
+    s = L.Source()
+    s.kind = kinds[kind]
+    if kind in ("box", "ellipsoid"):
+        c = np.asarray(center, np.float64).reshape(3)
+        R = np.ascontiguousarray(np.asarray(rotation, np.float64).reshape(3, 3))
+        a = np.asarray(axes, np.float64).reshape(3)
+        for k in range(3):
+            s.center[k] = float(c[k])
+            s.axes[k] = float(a[k])
+        for k in range(9):
+            s.rotation[k] = float(R.reshape(9)[k])
+    else:
+        table = np.ascontiguousarray(cdf, np.float64)
+        s.area = float(area)
+        s.cdf = table.ctypes.data
+        s._keep = table  # noqa: SLF001 -- holds the table for the call
+    return s
+
 
 class _MultiSolve:
     """Shared wrapper of the multi-structure entry points (ehss_<prefix>_*):

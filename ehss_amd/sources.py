@@ -1,9 +1,10 @@
 """Alternative incidence sources of the EHSS estimator, for the device engine.
 
-Host-side generation of the source states (as in the reference, where these
-sources are NumPy code); the states are then traced by the device
-(PreparedStaticGPUEHSS.engine.trace) and, for the PA sources, tested by the
-device any-hit kernel (SphereIndex.query).
+Two forms.  The host form generates the source states with NumPy (as the
+reference does) for the device to trace (PreparedStaticGPUEHSS.engine.trace)
+and, for the PA sources, test (SphereIndex.query).  The fused form
+(CompactBoundary.solve, reverse_pa_solve) generates the same states on the
+device and traces and reduces them there.
 
 Mirrors (reference file:line):
   ball_metric_bounds       pa_ellipsoidal.py:15-66
@@ -13,6 +14,8 @@ Mirrors (reference file:line):
   inside_out               inside_out_pa.py:115-118 (any-hit on the device BVH instead
                            of the all-pairs loop; same predicate, same flags)
   reverse_pa_source        ehss_state.py:55-69
+  CompactBoundary.solve,   the same sources generated, traced and reduced on the
+  reverse_pa_solve         device (ehss_solve_source), nothing per ray crossing PCIe
 
 Sobol points come from engine.sobol_points, which reproduces
 scipy.stats.qmc.Sobol(d, scramble=True, seed).random_base2(m) bit for bit
@@ -185,6 +188,20 @@ class CompactBoundary:
         return EHSSSource(origins, d, d.copy(), area / n, np.full(n, -1), np.zeros(n, dtype=np.int64),
                           f"isotropic projected {self.kind}")
 
+    def solve(self, scene, power: int, seed: int, cap: int = 64, keep_rays: bool = False) -> dict:
+        """This source generated, traced and reduced on the device in one call
+        (ehss_solve_source): the states of source(power, seed), traced against
+        `scene` (a GPUScene of the same structure), with the EHSSResult readout
+        and per-ray weights.  Nothing crosses PCIe but the totals.  The 'sphere'
+        kind is the fused solve's impact disk (the same rays as
+        external_source when the boundary is the fitted one)."""
+        from .engine import scramble, scramble5, source_params
+        if self.kind == "sphere":
+            return scene.solve(scramble(seed), power, cap, pa=False)
+        params = source_params(self.kind, self.center, self.rotation, self.axes)
+        return scene.solve_source(params, scramble5(seed, 5 if self.kind == "box" else 4), power, cap,
+                                  keep_rays=keep_rays)
+
 
 def sample_source(spheres: Spheres, power: int, seed: int) -> Source:
     """Area-selected sphere surface points with cosine-hemisphere directions
@@ -240,3 +257,16 @@ def reverse_pa_source(spheres: Spheres, power: int, seed: int, index=None) -> EH
     if not np.isclose(result.weights.sum(), pa, rtol=1e-13, atol=1e-13):
         raise AssertionError("PA source measure changed during filtering")
     return result
+
+
+def reverse_pa_solve(scene, spheres: Spheres, power: int, seed: int, cap: int = 64, keep_rays: bool = False) -> dict:
+    """reverse_pa_source (ehss_state.py:55-69) on the device in one call: the
+    area-selected surface samples, their inside-out any-hit test, the reflected
+    unblocked states, their trace against `scene` (a GPUScene of `spheres`) and
+    the readout (ehss_solve_source).  pa is the inside-out PA
+    (inside_out_pa.py:66-67); pa_from_ehss the result's PA."""
+    from .engine import scramble5, source_params
+    w = spheres.radii ** 2
+    cdf = np.cumsum(w) / w.sum()  # inside_out_pa.py:79
+    params = source_params("reverse-pa", area=spheres.area, cdf=cdf)
+    return scene.solve_source(params, scramble5(seed, 5), power, cap, keep_rays=keep_rays)

@@ -340,6 +340,68 @@ int ehss_source_rays(ehss_scene* scene, const ehss_sobol* sobol, int power, doub
                      double* directions);
 
 /* ---------------------------------------------------------------------------
+ * Alternative incidence sources, generated, traced and reduced on the device.
+ *
+ *   EHSS_SOURCE_BOX        CompactBoundary(kind='box').source (ehss_compact_source.py:56-70):
+ *                          directions uniform on S2, a face chosen in proportion to
+ *                          its projected area, a uniform point on it; weight =
+ *                          projected area / n.  5-D net.
+ *   EHSS_SOURCE_ELLIPSOID  CompactBoundary(kind='ellipsoid').source (ehss_compact_source.py:71-84):
+ *                          a uniform point of the ellipsoid's projection; weight =
+ *                          projected area / n.  4-D net (dimensions 0-3 of the table).
+ *   EHSS_SOURCE_REVERSE_PA reverse_pa_source (ehss_state.py:55-69): sample_source's
+ *                          area-selected surface points and cosine hemisphere
+ *                          (inside_out_pa.py:72-90, 5-D net), the inside-out any-hit
+ *                          test against every other sphere (inside_out_pa.py:115-118),
+ *                          then the unblocked states reflected once (bounce count 1,
+ *                          last atom = owner), weight = area / (4 n).
+ * The scrambled net is SciPy's qmc.Sobol(dims, scramble=True, seed) as direction
+ * numbers and shift (dims 5, or 4 for the ellipsoid: sv[4] and shift[4] unused).
+ * Ray i of the net is point i of random_base2(power).  Parameters computed from
+ * the structure with NumPy in the reference (fit, area, the searchsorted table)
+ * are inputs, so their bits are the caller's.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    uint32_t sv[5][32];
+    uint32_t shift[5];
+} ehss_sobol5;
+
+#define EHSS_SOURCE_BOX 1
+#define EHSS_SOURCE_ELLIPSOID 2
+#define EHSS_SOURCE_REVERSE_PA 3
+
+typedef struct {
+    int32_t kind;
+    int32_t pad;
+    double center[3];    /* box / ellipsoid: CompactBoundary.center */
+    double rotation[9];  /* row-major CompactBoundary.rotation (local = d @ rotation) */
+    double axes[3];      /* half-widths (box) or semi-axes (ellipsoid) */
+    double area;         /* reverse PA: Spheres.area = 4 pi r.r (inside_out_pa.py:42) */
+    const double* cdf;   /* reverse PA: host array, n atoms: cumsum(r^2) / sum(r^2) (inside_out_pa.py:79) */
+} ehss_source;
+
+/* Traces the source's 2^power states against the scene and reduces them as
+ * EHSSResult does (ehss_state.py:84-103) with per-ray weights: omega = sum w
+ * (1 - u.v) over escaped rays, residual_mass = sum w over unresolved rays,
+ * pa_from_ehss = sum w over rays with bounces > 0 or unresolved (fixed-order
+ * compensated sums).  weight = the sum of all weights.  Reverse PA: pa =
+ * area * mean(unblocked) / 4 (inside_out_pa.py:66-67), pa_hits = blocked
+ * samples (not traced; their records are zero).  Optional per-ray outputs in ray
+ * order: info (bounces | flags << 20, as ehss_scene_last_rays) and contrib
+ * (weighted); histogram: cap + 1 bounce counts.  The states are traced by the
+ * fused solve's queue drain (FP32 filter, FP64 fix-up: per-ray results equal
+ * the FP64 trace's except on rays flagged grazing).  Synchronous. */
+int ehss_solve_source(ehss_scene* scene, const ehss_source* source, const ehss_sobol5* sobol, int power, int cap,
+                      ehss_totals* totals, int64_t* histogram, uint32_t* info, double* contrib);
+/* The source's states as generated on the device (parity tests): origins,
+ * directions (outgoing), incoming: n x 3; weights, owners (last atom, -1 for
+ * external states), bounces (initial count), blocked (reverse PA; 0 otherwise):
+ * n each, ray order; blocked states have weight 0.  Any output may be NULL. */
+int ehss_source_states(ehss_scene* scene, const ehss_source* source, const ehss_sobol5* sobol, int power,
+                       double* origins, double* directions, double* incoming, double* weights, int64_t* owners,
+                       int64_t* bounces, uint8_t* blocked);
+
+/* ---------------------------------------------------------------------------
  * Finite-grid PA: the CLI's adaptive method (ccs_cli.py:142-160; the union
  * count of pa_sparse_blocks.py:125-174 / pa_dyadic_bitmask.py:86-97).
  * ------------------------------------------------------------------------- */

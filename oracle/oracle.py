@@ -16,6 +16,15 @@ Pinning summary (see tests/test_oracle_*.py):
                                   radius R sqrt(u2)), planted swaps failing them, and
                                   the single-sphere pi r^2 (tests/test_oracle_source.py).
   readout                         math.fsum of the same contributions; clip bounds.
+  compact_source (box, ellipsoid) per-ray closed forms: box area = sum |d_k| x face area,
+                                  entry on the face facing the ray, inside its rectangle;
+                                  ellipsoid area = pi abc |diag(1/axes) R^T d|; every ray
+                                  enters the body; mean weight = surface / 4 (Cauchy)
+                                  (tests/test_oracle_compact_source.py).  Which of u3/u4
+                                  spans which in-face axis follows the reference text:
+                                  parity unpinned beyond it (no reference fixture).
+  sample_source / reverse_pa_source  the reference's single-sphere test restated (all
+                                  rays escape after one bounce, omega = PA = pi r^2).
   trace / static_tree / pa_*      through ehss_oracle.c (see its header).
 """
 from __future__ import annotations
@@ -242,6 +251,49 @@ def sample_source(centers, radii, power, seed):
     area = float(4 * np.pi * np.dot(radii, radii))
     return dict(origins=np.ascontiguousarray(origins), directions=np.ascontiguousarray(directions),
                 owners=ids.astype(np.int64), area=area)
+
+
+def compact_source(kind, center, rotation, axes, power, seed):
+    """ehss_compact_source.py:44-87, kinds 'box' and 'ellipsoid': directions
+    uniform on S2, a uniform point of the body's projection (box: a face chosen
+    in proportion to its projected area), weight = projected area / n."""
+    center, rotation, axes = _f64(center), _f64(rotation).reshape(3, 3), _f64(axes)
+    sv, shift = sobol_scramble(seed, dim=5 if kind == "box" else 4)
+    u = sobol_points(sv, shift, power)
+    d = unit_sphere(u[:, :2])
+    n = len(d)
+    if kind == "box":
+        local_d = d @ rotation
+        face_areas = 4 * np.prod(axes) / axes
+        projected = np.abs(local_d) * face_areas
+        area = projected.sum(axis=1)
+        face = (u[:, 2, None] * area[:, None] >= np.cumsum(projected, axis=1)).sum(axis=1)
+        p = np.empty((n, 3))
+        for axis in range(3):
+            selected = face == axis
+            p[selected, axis] = -np.sign(local_d[selected, axis]) * axes[axis]
+            other = [k for k in range(3) if k != axis]
+            p[np.ix_(selected, other)] = (2 * u[selected, 3:5] - 1) * axes[other]
+        origins = center + p @ rotation.T - 1e-10 * max(1., axes.max()) * d
+    elif kind == "ellipsoid":
+        t, b = tangent_frames(d)
+        lt, lb = (t @ rotation) * axes, (b @ rotation) * axes
+        g00 = np.sum(lt * lt, axis=1)
+        g01 = np.sum(lt * lb, axis=1)
+        g11 = np.sum(lb * lb, axis=1)
+        l00 = np.sqrt(g00)
+        l10 = g01 / l00
+        l11 = np.sqrt(np.maximum(0., g11 - l10 * l10))
+        rho, phi = np.sqrt(u[:, 2]), 2 * np.pi * u[:, 3]
+        x, y = rho * np.cos(phi), rho * np.sin(phi)
+        offset = (l00 * x)[:, None] * t + (l10 * x + l11 * y)[:, None] * b
+        extent = np.linalg.norm((d @ rotation) * axes, axis=1)
+        origins = center + offset - (extent * (1 + 1e-10))[:, None] * d
+        area = np.pi * l00 * l11
+    else:
+        raise ValueError("kind must be box or ellipsoid")
+    return dict(origins=np.ascontiguousarray(origins), directions=np.ascontiguousarray(d), incoming=d.copy(),
+                weights=area / n, last_atom=np.full(n, -1, np.int64), initial_bounces=np.zeros(n, np.int64))
 
 
 def reverse_pa_source(centers, radii, power, seed):
