@@ -569,6 +569,26 @@ def main():
                              pa_mean=float(np.mean([v["pa"] for v in vals])),
                              ms_per_repeat=float(np.mean([v["kernel_ms"] for v in vals])))
 
+        say("alternative sources on the device (globule650, 2^20 states, cap 64)")
+        from ehss_amd.engine import scramble5, source_params
+        lo_b = np.min(centers - radii[:, None], axis=0)
+        hi_b = np.max(centers + radii[:, None], axis=0)
+        w2 = radii ** 2
+        alt = {}
+        with GPUScene(centers, radii) as s_alt:
+            for name, params, dim in (
+                    ("box", source_params("box", (lo_b + hi_b) / 2, np.eye(3), (hi_b - lo_b) / 2 * (1 + 1e-9)), 5),
+                    ("reverse_pa", source_params("reverse-pa", area=float(4 * np.pi * np.dot(radii, radii)),
+                                                 cdf=np.cumsum(w2) / w2.sum()), 5)):
+                s_alt.solve_source(params, scramble5(1, dim), POWER, CAP)  # warm-up (workspace)
+                t = time.perf_counter()
+                outs = [s_alt.solve_source(params, scramble5(1 + k, dim), POWER, CAP) for k in range(5)]
+                el = time.perf_counter() - t
+                alt[name] = dict(states_per_s=5 * (1 << POWER) / el, omega_mean=float(np.mean([o["omega"] for o in outs])),
+                                 kernel_ms=float(np.mean([o["kernel_ms"] for o in outs])),
+                                 pa=float(np.mean([o["pa"] for o in outs])) if name == "reverse_pa" else None)
+        extras["alt_sources"] = alt
+
         say("C4: 18 structures x 2^20 rays, autotuned plan under hipGraph")
         extras["batched18"] = batched18(molecules, scramble, replicate_seed)
 
