@@ -80,7 +80,7 @@ class SceneInfo(ct.Structure):
     _fields_ = [("atoms", ct.c_int64), ("groups", ct.c_int64), ("nodes", ct.c_int64), ("pa_nodes", ct.c_int64),
                 ("leaf_size", ct.c_int32), ("max_depth", ct.c_int32), ("center", ct.c_double * 3),
                 ("radius", ct.c_double), ("scale", ct.c_double), ("device_bytes", ct.c_int64),
-                ("build_path", ct.c_int32), ("reserved", ct.c_int32)]
+                ("build_path", ct.c_int32), ("packet_nodes", ct.c_int32)]
 
 
 BUILD_GENERAL, BUILD_SHAPED = 0, 1

@@ -274,6 +274,9 @@ int upload_compat(CompatScene& cs, int64_t nn, const double* lo, const double* h
         EHSS_TRY(launch_make_filter(v, cs.nodes32.as<Node32>(), cs.sph32.as<float4>(), cs.sphs.as<double4>(),
                                     nullptr, s));
         v.nodes32 = cs.nodes32.as<Node32>();
+        v.pk_nodes32 = v.nodes32;
+        v.pk_offsets = v.offsets;
+        v.pk_nodes = v.n_nodes;
         v.sph32 = cs.sph32.as<float4>();
         v.sphs = cs.sphs.as<double4>();
     }

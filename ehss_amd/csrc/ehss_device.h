@@ -22,6 +22,54 @@ constexpr int kFrontier = 8 * kWave;              // cooperative-query subtree r
 // scene measured on MI355X, up to 8000 groups (24000 atoms).
 constexpr int32_t kBruteGroups = 16384;
 
+// Leaves of the static group tree (the halving tree over k groups: a node over
+// groups [a, b) has children over [a, a + (b - a) / 2) and the rest) when every
+// subtree of at most `cut` groups is made one leaf.  The sizes at one depth are
+// q and q + 1; their counts follow the halving in closed form.
+__host__ __device__ inline int32_t cut_leaves(int32_t k, int32_t cut) {
+    int64_t q = k, cq = 1, cq1 = 0, leaves = 0;
+    for (;;) {
+        if (q <= cut) {
+            leaves += cq;
+            cq = 0;
+        }
+        if (q + 1 <= cut) {
+            leaves += cq1;
+            cq1 = 0;
+        }
+        if (cq + cq1 == 0) break;
+        const int64_t h = q / 2;  // q = 2h -> (h, h), 2h + 1 -> (h, h + 1), 2h + 2 -> (h + 1, h + 1)
+        const int64_t nq = (q % 2 == 0) ? 2 * cq + cq1 : cq;
+        const int64_t nq1 = (q % 2 == 0) ? cq1 : cq + 2 * cq1;
+        q = h;
+        cq = nq;
+        cq1 = nq1;
+    }
+    return (int32_t)leaves;
+}
+
+// Nodes of that cut tree (a full binary tree).
+__host__ __device__ inline int32_t cut_nodes(int32_t k, int32_t cut) { return 2 * cut_leaves(k, cut) - 1; }
+
+// Groups per leaf of the packet tree, the static tree cut as above, that the
+// packet (wave-uniform) queries walk: at most kPacketLeafAtoms / (atoms per
+// group) groups per leaf.  Any tree whose boxes enclose their spheres gives the
+// same answers under the FP32 filter.  Measured at 16 x 2^20 rays, cap 64: on
+// the 650-atom globule static trees of leaf size 2 / 4 / 8 / 16 run 10.72 /
+// 10.10 / 9.69 / 10.28 ms per launch, and the leaf-4 tree cut at 4 / 6 / 10
+// atoms runs 10.07 / 9.71 / 9.62 (10.06 uncut); at 10 atoms C60, the
+// 2000-atom ellipsoid and globules of 272, 829 and 4241 atoms run 4.19, 13.17,
+// 5.65, 8.78 and 24.67 ms against 4.28, 13.24, 5.73, 8.76 and 24.89 uncut.
+#ifndef EHSS_PACKET_LEAF_ATOMS
+#define EHSS_PACKET_LEAF_ATOMS 10
+#endif
+constexpr int32_t kPacketLeafAtoms = EHSS_PACKET_LEAF_ATOMS;
+__host__ __device__ inline int32_t packet_cut(int32_t n_atoms, int32_t n_groups) {
+    if (n_groups <= 1 || n_atoms <= 0) return 1;
+    const int64_t c = (int64_t)kPacketLeafAtoms * n_groups / n_atoms;
+    return c < 1 ? 1 : (c > 16 ? 16 : (int32_t)c);
+}
+
 // std::max / std::min semantics: the first argument wins unless strictly beaten.
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }

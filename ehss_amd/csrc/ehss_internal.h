@@ -102,6 +102,13 @@ struct SceneView {
     const int32_t* pa_ids = nullptr;  // PA id slots
     const int32_t* frontier = nullptr; // roots of <= kFrontier disjoint subtrees covering every leaf
     int32_t n_frontier = 0;
+    // Packet tree (the static tree cut at packet_cut groups, FP32 boxes, the
+    // nodes32 encoding): the tree the packet queries walk; its leaf j holds
+    // slots pk_offsets[j] .. pk_offsets[j + 1] - 1.  Equal to nodes32 / offsets
+    // when the cut is 1.
+    const Node32* pk_nodes32 = nullptr;
+    const int32_t* pk_offsets = nullptr;
+    int32_t pk_nodes = 0;
     double cx = 0, cy = 0, cz = 0;    // filter origin (scene centre)
     double scale = 1;                 // bound on |coordinates - centre| and radii
     float m_box = 0, m_p = 0, m_disc = 0;  // FP32 filter margins
@@ -116,6 +123,10 @@ void set_filter_margins(SceneView& v);
 // per-group leaf boxes.
 int launch_make_filter(const SceneView& v, Node32* nodes32, float4* sph32, double4* sphs, Node32* leaves32,
                        hipStream_t s);
+
+// Packet tree of a scene whose nodes32 and offsets are built (cut > 1):
+// cut_nodes(n_groups, cut) nodes and cut_leaves(n_groups, cut) + 1 offsets.
+int launch_packet_tree(const SceneView& v, int32_t cut, Node32* pk_nodes32, int32_t* pk_offsets, hipStream_t s);
 
 // Per-ray trace of caller rays (AoS p, d as in ehss_trace).
 struct TraceIO {
@@ -222,7 +233,7 @@ struct ehss_scene {
     double center[3] = {0, 0, 0};
     double radius = 0, scale = 1;
     ehss::DevBuf centers, radii, sph, groups, offsets, eids, nodes, nodes32, sph32, sphs, leaves32, pa, pa_ids,
-        frontier;
+        frontier, pk_nodes32, pk_offsets;
     ehss::SceneView view;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_done = nullptr;

@@ -263,9 +263,9 @@ __device__ __forceinline__ double4 load_sphere64(const double4* base, int32_t k)
     return make_double4(f[0], f[1], f[2], f[3]);
 }
 
-// kLds: the view's nodes32, offsets, ids and sph32 point into LDS (a block's
-// staged copy); every lane then reads the same LDS address (a broadcast)
-// instead of issuing a scalar load.
+// kLds: the view's arrays that the packet queries read point into LDS (a
+// block's staged copy); every lane then reads the same LDS address (a
+// broadcast) instead of issuing a scalar load.
 typedef const __attribute__((address_space(3))) float* LFloat;
 typedef const __attribute__((address_space(3))) int32_t* LInt32;
 
@@ -291,17 +291,18 @@ __device__ __forceinline__ float4 load_sphere32_lds(const float4* base, int32_t 
     return make_float4(f[0], f[1], f[2], f[3]);
 }
 
-// Wave-uniform reads of the scene for the packet queries: scalar loads, or LDS
-// broadcasts when the block has staged the scene (kLds).
+// Wave-uniform reads of the scene for the packet queries (the packet tree, its
+// leaf offsets, slot ids, FP32 spheres): scalar loads, or LDS broadcasts when
+// the block has staged the scene (kLds).
 template <int kLds>
 struct PacketMem {
     static __device__ __forceinline__ Node32 node(const SceneView& S, int32_t i) {
-        if constexpr (kLds) return load_node_lds(S.nodes32, i);
-        else return load_node(S.nodes32, i);
+        if constexpr (kLds) return load_node_lds(S.pk_nodes32, i);
+        else return load_node(S.pk_nodes32, i);
     }
     static __device__ __forceinline__ int32_t offset(const SceneView& S, int32_t g) {
-        if constexpr (kLds) return __builtin_amdgcn_readfirstlane(((LInt32)S.offsets)[g]);
-        else return ((CInt32)S.offsets)[g];
+        if constexpr (kLds) return __builtin_amdgcn_readfirstlane(((LInt32)S.pk_offsets)[g]);
+        else return ((CInt32)S.pk_offsets)[g];
     }
     static __device__ __forceinline__ int32_t id(const SceneView& S, int32_t k) {
         if constexpr (kLds) return __builtin_amdgcn_readfirstlane(((LInt32)S.ids)[k]);
@@ -326,7 +327,7 @@ __device__ __forceinline__ void nearest_packet(const SceneView& S, const V3& o, 
     auto sphere_at = [&](int32_t k) { return M::sphere(S, k); };
     float bestf = INFINITY;
     int32_t node = 0;
-    const int32_t nn = S.n_nodes;
+    const int32_t nn = S.pk_nodes;
     while (node < nn) {
         const Node32 n = node_at(node);
         boxes += live ? 1 : 0;

@@ -589,15 +589,34 @@ __device__ __forceinline__ void solve_ray(const SceneView& S, const SolveArgs& A
     solve_ray_at<kMode, kPA, kLds>(S, A, sob, valid, g, t, ctr, queue, queue_count, job);
 }
 
+__host__ __device__ inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// LDS bytes of the arrays every staged view holds.
+__host__ __device__ inline size_t staged_base_bytes(const SceneView& v) {
+    return sizeof(Node32) * (v.n_nodes + v.n_groups) + sizeof(float4) * v.n_atoms +
+           sizeof(int32_t) * (v.n_groups + 1) + sizeof(int32_t) * v.n_atoms;
+}
+
 // Copies the scene arrays the FP32 filter reads (tree, leaf boxes, leaf
-// offsets, slot ids, FP32 spheres) into the block's LDS and returns the view
-// pointing there.  All threads of the block call it.
+// offsets, slot ids, FP32 spheres, and for kPacket the packet tree and its leaf
+// offsets) into the block's LDS and returns the view pointing there.  All
+// threads of the block call it.
+template <bool kPacket = false>
 __device__ __forceinline__ SceneView stage_scene(const SceneView& S, unsigned char* lds) {
     Node32* nodes = reinterpret_cast<Node32*>(lds);
     Node32* leaves = nodes + S.n_nodes;
     float4* sph = reinterpret_cast<float4*>(leaves + S.n_groups);
     int32_t* offsets = reinterpret_cast<int32_t*>(sph + S.n_atoms);
     int32_t* ids = offsets + (S.n_groups + 1);
+    // Packet tree, when it is not the static tree itself.
+    const bool own_pk = kPacket && S.pk_nodes32 != S.nodes32;
+    Node32* pk = reinterpret_cast<Node32*>(lds + align_up(staged_base_bytes(S), alignof(Node32)));
+    int32_t* pk_off = reinterpret_cast<int32_t*>(pk + S.pk_nodes);
+    if (own_pk) {
+        const int32_t leaves_pk = (S.pk_nodes + 1) / 2;
+        for (int32_t k = threadIdx.x; k < S.pk_nodes; k += blockDim.x) pk[k] = S.pk_nodes32[k];
+        for (int32_t k = threadIdx.x; k <= leaves_pk; k += blockDim.x) pk_off[k] = S.pk_offsets[k];
+    }
     for (int32_t k = threadIdx.x; k < S.n_nodes; k += blockDim.x) nodes[k] = S.nodes32[k];
     for (int32_t k = threadIdx.x; k < S.n_groups; k += blockDim.x) leaves[k] = S.leaves32[k];
     for (int32_t k = threadIdx.x; k < S.n_atoms; k += blockDim.x) {
@@ -612,13 +631,21 @@ __device__ __forceinline__ SceneView stage_scene(const SceneView& S, unsigned ch
     L.sph32 = sph;
     L.offsets = offsets;
     L.ids = ids;
+    if (own_pk) {
+        L.pk_nodes32 = pk;
+        L.pk_offsets = pk_off;
+    } else {
+        L.pk_nodes32 = nodes;
+        L.pk_offsets = offsets;
+    }
     return L;
 }
 
-// LDS bytes stage_scene uses for a scene.
-__host__ __device__ inline size_t staged_scene_bytes(const SceneView& v) {
-    return sizeof(Node32) * (v.n_nodes + v.n_groups) + sizeof(float4) * v.n_atoms +
-           sizeof(int32_t) * (v.n_groups + 1) + sizeof(int32_t) * v.n_atoms;
+// LDS bytes stage_scene<kPacket> uses for a scene.
+__host__ __device__ inline size_t staged_scene_bytes(const SceneView& v, bool packet = false) {
+    if (!packet || v.pk_nodes32 == v.nodes32) return staged_base_bytes(v);
+    return align_up(staged_base_bytes(v), alignof(Node32)) + sizeof(Node32) * v.pk_nodes +
+           sizeof(int32_t) * ((v.pk_nodes + 1) / 2 + 1);
 }
 
 // Persistent per-lane drain of queued rays.  Idle lanes refill from the queue

@@ -49,7 +49,7 @@ __global__ void EHSS_SOLVE_KERNEL k_plan_solve(const PlanJob* __restrict__ jobs)
     const PlanJob& J = jobs[j];
     const int64_t n = J.args.n;
     if ((int64_t)blockIdx.x * blockDim.x >= n) return;  // whole block: past this job's rays
-    const SceneView S = kLds ? stage_scene(J.view, lds) : J.view;
+    const SceneView S = kLds ? stage_scene<true>(J.view, lds) : J.view;
     const SolveArgs A = J.args;
     uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
     solve_ray<kMode, kPA, kLds>(S, A, (CSobol)&J.sob, (CMap)&J.map, blockIdx.x * blockDim.x + threadIdx.x, ctr,
@@ -292,7 +292,7 @@ int fill_plan(ehss_plan* p, int n_jobs, ehss_scene* const* scenes, const ehss_so
     p->lds_bytes = 0;
     if ((flags & EHSS_FLAG_LDS_PACKETS) && p->variant >= 2) {
         size_t most = 0;
-        for (int j = 0; j < n_jobs; ++j) most = std::max(most, staged_scene_bytes(scenes[j]->view));
+        for (int j = 0; j < n_jobs; ++j) most = std::max(most, staged_scene_bytes(scenes[j]->view, true));
         if (most <= kPlanLdsLimit) p->lds_bytes = most;
     }
     // The plan copied each scene's views (device pointers, counts, disk):

@@ -31,7 +31,7 @@ int64_t ehss_scene::device_bytes() const {
     const ehss::DevBuf* bufs[] = {&centers, &radii, &sph,   &groups, &offsets, &eids,    &nodes,   &nodes32,
                                   &sph32,   &pa,    &pa_ids, &ray_info, &ray_contrib, &partials, &hist,  &ledger,
                                   &counters, &queue, &frontier, &sphs, &leaves32, &ray_times, &tables, &queue2,
-                                  &drain_stripes, &src_weights, &src_cdf};
+                                  &drain_stripes, &src_weights, &src_cdf, &pk_nodes32, &pk_offsets};
     int64_t total = 0;
     for (const ehss::DevBuf* b : bufs) total += (int64_t)b->bytes;
     return total;
@@ -178,6 +178,19 @@ int build_into(ehss_scene* sc, int64_t n, const double* centers, const double* r
     v.sph32 = sc->sph32.as<float4>();
     v.sphs = sc->sphs.as<double4>();
     v.leaves32 = sc->leaves32.as<Node32>();
+    const int32_t cut = packet_cut(v.n_atoms, v.n_groups);
+    if (cut > 1) {
+        v.pk_nodes = cut_nodes(v.n_groups, cut);
+        EHSS_TRY(sc->pk_nodes32.reserve(sizeof(Node32) * v.pk_nodes));
+        EHSS_TRY(sc->pk_offsets.reserve(sizeof(int32_t) * (cut_leaves(v.n_groups, cut) + 1)));
+        EHSS_TRY(launch_packet_tree(v, cut, sc->pk_nodes32.as<Node32>(), sc->pk_offsets.as<int32_t>(), s));
+        v.pk_nodes32 = sc->pk_nodes32.as<Node32>();
+        v.pk_offsets = sc->pk_offsets.as<int32_t>();
+    } else {
+        v.pk_nodes = v.n_nodes;
+        v.pk_nodes32 = v.nodes32;
+        v.pk_offsets = v.offsets;
+    }
     EHSS_TRY(put_frontier(sc, s));
     // Work on other streams that reads the scene (plans, caller streams) waits for this.
     EHSS_HIP(hipEventRecord(sc->ev_built, s));
@@ -290,7 +303,7 @@ int ehss_scene_get_info(const ehss_scene* scene, ehss_scene_info* info) {
     info->scale = scene->scale;
     info->device_bytes = scene->device_bytes();
     info->build_path = scene->build_path;
-    info->reserved = 0;
+    info->packet_nodes = scene->view.pk_nodes;
     return EHSS_OK;
 }
 

@@ -38,7 +38,7 @@ template <int kMode, int kPA, int kLds>
 __global__ void EHSS_SOLVE_KERNEL k_solve(SceneView S, SolveArgs A, const ehss_sobol* sobs, const RayMap* maps,
                                                RayState* queue, uint32_t* queue_count) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const SceneView V = kLds ? stage_scene(S, lds) : S;
+    const SceneView V = kLds ? stage_scene<true>(S, lds) : S;
     const int32_t r = (int32_t)blockIdx.y;
     const SolveArgs B = replicate_args(A, r);
     uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
@@ -59,7 +59,7 @@ template <int kMode, int kPA, int kLds>
 __global__ void EHSS_SOLVE_KERNEL k_solve_x(SceneView S0, SolveArgs A, CrossTables X, int32_t reps,
                                                  int32_t groups, RayState* queue, uint32_t* queue_count) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const SceneView S = kLds ? stage_scene(S0, lds) : S0;
+    const SceneView S = kLds ? stage_scene<true>(S0, lds) : S0;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const uint32_t c0 = (uint32_t)(wave / groups) * kCrossCells;
@@ -726,7 +726,7 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     const int64_t cell_waves = (n + kCrossCells - 1) / kCrossCells;
     const dim3 xgrid((unsigned)((cell_waves * groups * 64 + 255) / 256));
     // Packet variants stage the scene in LDS on request (EHSS_FLAG_LDS_PACKETS).
-    const size_t stage_bytes = staged_scene_bytes(sc->view);
+    const size_t stage_bytes = staged_scene_bytes(sc->view, true);
     const bool lds_packets = (variant >= 2) && stage_bytes <= kSolveLdsLimit && (flags & EHSS_FLAG_LDS_PACKETS);
     const size_t solve_lds = lds_packets ? stage_bytes : 0;
 #define EHSS_LAUNCH_L(M, P, K)                                                                               \
@@ -1145,7 +1145,7 @@ int ehss_solve_noalloc(ehss_scene* sc, const ehss_sobol* sobol, int power, int c
     const bool pa = (flags & EHSS_FLAG_PA) != 0;
     uint32_t* qcount = reinterpret_cast<uint32_t*>(ctr + kCounterWords);
     const dim3 grid((unsigned)((n + 255) / 256)), block(256);
-    const size_t stage_bytes = staged_scene_bytes(sc->view);
+    const size_t stage_bytes = staged_scene_bytes(sc->view, true);
     const bool lds_packets = (variant >= 2) && stage_bytes <= kSolveLdsLimit && (flags & EHSS_FLAG_LDS_PACKETS);
     const size_t solve_lds = lds_packets ? stage_bytes : 0;
 #define EHSS_LAUNCH_L(M, P, K)                                                                                 \

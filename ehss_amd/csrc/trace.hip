@@ -123,7 +123,52 @@ __global__ void k_make_filter(SceneView S, Node32* nodes32, float4* sph32, doubl
     }
 }
 
+// Packet tree: thread i takes static node i.  The walk from the root gives the
+// node's group range and, in the cut tree, its preorder index (a left step adds
+// 1, a right step 1 + the cut nodes of the left sibling) and the index of its
+// first leaf.  Nodes below a subtree of at most `cut` groups are dropped; the
+// rest keep their FP32 filter box with links renumbered for the cut tree.
+__global__ void k_packet_tree(SceneView S, int32_t cut, Node32* pk, int32_t* pk_off) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.n_nodes) return;
+    int32_t node = 0, a = 0, b = S.n_groups, at = 0, leaf = 0;
+    while (node != i) {
+        const int32_t k = b - a;
+        if (k <= cut) return;  // inside a subtree that became one leaf
+        const int32_t h = k / 2, right = node + 2 * h;
+        if (i < right) {
+            node += 1;
+            b = a + h;
+            at += 1;
+        } else {
+            node = right;
+            at += 1 + cut_nodes(h, cut);
+            leaf += cut_leaves(h, cut);
+            a += h;
+        }
+    }
+    const int32_t k = b - a;
+    Node32 m = S.nodes32[i];
+    if (k <= cut) {
+        m.end = at + 1;
+        m.leaf = leaf;
+        pk_off[leaf] = S.offsets[a];
+        if (b == S.n_groups) pk_off[leaf + 1] = S.offsets[b];
+    } else {
+        m.end = at + cut_nodes(k, cut);
+        m.leaf = right_child(at + 1 + cut_nodes(k / 2, cut));
+    }
+    pk[at] = m;
+}
+
 }  // namespace
+
+int launch_packet_tree(const SceneView& v, int32_t cut, Node32* pk_nodes32, int32_t* pk_offsets, hipStream_t s) {
+    if (v.n_nodes == 0) return EHSS_OK;
+    hipLaunchKernelGGL(k_packet_tree, dim3((v.n_nodes + 255) / 256), dim3(256), 0, s, v, cut, pk_nodes32, pk_offsets);
+    EHSS_HIP(hipGetLastError());
+    return EHSS_OK;
+}
 
 void set_filter_margins(SceneView& v) {
     const double u = 5.9604644775390625e-08;  // 2^-24

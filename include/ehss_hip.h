@@ -145,7 +145,8 @@ typedef struct {
     double scale;     /* FP32 filter scale: bound on |coordinate - center| */
     int64_t device_bytes;
     int32_t build_path; /* EHSS_BUILD_SHAPED or EHSS_BUILD_GENERAL: how the last (re)build ran */
-    int32_t reserved;
+    int32_t packet_nodes; /* nodes of the tree the packet queries walk: the static tree with
+                           * subtrees of about 10 atoms merged into leaves (== nodes when none are) */
 } ehss_scene_info;
 
 /* Tree builds.  Without caller groups, with leaf_size <= 32 and no two atoms

@@ -156,6 +156,31 @@ def test_plain_and_fp32_fixup_kernels_are_bit_identical(oracle, name):
         assert again["omega"] == b[0]["omega"]
 
 
+@pytest.mark.parametrize("leaf", [1, 2, 4, 8, "groups"])
+def test_packet_tree_cuts_keep_results(leaf):
+    """The packet queries walk the static tree with subtrees of about 10 atoms
+    merged into leaves (here 10, 7, 3 and 1 groups per leaf for leaf sizes 1, 2,
+    4 and 8 of the 650-atom globule, and 10 for one caller group per atom):
+    any tree whose boxes enclose their spheres gives the plain FP64 kernel's
+    per-ray records."""
+    c, r = scenes.globule(650, 1)
+    kw = dict(groups=np.arange(len(r))) if leaf == "groups" else dict(leaf_size=leaf)
+    with GPUScene(c, r, **kw) as sc:
+        info = sc.info
+        assert (info["packet_nodes"] < info["nodes"]) == (leaf != 8), info
+        ref = sc.solve(scramble(11), 16, 64, precision="fp64", pa=True, keep_rays=True, histogram=True)
+        ref_rays = sc.last_rays(1 << 16)
+        phys = ~np.uint32(16)  # every flag bit except the kernel-specific grazing diagnostic
+        for opts in (dict(variant=4), dict(variant=4, lds=True), dict(variant=3), dict(variant=2)):
+            say("packet tree", leaf, info["packet_nodes"], info["nodes"], opts)
+            res = sc.solve(scramble(11), 16, 64, pa=True, keep_rays=True, histogram=True, **opts)
+            rays = sc.last_rays(1 << 16)
+            assert res["omega"] == ref["omega"] and res["pa"] == ref["pa"], opts
+            np.testing.assert_array_equal(rays["contributions"], ref_rays["contributions"], err_msg=str(opts))
+            np.testing.assert_array_equal(rays["flags"] & phys, ref_rays["flags"] & phys, err_msg=str(opts))
+            np.testing.assert_array_equal(res["histogram"], ref["histogram"], err_msg=str(opts))
+
+
 def test_large_scene_drain_walks_frontier_subtrees():
     """Above 16384 groups the drain's cooperative queries walk the frontier
     subtrees instead of scanning every leaf box; the answers still equal the
