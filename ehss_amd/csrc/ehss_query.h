@@ -355,52 +355,44 @@ __device__ __forceinline__ void nearest_packet(const SceneView& S, const V3& o, 
     }
 }
 
-// Packet traversal in front-to-back order: at an internal node both children's
-// boxes are tested; when lanes enter both, the packet descends first into the
-// child that the first lane entering both reaches first, and keeps the other on
-// a wave stack held across the lanes of one register (entry k in lane k).  The
-// group tree halves its range at every level, so at most 31 entries are ever
-// pushed.  A stacked node is re-tested against the lanes' bounds when it is
-// popped.  Reaching the near hits first shrinks every lane's bound sooner, so
-// fewer far subtrees are entered; every sphere that can win is still evaluated
-// in FP64 on every lane that needs it, so the answer is nearest_packet's.
-template <int kLds = 0, class C = uint64_t>
-__device__ __forceinline__ void nearest_packet_ordered(const SceneView& S, const V3& o, const V3& d, int32_t last,
-                                                       double tol, bool live, Query& q, C& boxes, C& spheres,
-                                                       C& retests, PAState* pa = nullptr) {
+// Adds per_lane to the lane's work counter when `on`.  (Wave-uniform sums kept
+// in scalar registers from ballots were measured slower: the scalar unit is as
+// loaded as the vector unit in these loops.)
+template <class C>
+__device__ __forceinline__ void count_lanes(C& c, uint32_t per_lane, bool on) {
+    c += on ? per_lane : 0;
+}
+
+// Front-to-back packet walk: the 64 lanes walk one node sequence of the packet
+// tree.  At an internal node both children's boxes are tested; when lanes enter
+// both, the packet descends first into the child that the first lane entering
+// both reaches first, and keeps the other on a wave stack held across the lanes
+// of one register (entry k in lane k).  The group tree halves its range at
+// every level, so at most 31 entries are ever pushed.  A stacked node is
+// re-tested against the lanes' bounds when it is popped.  `leaf(g, hit)` is
+// called (by every lane) for each leaf g some lane enters; `hit` is the lane's
+// own box test.  `bestf` is each lane's current pruning bound; the leaf
+// callback may lower it.
+template <int kLds, class C, class LeafFn>
+__device__ __forceinline__ void packet_walk(const SceneView& S, const Ray32& r, bool live, const float& bestf,
+                                            C& boxes, LeafFn&& leaf) {
     typedef PacketMem<kLds> M;
-    query_init(q);
-    if (__ballot(live) == 0) return;
-    const Ray32 r = ray32(S, o, d);
-    float bestf = INFINITY;
     int32_t stack = 0;
     int32_t sp = 0;
     int32_t node = 0;
     Node32 n = M::node(S, 0);
-    boxes += live ? 1 : 0;
+    count_lanes(boxes, 1, live);
     bool hit = box32(r, n, bestf) & live;
     if (__ballot(hit) == 0) return;
     for (;;) {
         bool descend = false;
         if (n.leaf >= 0) {
-            const int32_t g = n.leaf;
-            const int32_t k1 = M::offset(S, g + 1);
-            for (int32_t k = M::offset(S, g); k < k1; ++k) {
-                const int32_t wanted = M::id(S, k);
-                const float4 s = M::sphere(S, k);
-                if (!hit || wanted == last) continue;
-                ++spheres;
-                if (!needs_exact(S, r, s, q.best, tol, pa)) continue;
-                ++retests;
-                const int32_t before = q.atom;
-                eval64(o, d, load_sphere64(S.sphs, k), wanted, tol, q, pa);
-                if (q.atom != before) bestf = box_bound(S, q.best, tol);
-            }
+            leaf(n.leaf, hit);
         } else {
             const int32_t a = node + 1, b = right_child(n.leaf);
             const Node32 na = M::node(S, a);
             const Node32 nb = M::node(S, b);
-            boxes += live ? 2 : 0;
+            count_lanes(boxes, 2, live);
             // Both tests on every lane, then masked: no branch, so both
             // children's records are fetched together.
             float neara, nearb;
@@ -446,7 +438,7 @@ __device__ __forceinline__ void nearest_packet_ordered(const SceneView& S, const
             --sp;
             node = __builtin_amdgcn_readlane(stack, sp);
             n = M::node(S, node);
-            boxes += live ? 1 : 0;
+            count_lanes(boxes, 1, live);
             hit = box32(r, n, bestf) & live;
             if (__ballot(hit)) {
                 found = true;
@@ -455,6 +447,187 @@ __device__ __forceinline__ void nearest_packet_ordered(const SceneView& S, const
         }
         if (!found) break;
     }
+}
+
+// Packet traversal in front-to-back order (packet_walk) with the exact FP64
+// re-test of every sphere the FP32 filter cannot exclude run as the walk meets
+// it.  Reaching the near hits first shrinks every lane's bound sooner, so fewer
+// far subtrees are entered; every sphere that can win is still evaluated in
+// FP64 on every lane that needs it, so the answer is nearest_packet's.
+template <int kLds = 0, class C = uint64_t>
+__device__ __forceinline__ void nearest_packet_ordered(const SceneView& S, const V3& o, const V3& d, int32_t last,
+                                                       double tol, bool live, Query& q, C& boxes, C& spheres,
+                                                       C& retests, PAState* pa = nullptr) {
+    typedef PacketMem<kLds> M;
+    query_init(q);
+    if (__ballot(live) == 0) return;
+    const Ray32 r = ray32(S, o, d);
+    float bestf = INFINITY;
+    packet_walk<kLds>(S, r, live, bestf, boxes, [&](int32_t g, bool hit) {
+        const int32_t k1 = M::offset(S, g + 1);
+        for (int32_t k = M::offset(S, g); k < k1; ++k) {
+            const int32_t wanted = M::id(S, k);
+            const float4 s = M::sphere(S, k);
+            if (!hit || wanted == last) continue;
+            ++spheres;
+            if (!needs_exact(S, r, s, q.best, tol, pa)) continue;
+            ++retests;
+            const int32_t before = q.atom;
+            eval64(o, d, load_sphere64(S.sphs, k), wanted, tol, q, pa);
+            if (q.atom != before) bestf = box_bound(S, q.best, tol);
+        }
+    });
+}
+
+// Candidate lists of the two-pass query: kCand (slot, tlo) pairs per thread of a
+// block of at most 256 threads, pair j of thread t at [j][t] (consecutive lanes
+// read consecutive 8-byte words).  Row kCand takes the writes of lanes that have
+// no candidate to add or whose list is full, so that appends need no branch.
+#ifndef EHSS_PACKET_CANDIDATES
+#define EHSS_PACKET_CANDIDATES 8
+#endif
+constexpr int kCand = EHSS_PACKET_CANDIDATES;
+struct CandLists {
+    int2 e[kCand + 1][256];  // (slot, bits of the FP32 lower bound)
+};
+static __shared__ CandLists g_cand;
+
+// Drops the thread's candidates whose lower bound exceeds ut; returns the count kept.
+__device__ __forceinline__ int32_t compact_candidates(int tid, int32_t nc, float ut) {
+    int32_t w = 0;
+    for (int32_t j = 0; j < nc; ++j) {
+        const int2 c = g_cand.e[j][tid];
+        if (!(__int_as_float(c.y) > ut)) {
+            g_cand.e[w][tid] = c;
+            ++w;
+        }
+    }
+    return w;
+}
+
+// An FP32 value >= u + ftol (u, ftol >= 0): the sum rounded to nearest, then
+// scaled by 1 + 2^-22 (more than the two roundings can lose); FLT_MAX in place
+// of +inf, so that an excluded sphere's key (+inf) never compares below it.
+__device__ __forceinline__ float upper_sum(float u, float ftol) {
+    return fminf((u + ftol) * (1.0f + 0x1p-22f), 3.40282347e+38f);
+}
+
+// Slot id read by one lane (its own slot; an LDS read when the scene is staged).
+template <int kLds>
+__device__ __forceinline__ int32_t lane_slot_id(const SceneView& S, int32_t k) {
+    if constexpr (kLds) return ((LInt32)S.ids)[k];
+    else return S.ids[k];
+}
+
+// Two-pass packet query, same answer as nearest_packet_ordered.
+//
+// Pass 1 walks the packet tree (packet_walk) in FP32 only.  For each sphere of
+// an entered leaf every lane computes the projection p of x = o - c on the ray,
+// the perpendicular part x - p d and disc = r^2 - |x - p d|^2 (FMA forms: each
+// fused operation rounds once, within the rounding the FP32 margins m_p and
+// m_disc allow for the separate operations).  A sphere is excluded on a lane
+// for every FP64 outcome within tol when it is the last atom, behind, or a
+// clear miss (needs_exact's tests); when every lane excludes it, one uniform
+// branch skips it.  Otherwise each lane computes
+//   - tlo, a lower bound of the FP64 entry (needs_exact's bound, with the square
+//     root taken as x * rsq(x) scaled up by 1 + 2^-20: the hardware rsq is within
+//     1 ulp and the two products round by 2^-24 each), as key, or +inf where
+//     the sphere is excluded;
+//   - thi, an upper bound of the FP64 entry when the sphere is a robust hit in
+//     front (FP64 disc surely > 0 and tlo > 0, so the FP64 entry is
+//     -projection - sqrt(disc / aa), finite), else +inf.  The square root's
+//     lower bound is (disc - m) / sqrt(disc + m) <= sqrt(disc - m) with the same
+//     rsq, scaled down by 1 - 2^-20; a slack of 2^-21 of the projection term
+//     covers the FP32 roundings of the two differences.
+// U, the smallest thi so far, bounds the FP64 best from above.  A sphere with
+// key <= U + tol joins the lane's candidate list; boxes are pruned at U + tol.
+//
+// Pass 2 runs the exact FP64 re-tests in lock step, each lane on its own j-th
+// candidate, skipping those whose tlo exceeds the final U + tol.
+//
+// Every sphere whose FP64 entry lies within best + tol has tlo <= U + tol at its
+// visit and at the end, so it is evaluated; so is every near-threshold sphere
+// that the grazing test and the fused PA test can see.  The winner, the tie
+// rule, `second`, the grazing flag and the PA flags are therefore those of the
+// one-pass query.  Compared with that query, the FP64 block runs once per
+// candidate rank instead of once per sphere some lane needs, the walk carries
+// only FP32 state, and control flow stays uniform except for the rare
+// compaction of a full list.  A lane whose list overflows (kCand candidates
+// left after dropping those a smaller U excludes) sets `overflowed`: its q is
+// then not the answer, and the caller must have the query done by another path
+// (the ray's state is unchanged, so any query path gives the same answer); its
+// PA state is still exact when `decided`.
+template <int kLds = 0, class C = uint64_t>
+__device__ __forceinline__ void nearest_packet_2p(const SceneView& S, const V3& o, const V3& d, int32_t last,
+                                                  double tol, bool live, Query& q, C& boxes, C& spheres,
+                                                  C& retests, bool& overflowed, PAState* pa = nullptr) {
+    typedef PacketMem<kLds> M;
+    query_init(q);
+    overflowed = false;
+    if (__ballot(live) == 0) return;
+    const int tid = (int)threadIdx.x;
+    const float ftol = (float)tol * 1.01f;
+    const float plim = S.m_p + ftol;
+    const float k2 = 2.01f * ftol;
+    int32_t nc = 0;
+    int32_t over = 0;
+    int32_t decided = 0;
+    float U = INFINITY;          // FP32 upper bound of the FP64 winning entry
+    float ut = 3.40282347e+38f;  // >= U + tol (FLT_MAX while U is +inf)
+    {
+        const Ray32 r = ray32(S, o, d);
+        float bestf = INFINITY;
+        packet_walk<kLds>(S, r, live, bestf, boxes, [&](int32_t g, bool hit) {
+            if (!hit) return;
+            const int32_t k1 = M::offset(S, g + 1);
+            for (int32_t k = M::offset(S, g); k < k1; ++k) {
+                const int32_t wanted = M::id(S, k);
+                const float4 s = M::sphere(S, k);
+                const float x0 = r.ox - s.x, x1 = r.oy - s.y, x2 = r.oz - s.z;
+                const float p = __builtin_fmaf(x2, r.dz, __builtin_fmaf(x1, r.dy, x0 * r.dx));
+                const float q0 = __builtin_fmaf(-p, r.dx, x0), q1 = __builtin_fmaf(-p, r.dy, x1),
+                            q2 = __builtin_fmaf(-p, r.dz, x2);
+                const float disc = __builtin_fmaf(s.w, s.w, -__builtin_fmaf(q2, q2, __builtin_fmaf(q1, q1, q0 * q0)));
+                const float mdisc = __builtin_fmaf(s.w, k2, S.m_disc);
+                const bool other = wanted != last;
+                count_lanes(spheres, 1, other);
+                // Excluded on every lane (the common case): one uniform branch.
+                const bool keep = other && p < plim && disc >= -mdisc;
+                if (__ballot(keep) == 0) continue;
+                if (pa) decided |= (keep && disc > mdisc && p <= -plim) ? 1 : 0;  // robust hit in both tests
+                const float xh = fmaxf(disc, 0.f) + S.m_disc;
+                const float ri = __builtin_amdgcn_rsqf(xh);
+                const float tlo = (-p - S.m_p) - xh * ri * (1.0f + 0x1p-20f);
+                const float a = S.m_p - p;
+                const float thi0 = (a - (disc - S.m_disc) * ri * (1.0f - 0x1p-20f)) + a * 0x1p-21f;
+                const float key = keep ? tlo : INFINITY;
+                float thi = disc > S.m_disc ? thi0 : INFINITY;
+                thi = tlo > 0.f ? thi : INFINITY;
+                thi = keep ? thi : INFINITY;
+                const bool ok = key <= ut;
+                if (__ballot(ok && nc == kCand)) {
+                    if (ok && nc == kCand) nc = compact_candidates(tid, nc, ut);
+                }
+                const bool room = nc < kCand;
+                g_cand.e[ok && room ? nc : kCand][tid] = make_int2(k, __float_as_int(tlo));
+                nc += ok && room ? 1 : 0;
+                over |= ok && !room ? 1 : 0;
+                U = fminf(U, thi);
+                ut = upper_sum(U, ftol);
+                bestf = ut * (1.0f + 4.0f * kU32) + S.m_box;
+            }
+        });
+    }
+    if (pa && decided) pa->decided = pa->hit = true;
+    for (int32_t j = 0;; ++j) {
+        const bool act = j < nc && !over;
+        if (__ballot(act) == 0) break;
+        const int2 c = g_cand.e[act ? j : kCand][tid];
+        const bool run = act && !(__int_as_float(c.y) > ut);
+        count_lanes(retests, 1, run);
+        if (run) eval64(o, d, S.sphs[c.x], lane_slot_id<kLds>(S, c.x), tol, q, pa);
+    }
+    overflowed = over != 0;
 }
 
 // Cooperative nearest query: one ray, a group of G lanes (G a power of two <= 64,

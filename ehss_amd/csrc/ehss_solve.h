@@ -37,6 +37,12 @@
 #ifndef EHSS_ORDERED_PACKETS
 #define EHSS_ORDERED_PACKETS 1
 #endif
+// Packet queries of the fast kernel in two passes (nearest_packet_2p: FP32 walk
+// collecting candidates, then lock-step FP64 re-tests; 0: re-tests during the
+// walk, nearest_packet_ordered).
+#ifndef EHSS_TWO_PASS_PACKETS
+#define EHSS_TWO_PASS_PACKETS 1
+#endif
 #ifndef EHSS_SOLVE_WAVES
 #define EHSS_SOLVE_WAVES 7
 #endif
@@ -496,10 +502,16 @@ __device__ __forceinline__ void solve_ray_at(const SceneView& S, const SolveArgs
     } else {
         PAState pa = {false, false, false};
         PAState* pap = kPA ? &pa : nullptr;
+        // Two-pass packet queries: a ray whose candidate list overflowed has no
+        // answer from the query; it keeps its state, takes no further packet
+        // query, and is queued for the drain, which repeats that query.
+        bool deferred = false;
         if (kMode == 1) {
             if (valid) nearest_fast(S, R.o, R.d, -1, R.tol, q, boxes, spheres, retests, pap);
         } else {
-            if constexpr (kMode == 4 && EHSS_ORDERED_PACKETS)
+            if constexpr (kMode == 4 && EHSS_ORDERED_PACKETS && EHSS_TWO_PASS_PACKETS)
+                nearest_packet_2p<kLds>(S, R.o, R.d, -1, R.tol, valid, q, boxes, spheres, retests, deferred, pap);
+            else if constexpr (kMode == 4 && EHSS_ORDERED_PACKETS)
                 nearest_packet_ordered<kLds>(S, R.o, R.d, -1, R.tol, valid, q, boxes, spheres, retests, pap);
             else
                 nearest_packet<kLds>(S, R.o, R.d, -1, R.tol, valid, q, boxes, spheres, retests, pap);
@@ -508,7 +520,9 @@ __device__ __forceinline__ void solve_ray_at(const SceneView& S, const SolveArgs
             if (kPA) {
                 bool hit = pa.hit, pgraze = false;
                 if (!pa.decided) {
-                    if (q.atom >= 0)  // a winner without a robust any-hit: run the full any-hit query
+                    // A winner without a robust any-hit (or no answer from the
+                    // query): run the full any-hit query.
+                    if (deferred || q.atom >= 0)
                         hit = pa_fast(S, R.o, R.d, -1, R.tol, pboxes, ptests, pretests, pgraze);
                     else
                         pgraze = pa.critical;
@@ -516,7 +530,7 @@ __device__ __forceinline__ void solve_ray_at(const SceneView& S, const SolveArgs
                 if (hit) R.flags |= EHSS_RAY_PA_HIT;
                 if (pgraze) R.flags |= EHSS_RAY_GRAZING;
             }
-            ray_process(S, A.cap, R, q);
+            if (!deferred) ray_process(S, A.cap, R, q);
         }
         auto scene_of = [&](int32_t) -> const SceneView& { return S; };
         auto args_of = [&](int32_t) -> const SolveArgs& { return A; };
@@ -527,18 +541,25 @@ __device__ __forceinline__ void solve_ray_at(const SceneView& S, const SolveArgs
         };
         if (kMode == 3 || kMode == 4) {
             for (int k = 1; kMode == 3 || k < A.packets; ++k) {
-                const bool live = !R.done;
+                const bool live = !R.done && !deferred;
                 const uint64_t m = __ballot(live);
                 if (m == 0) break;
                 if (kMode == 3 && __popcll(m) <= (unsigned)A.coop) {
                     finish_coop(R, 0, scene_of, args_of, account);
                     break;
                 }
-                if constexpr (kMode == 4 && EHSS_ORDERED_PACKETS)
+                bool overflowed = false;
+                if constexpr (kMode == 4 && EHSS_ORDERED_PACKETS && EHSS_TWO_PASS_PACKETS)
+                    nearest_packet_2p<kLds>(S, R.o, R.d, R.last, R.tol, live, q, boxes, spheres, retests,
+                                            overflowed);
+                else if constexpr (kMode == 4 && EHSS_ORDERED_PACKETS)
                     nearest_packet_ordered<kLds>(S, R.o, R.d, R.last, R.tol, live, q, boxes, spheres, retests);
                 else
                     nearest_packet<kLds>(S, R.o, R.d, R.last, R.tol, live, q, boxes, spheres, retests);
-                if (live) ray_process(S, A.cap, R, q);
+                if (live) {
+                    if (overflowed) deferred = true;
+                    else ray_process(S, A.cap, R, q);
+                }
             }
             if (kMode == 4) {
                 const bool alive = !R.done;

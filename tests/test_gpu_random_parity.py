@@ -3,8 +3,11 @@ packet query, with one lane per drained ray, and with the packet queries reading
 an LDS-staged scene) against the plain FP64 kernel,
 bit for bit (Omega, PA, every ray's contribution and physical flags, the bounce
 histogram), on random molecules across scales, sizes and degenerate placements:
-clouds at scales 1e-3 to 1e3, tangent-sphere lattices, coincident centres and
-radii spread over four decades.  The FP32 filter's margins are relative to the
+clouds at scales 1e-3 to 1e3, tangent-sphere lattices, coincident centres,
+radii spread over four decades, and stacks of a dozen identical spheres (every
+ray through a stack has more equal-entry candidates than a packet query's
+candidate list holds, so those rays are handed to the queue drain, and the tie
+rule picks the lowest id).  The FP32 filter's margins are relative to the
 scene scale, so every scale must give the plain kernel's bits."""
 import numpy as np
 import pytest
@@ -34,6 +37,14 @@ def molecule(kind, seed):
         base = rng.normal(scale=3.0, size=(n, 3))
         c = np.concatenate([base, base[: n // 2]])
         r = np.concatenate([rng.uniform(1.0, 2.0, n), rng.uniform(0.5, 2.5, n // 2)])
+    elif kind == "stack":  # a cloud with identical spheres stacked at a few centres
+        n = int(rng.integers(20, 600))
+        c = rng.normal(scale=6.0, size=(n, 3))
+        r = rng.uniform(1.0, 2.0, size=n)
+        k = 12 + seed % 5
+        tops = rng.normal(scale=4.0, size=(3, 3))
+        c = np.concatenate([c] + [np.repeat(t[None], k, axis=0) for t in tops])
+        r = np.concatenate([r, np.full(3 * k, 1.7)])
     else:  # "radii": four decades of radii
         n = int(rng.integers(50, 1500))
         c = rng.normal(scale=8.0, size=(n, 3))
@@ -41,7 +52,7 @@ def molecule(kind, seed):
     return c, r
 
 
-CASES = [(kind, seed) for kind in ("cloud", "lattice", "coincident", "radii") for seed in range(5)]
+CASES = [(kind, seed) for kind in ("cloud", "lattice", "coincident", "radii", "stack") for seed in range(5)]
 
 
 @pytest.mark.parametrize("kind,seed", CASES)
