@@ -369,25 +369,28 @@ __device__ __forceinline__ void count_lanes(C& c, uint32_t per_lane, bool on) {
 // both reaches first, and keeps the other on a wave stack held across the lanes
 // of one register (entry k in lane k).  The group tree halves its range at
 // every level, so at most 31 entries are ever pushed.  A stacked node is
-// re-tested against the lanes' bounds when it is popped.  `leaf(g, hit)` is
-// called (by every lane) for each leaf g some lane enters; `hit` is the lane's
-// own box test.  `bestf` is each lane's current pruning bound; the leaf
-// callback may lower it.
+// re-tested against the lanes' bounds when it is popped.  Which lanes enter a
+// node is kept as a wave-uniform lane mask (ballot of the box test, and of the
+// parent's mask), so the walk's decisions stay in scalar registers.
+// `leaf(g, mask)` is called (by every lane) for each leaf g some lane enters;
+// `mask` holds the lanes that enter it.  `bestf` is each lane's current pruning
+// bound; the leaf callback may lower it.
 template <int kLds, class C, class LeafFn>
 __device__ __forceinline__ void packet_walk(const SceneView& S, const Ray32& r, bool live, const float& bestf,
                                             C& boxes, LeafFn&& leaf) {
     typedef PacketMem<kLds> M;
+    const uint64_t lanes = __ballot(live);
     int32_t stack = 0;
     int32_t sp = 0;
     int32_t node = 0;
     Node32 n = M::node(S, 0);
     count_lanes(boxes, 1, live);
-    bool hit = box32(r, n, bestf) & live;
-    if (__ballot(hit) == 0) return;
+    uint64_t hm = __ballot(box32(r, n, bestf)) & lanes;
+    if (hm == 0) return;
     for (;;) {
         bool descend = false;
         if (n.leaf >= 0) {
-            leaf(n.leaf, hit);
+            leaf(n.leaf, hm);
         } else {
             const int32_t a = node + 1, b = right_child(n.leaf);
             const Node32 na = M::node(S, a);
@@ -396,9 +399,8 @@ __device__ __forceinline__ void packet_walk(const SceneView& S, const Ray32& r, 
             // Both tests on every lane, then masked: no branch, so both
             // children's records are fetched together.
             float neara, nearb;
-            const bool ha = box32_near(r, na, bestf, neara) & hit;
-            const bool hb = box32_near(r, nb, bestf, nearb) & hit;
-            const uint64_t ma = __ballot(ha), mb = __ballot(hb);
+            const uint64_t ma = __ballot(box32_near(r, na, bestf, neara)) & hm;
+            const uint64_t mb = __ballot(box32_near(r, nb, bestf, nearb)) & hm;
             if (ma && mb) {
                 // Order by the first lane that enters both children; when no lane
                 // does, visit first the child more lanes enter.
@@ -417,17 +419,17 @@ __device__ __forceinline__ void packet_walk(const SceneView& S, const Ray32& r, 
                 ++sp;
                 node = a_first ? a : b;
                 n = a_first ? na : nb;
-                hit = a_first ? ha : hb;
+                hm = a_first ? ma : mb;
                 descend = true;
             } else if (ma) {
                 node = a;
                 n = na;
-                hit = ha;
+                hm = ma;
                 descend = true;
             } else if (mb) {
                 node = b;
                 n = nb;
-                hit = hb;
+                hm = mb;
                 descend = true;
             }
         }
@@ -439,8 +441,8 @@ __device__ __forceinline__ void packet_walk(const SceneView& S, const Ray32& r, 
             node = __builtin_amdgcn_readlane(stack, sp);
             n = M::node(S, node);
             count_lanes(boxes, 1, live);
-            hit = box32(r, n, bestf) & live;
-            if (__ballot(hit)) {
+            hm = __ballot(box32(r, n, bestf)) & lanes;
+            if (hm) {
                 found = true;
                 break;
             }
@@ -448,6 +450,9 @@ __device__ __forceinline__ void packet_walk(const SceneView& S, const Ray32& r, 
         if (!found) break;
     }
 }
+
+// Whether this lane's bit is set in a wave-uniform lane mask.
+__device__ __forceinline__ bool lane_in(uint64_t mask) { return (mask >> (threadIdx.x & 63)) & 1u; }
 
 // Packet traversal in front-to-back order (packet_walk) with the exact FP64
 // re-test of every sphere the FP32 filter cannot exclude run as the walk meets
@@ -463,7 +468,8 @@ __device__ __forceinline__ void nearest_packet_ordered(const SceneView& S, const
     if (__ballot(live) == 0) return;
     const Ray32 r = ray32(S, o, d);
     float bestf = INFINITY;
-    packet_walk<kLds>(S, r, live, bestf, boxes, [&](int32_t g, bool hit) {
+    packet_walk<kLds>(S, r, live, bestf, boxes, [&](int32_t g, uint64_t mask) {
+        const bool hit = lane_in(mask);
         const int32_t k1 = M::offset(S, g + 1);
         for (int32_t k = M::offset(S, g); k < k1; ++k) {
             const int32_t wanted = M::id(S, k);
@@ -577,8 +583,8 @@ __device__ __forceinline__ void nearest_packet_2p(const SceneView& S, const V3& 
     {
         const Ray32 r = ray32(S, o, d);
         float bestf = INFINITY;
-        packet_walk<kLds>(S, r, live, bestf, boxes, [&](int32_t g, bool hit) {
-            if (!hit) return;
+        packet_walk<kLds>(S, r, live, bestf, boxes, [&](int32_t g, uint64_t mask) {
+            if (!lane_in(mask)) return;
             const int32_t k1 = M::offset(S, g + 1);
             for (int32_t k = M::offset(S, g); k < k1; ++k) {
                 const int32_t wanted = M::id(S, k);
