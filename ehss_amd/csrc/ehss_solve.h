@@ -108,8 +108,16 @@ static_assert(sizeof(Partial) == 8 * 8, "Partial occupies words 8..15 of a repli
 // are the corresponding rows of the scrambled generator matrices,
 // g = T^-1 (bits of w || bits of lane); col[k] is column k of T^-1.  The ray index
 // (record position) is the inverse Gray code of g.
+//
+// The device holds the map composed with the scramble: q[k][d] = XOR over the
+// set bits j of column k of T^-1 of the scrambled direction number sv[d][j]
+// (for m < 6, of the Gray code's column k).  The Sobol words of the point that
+// thread t = (w << 6) | lane traces are then
+//   shift_d ^ XOR_{bit k of w} q[k][d] ^ XOR_{bit k of lane} q[m - 6 + k][d],
+// the wave part on the scalar unit and the lane part from six columns, instead
+// of 30 bit tests per coordinate on every lane.
 struct RayMap {
-    uint32_t col[32];
+    uint32_t q[32][4];
 };
 
 __device__ __forceinline__ uint32_t inverse_gray(uint32_t g) {
@@ -121,23 +129,12 @@ __device__ __forceinline__ uint32_t inverse_gray(uint32_t g) {
     return g;
 }
 
-typedef const __attribute__((address_space(4))) uint32_t* CMap;
-__device__ __forceinline__ uint32_t map_col(const RayMap& m, int k) { return m.col[k]; }
-__device__ __forceinline__ uint32_t map_col(CMap m, int k) { return m[k]; }
+// The map of a net of 2^m points composed with its scramble (host).
+void ray_map_words(const ehss_sobol& sob, int m, RayMap& out);
 
-// Gray index of thread t (wave t >> 6, lane t & 63) for a net of 2^power points.
-template <class Map>
-__device__ __forceinline__ uint32_t ray_gray(Map map, uint32_t t, int power) {
-    if (power < 6) return t ^ (t >> 1);
-    const uint32_t w = __builtin_amdgcn_readfirstlane(t >> 6);
-    const uint32_t lane = t & 63u;
-    uint32_t gw = 0, gl = 0;
-    for (int k = 0; k < power - 6; ++k)
-        if ((w >> k) & 1u) gw ^= map_col(map, k);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) gl ^= map_col(map, power - 6 + k) & (0u - ((lane >> k) & 1u));
-    return gw ^ gl;
-}
+typedef const __attribute__((address_space(4))) uint32_t* CMap;
+__device__ __forceinline__ uint32_t map_word(const RayMap& m, int k, int d) { return m.q[k][d]; }
+__device__ __forceinline__ uint32_t map_word(CMap m, int k, int d) { return m[4 * k + d]; }
 
 // Scrambled direction numbers read through the constant address space (scalar loads).
 typedef const __attribute__((address_space(4))) uint32_t* CSobol;
@@ -182,6 +179,28 @@ struct CrossTables {
 struct PointQ {
     uint32_t q[4];
 };
+
+// Sobol words of the point thread t traces (see RayMap).
+template <class Map, class Sob>
+__device__ __forceinline__ PointQ map_point(Map map, Sob sob, uint32_t t, int power) {
+    PointQ p;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) p.q[d] = sob_shift(sob, d);
+    const int lb = power < 6 ? 0 : power - 6;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(t >> 6);
+    for (int k = 0; k < lb; ++k)
+        if ((w >> k) & 1u)
+#pragma unroll
+            for (int d = 0; d < 4; ++d) p.q[d] ^= map_word(map, k, d);
+    const uint32_t lane = t & 63u;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const uint32_t m = 0u - ((lane >> k) & 1u);
+#pragma unroll
+        for (int d = 0; d < 4; ++d) p.q[d] ^= map_word(map, lb + k, d) & m;
+    }
+    return p;
+}
 
 __device__ __forceinline__ void sobol_point_gray(const PointQ& p, uint32_t, double u[4]) {
     const double scale = 9.313225746154785e-10;  // 2^-30
@@ -606,8 +625,8 @@ __device__ __forceinline__ void solve_ray(const SceneView& S, const SolveArgs& A
                                           uint64_t* ctr, RayState* queue = nullptr, uint32_t* queue_count = nullptr,
                                           int32_t job = 0) {
     const bool valid = t < (uint64_t)A.n;
-    const uint32_t g = valid ? ray_gray(map, t, A.power) : 0u;
-    solve_ray_at<kMode, kPA, kLds>(S, A, sob, valid, g, t, ctr, queue, queue_count, job);
+    solve_ray_at<kMode, kPA, kLds>(S, A, map_point(map, sob, t, A.power), valid, 0u, t, ctr, queue, queue_count,
+                                   job);
 }
 
 __host__ __device__ inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }

@@ -275,7 +275,7 @@ int fill_plan(ehss_plan* p, int n_jobs, ehss_scene* const* scenes, const ehss_so
         J.args.trig = trig_compat();
         J.args.times = nullptr;
         J.sob = sobols[j];
-        make_ray_map(J.sob, powers[j], J.map.col);
+        ray_map_words(J.sob, powers[j], J.map);
         J.red_blocks = (n + kReduceRays - 1) / kReduceRays;
         J.partials = p->partials.as<Partial>() + red_off[j];
         J.hist = H + bin_off[j];
@@ -449,7 +449,7 @@ int ehss_plan_set_sobol(ehss_plan* plan, int job, const ehss_sobol* sobol) {
     EHSS_CHECK(job >= 0 && job < plan->n_jobs, "job %d out of range", job);
     PlanJob& J = plan->jobs[job];
     J.sob = *sobol;
-    make_ray_map(*sobol, J.args.power, J.map.col);
+    ray_map_words(*sobol, J.args.power, J.map);
     plan->dirty = true;
     return EHSS_OK;
 }

@@ -167,7 +167,7 @@ __global__ void k_put_tables(ehss_sobol sob, RayMap map, ehss_sobol* d_sob, RayM
     const int i = threadIdx.x;
     if (i < 4 * 32) d_sob->sv[i / 32][i % 32] = sob.sv[i / 32][i % 32];
     if (i < 4) d_sob->shift[i] = sob.shift[i];
-    if (i < 32) d_map->col[i] = map.col[i];
+    if (i < 4 * 32) d_map->q[i / 4][i % 4] = map.q[i / 4][i % 4];
 }
 
 // Allocation-free solve: totals, histogram and ledger from the device counters,
@@ -451,6 +451,21 @@ void make_ray_map(const ehss_sobol& sob, int m, uint32_t cols[32], uint32_t* row
 
 void make_ray_map(const ehss_sobol& sob, int m, uint32_t cols[32]) { make_ray_map(sob, m, cols, nullptr); }
 
+void ray_map_words(const ehss_sobol& sob, int m, RayMap& out) {
+    uint32_t cols[32];
+    make_ray_map(sob, m, cols, nullptr);
+    if (m < 6)  // the Gray code g = t ^ (t >> 1): column k holds bits k and k - 1
+        for (int k = 0; k < m; ++k) cols[k] = (1u << k) | (k > 0 ? 1u << (k - 1) : 0u);
+    for (int k = 0; k < 32; ++k)
+        for (int d = 0; d < 4; ++d) {
+            uint32_t v = 0;
+            if (k < m)
+                for (int j = 0; j < 30; ++j)
+                    if ((cols[k] >> j) & 1u) v ^= sob.sv[d][j];
+            out.q[k][d] = v;
+        }
+}
+
 // Replicate-fastest tables of the cross schedule (CrossTables) for replicates
 // padded to rp: words [m][4][rp] col, [4][rp] off, [m][rp] ucol, [rp] uoff.
 size_t cross_table_words(int m, int rp) { return (size_t)rp * (4 * m + 4 + m + 1); }
@@ -629,8 +644,13 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     EHSS_TRY(sc->counters.reserve(sizeof(unsigned long long) * words));
     const int32_t groups = (reps + 63) / 64;
     const int32_t rp = groups * 64;
-    const size_t table_bytes = (sizeof(ehss_sobol) + sizeof(RayMap) + sizeof(CrossMap)) * reps +
-                               sizeof(uint32_t) * cross_table_words(power, rp);
+    // Host/device tables: scrambles, cross maps and cross words (always
+    // uploaded), then the per-replicate ray maps (uploaded when the replicates
+    // are not scheduled across lanes, which is the only schedule that reads them).
+    const size_t base_bytes = align_up((sizeof(ehss_sobol) + sizeof(CrossMap)) * reps +
+                                           sizeof(uint32_t) * cross_table_words(power, rp),
+                                       alignof(RayMap));
+    const size_t table_bytes = base_bytes + sizeof(RayMap) * reps;
     EHSS_TRY(sc->tables.reserve(table_bytes));
     EHSS_TRY(sc->host_table.reserve(table_bytes));
     unsigned long long* ctr = sc->counters.as<unsigned long long>();
@@ -640,12 +660,9 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
 
     // Scrambles and ray maps, staged in pinned memory (not reused while pending).
     ehss_sobol* h_sob = sc->host_table.as<ehss_sobol>();
-    RayMap* h_map = reinterpret_cast<RayMap*>(h_sob + reps);
-    CrossMap* h_cross = reinterpret_cast<CrossMap*>(h_map + reps);
-    parallel_for(reps, [&](int r) {
-        h_sob[r] = sobs[r];
-        make_ray_map(sobs[r], power, h_map[r].col);
-    });
+    CrossMap* h_cross = reinterpret_cast<CrossMap*>(h_sob + reps);
+    RayMap* h_map = reinterpret_cast<RayMap*>(reinterpret_cast<unsigned char*>(h_sob) + base_bytes);
+    parallel_for(reps, [&](int r) { h_sob[r] = sobs[r]; });
     // Many replicates: schedule them across lanes when a shared partition exists.
     // Measured on the 650-atom globule: the cross schedule wins for small nets
     // (2^11 rays from 16 replicates, 2^14 from 64) and loses for large ones
@@ -655,10 +672,12 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
                        make_cross_maps(sobs, reps, power, h_cross);
     uint32_t* h_words = reinterpret_cast<uint32_t*>(h_cross + reps);
     if (cross) make_cross_tables(sobs, h_cross, reps, rp, power, h_words);
-    EHSS_HIP(hipMemcpyAsync(sc->tables.p, h_sob, table_bytes, hipMemcpyHostToDevice, s));
+    else parallel_for(reps, [&](int r) { ray_map_words(sobs[r], power, h_map[r]); });
+    EHSS_HIP(hipMemcpyAsync(sc->tables.p, h_sob, cross ? base_bytes : table_bytes, hipMemcpyHostToDevice, s));
     const ehss_sobol* d_sob = sc->tables.as<ehss_sobol>();
-    const RayMap* d_map = reinterpret_cast<const RayMap*>(d_sob + reps);
-    const uint32_t* d_words = reinterpret_cast<const uint32_t*>(reinterpret_cast<const CrossMap*>(d_map + reps) + reps);
+    const RayMap* d_map =
+        reinterpret_cast<const RayMap*>(reinterpret_cast<const unsigned char*>(sc->tables.p) + base_bytes);
+    const uint32_t* d_words = reinterpret_cast<const uint32_t*>(reinterpret_cast<const CrossMap*>(d_sob + reps) + reps);
     CrossTables XT;
     XT.rp = rp;
     XT.pad = 0;
@@ -801,7 +820,7 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
                             sizeof(unsigned long long) * bins * reps, hipMemcpyDeviceToHost, s));
     EHSS_HIP(hipEventRecord(sc->ev_done, s));
     sc->pending = true;
-    memcpy(sc->p_map, h_map[0].col, sizeof sc->p_map);
+    make_ray_map(sobs[0], power, sc->p_map);
     sc->p_power = power;
     sc->p_reps = reps;
     sc->p_rays = n;
@@ -1116,7 +1135,7 @@ int ehss_solve_noalloc(ehss_scene* sc, const ehss_sobol* sobol, int power, int c
     EHSS_TRY(launch_zero(hist, bins, s));
     EHSS_TRY(launch_zero(led, bins, s));
     RayMap map;
-    make_ray_map(*sobol, power, map.col);
+    ray_map_words(*sobol, power, map);
     hipLaunchKernelGGL(k_put_tables, dim3(1), dim3(128), 0, s, *sobol, map, d_sob, d_map);
 
     SolveArgs A;

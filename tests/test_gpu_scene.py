@@ -98,6 +98,10 @@ def test_fused_solve_matches_oracle_on_same_sobol_points(oracle, prec):
     order (1e-12 relative; the stated bar is 1e-6)."""
     cases = [("c60", scenes.c60(), 0, 14, 64), ("f29", scenes.fixture_29(), 881, 12, 4),
              ("globule650", scenes.globule(650, 1), 1, 14, 64), ("cross6", scenes.fixture_cross(), 3, 12, 1)]
+    # Small nets: below 2^6 rays the device map is the Gray code itself; at 2^6
+    # every index bit is a lane bit; at 2^7 one bit selects the wave.
+    cases += [("c60/2^3", scenes.c60(), 0, 3, 64), ("c60/2^5", scenes.c60(), 5, 5, 64),
+              ("globule650/2^6", scenes.globule(650, 1), 2, 6, 64), ("globule650/2^7", scenes.globule(650, 1), 7, 7, 64)]
     for name, (c, r), seed, power, cap in cases:
         say("solve", name, prec, seed, power, cap)
         res, rays, tr, ro, hits, src = _solve_vs_oracle(oracle, c, r, seed, power, cap, prec)
