@@ -498,6 +498,10 @@ struct CandLists {
 };
 static __shared__ CandLists g_cand;
 
+// Static LDS of the two-pass packet kernels; a staged scene (dynamic LDS) must
+// fit beside it in a workgroup's 64 KB.
+constexpr size_t kPacketStaticLds = sizeof(CandLists);
+
 // Drops the thread's candidates whose lower bound exceeds ut; returns the count kept.
 __device__ __forceinline__ int32_t compact_candidates(int tid, int32_t nc, float ut) {
     int32_t w = 0;

@@ -126,7 +126,10 @@ struct ehss_plan {
 namespace {
 
 constexpr int kPlanWords = kCounterWords + 2;  // per job: counters, then 4 uint32 queue words
-constexpr size_t kPlanLdsLimit = 48 * 1024;     // packet kernels: keeps several blocks per CU
+// Packet kernels: a staged scene of at most this size keeps several blocks per
+// CU and fits beside the kernels' static LDS in a workgroup's 64 KB.
+constexpr size_t kPlanLdsLimit = 40 * 1024;
+static_assert(kPlanLdsLimit + kPacketStaticLds <= 64 * 1024, "packet kernel LDS exceeds 64 KB per workgroup");
 
 void destroy_plan(ehss_plan* p) {
     if (!p) return;

@@ -577,7 +577,10 @@ constexpr int kCrossMinReps = 16;  // replicates per launch from which they are 
 #define EHSS_DRAIN_LDS_KB 144
 #endif
 constexpr size_t kDrainLdsLimit = (size_t)EHSS_DRAIN_LDS_KB * 1024;  // of the 160 KB per CU
-constexpr size_t kSolveLdsLimit = 48 * 1024;   // packet kernels: keeps several blocks per CU
+// Packet kernels: a staged scene of at most this size keeps several blocks per
+// CU and fits beside the kernels' static LDS in a workgroup's 64 KB.
+constexpr size_t kSolveLdsLimit = 40 * 1024;
+static_assert(kSolveLdsLimit + kPacketStaticLds <= 64 * 1024, "packet kernel LDS exceeds 64 KB per workgroup");
 
 // Raises k_drain<1>'s dynamic LDS limit to kDrainLdsLimit (once per process;
 // the library is bound to one device).
