@@ -140,7 +140,7 @@ _SIGS = {
     "ehss_scene_last_rays": (_INT, [_P, _I64, _P, _P, _P]),
     "ehss_scene_last_ray_times": (_INT, [_P, _I64, _P, _P]),
     "ehss_ray_map": (_INT, [ct.POINTER(Sobol), _INT, _P]),
-    "ehss_cross_map": (_INT, [_P, _INT, _INT, _P]),
+    "ehss_cross_map": (_INT, [_P, _INT, _INT, _INT, _P]),
     "ehss_source_rays": (_INT, [_P, ct.POINTER(Sobol), _INT, _P, _P, _P]),
     "ehss_solve_source": (_INT, [_P, ct.POINTER(Source), ct.POINTER(Sobol5), _INT, _INT, _P, _P, _P, _P]),
     "ehss_source_states": (_INT, [_P, ct.POINTER(Source), ct.POINTER(Sobol5), _INT] + [_P] * 7),

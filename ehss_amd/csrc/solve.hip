@@ -19,6 +19,10 @@ namespace ehss {
 
 namespace {
 
+// Counter stripes (see k_drain): kDrainStripes per replicate, kStripeWords words each.
+constexpr int kDrainStripes = 64;
+constexpr int kStripeWords = 8;
+
 // Replicate r of a multi-replicate solve: its slices of the record, counter,
 // histogram and ledger arrays (replicates are laid out one after another).
 __device__ __forceinline__ SolveArgs replicate_args(const SolveArgs& A, int32_t r) {
@@ -47,44 +51,60 @@ __global__ void EHSS_SOLVE_KERNEL k_solve(SceneView S, SolveArgs A, const ehss_s
     flush_counters(ctr, B.counters);
 }
 
-// Cross-replicate schedule (see CrossMap / CrossTables): wave w takes the
-// partition cells c = kCrossCells (w / groups) + j, j < kCrossCells, for
-// replicates 64 (w % groups) + lane.  Each ray is recorded at the slot its
-// replicate's own map gives it, so every replicate's records, and therefore its
-// totals, are those of a separate solve.  Work counters go to the ray's
-// replicate, once per lane.
-constexpr int kCrossCells = 1;
-
+// Cross-replicate schedule (see CrossMap / CrossTables): the replicates share
+// one spatial partition of m - b bits, so each has exactly 2^b points in every
+// cell.  Wave w takes cell c = w / groups for 64 / 2^b replicates (group
+// w % groups): lane l traces point (c, l mod 2^b) of replicate
+// (w % groups) 64 / 2^b + l / 2^b.  b = 0 for many replicates of small nets
+// (one point of each of 64 replicates per cell); b = 6 - log2(reps) for a
+// power-of-two count of at most 64 replicates, so that the wave's rays come
+// from a cell 64 / 2^b times smaller than one replicate's own wave-coherent
+// map gives.  Each ray is recorded at the slot its replicate's own map gives
+// it, so every replicate's records, and therefore its totals, are those of a
+// separate solve.  Work counters go to the ray's replicate, once per group of
+// 2^b lanes.
 template <int kMode, int kPA, int kLds>
 __global__ void EHSS_SOLVE_KERNEL k_solve_x(SceneView S0, SolveArgs A, CrossTables X, int32_t reps,
-                                                 int32_t groups, RayState* queue, uint32_t* queue_count) {
+                                                 int32_t groups, int32_t sub_bits, RayState* queue,
+                                                 uint32_t* queue_count, unsigned long long* stripes) {
     extern __shared__ __align__(16) unsigned char lds[];
     const SceneView S = kLds ? stage_scene<true>(S0, lds) : S0;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    const uint32_t c0 = (uint32_t)(wave / groups) * kCrossCells;
-    const int32_t r = (int32_t)(wave % groups) * 64 + lane;
+    const uint32_t c = __builtin_amdgcn_readfirstlane((uint32_t)(wave / groups));
+    const int32_t r = (int32_t)(wave % groups) * (64 >> sub_bits) + (lane >> sub_bits);
+    const uint32_t jsub = (uint32_t)lane & ((1u << sub_bits) - 1u);
     const int m = A.power;
+    const int cell_bits = m - sub_bits;
     const int32_t rr = r < reps ? r : 0;
+    const bool valid = r < reps && (int64_t)c < (A.n >> sub_bits);
     const SolveArgs B = replicate_args(A, rr);
     uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
-    for (int j = 0; j < kCrossCells; ++j) {
-        const uint32_t c = __builtin_amdgcn_readfirstlane(c0 + (uint32_t)j);
-        const bool valid = r < reps && (int64_t)c < A.n;
-        PointQ p;
-        uint32_t u = X.uoff[r];
-        for (int d = 0; d < 4; ++d) p.q[d] = X.off[d * X.rp + r];
-        for (int k = 0; k < m; ++k) {
-            if (!((c >> k) & 1u)) continue;  // wave-uniform
-            u ^= X.ucol[(int64_t)k * X.rp + r];
-            for (int d = 0; d < 4; ++d) p.q[d] ^= X.col[((int64_t)k * 4 + d) * X.rp + r];
-        }
-        const uint32_t t = m >= 6 ? (((u & ((1u << (m - 6)) - 1u)) << 6) | (u >> (m - 6))) : u;
-        solve_ray_at<kMode, kPA, kLds>(S, B, p, valid, 0u, t, ctr, queue, queue_count, rr);
+    PointQ p;
+    uint32_t u = X.uoff[rr];
+    for (int d = 0; d < 4; ++d) p.q[d] = X.off[d * X.rp + rr];
+    for (int k = 0; k < cell_bits; ++k) {
+        if (!((c >> k) & 1u)) continue;  // wave-uniform
+        u ^= X.ucol[(int64_t)k * X.rp + rr];
+        for (int d = 0; d < 4; ++d) p.q[d] ^= X.col[((int64_t)k * 4 + d) * X.rp + rr];
     }
-    if (r < reps)
-        for (int k = 0; k < 6; ++k)
-            if (ctr[k]) atomicAdd(&B.counters[k], (unsigned long long)ctr[k]);
+    for (int k = 0; k < sub_bits; ++k) {
+        if (!((jsub >> k) & 1u)) continue;
+        const int64_t kk = cell_bits + k;
+        u ^= X.ucol[kk * X.rp + rr];
+        for (int d = 0; d < 4; ++d) p.q[d] ^= X.col[(kk * 4 + d) * X.rp + rr];
+    }
+    const uint32_t t = m >= 6 ? (((u & ((1u << (m - 6)) - 1u)) << 6) | (u >> (m - 6))) : u;
+    solve_ray_at<kMode, kPA, kLds>(S, B, p, valid, 0u, t, ctr, queue, queue_count, rr);
+    // Counters: summed over the lanes of a replicate, then added to one of its
+    // stripes (picked by the wave): every wave of the launch holds the same
+    // replicates, and one address per counter would take an atomic per wave.
+    const int G = 1 << sub_bits;
+    unsigned long long* cs = stripes + ((int64_t)rr * kDrainStripes + wave % kDrainStripes) * kStripeWords;
+    for (int k = 0; k < 6; ++k) {
+        const uint64_t v = G > 1 ? group_sum(ctr[k], G) : ctr[k];
+        if (r < reps && jsub == 0 && v) atomicAdd(&cs[k], (unsigned long long)v);
+    }
 }
 
 // Persistent drain of the rays the two-phase variant queued (every replicate's
@@ -94,10 +114,10 @@ __global__ void EHSS_SOLVE_KERNEL k_solve_x(SceneView S0, SolveArgs A, CrossTabl
 // spheres are first copied into LDS, so every node, leaf and sphere fetch of the
 // drain is an LDS access instead of a dependent L2 round trip.
 // Several replicates: the drain's work counters of replicate r go to
-// stripes[(r * kDrainStripes + s) * 4 + k], stripe s picked by the wave, so
-// that the atomics of a launch spread over many addresses; k_finalize adds a
-// replicate's stripes into its counters.
-constexpr int kDrainStripes = 64;
+// stripes[(r * kDrainStripes + s) * kStripeWords + k], stripe s picked by the
+// wave, so that the atomics of a launch spread over many addresses; k_finalize
+// adds a replicate's stripes into its counters.  (The cross-replicate solve
+// kernel adds its six counters the same way.)
 
 template <int kLds>
 __global__ void EHSS_DRAIN_KERNEL k_drain(SceneView S, SolveArgs A, int32_t replicates, const RayState* queue,
@@ -119,7 +139,7 @@ __global__ void EHSS_DRAIN_KERNEL k_drain(SceneView S, SolveArgs A, int32_t repl
     const int64_t stripe = (int64_t)((blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) % kDrainStripes);
     auto flush_lane = [&]() {
         if (cur < 0) return;
-        unsigned long long* c = stripes + ((int64_t)cur * kDrainStripes + stripe) * 4;
+        unsigned long long* c = stripes + ((int64_t)cur * kDrainStripes + stripe) * kStripeWords;
         for (int k = 0; k < 3; ++k)
             if (ctr[k]) atomicAdd(&c[k], (unsigned long long)ctr[k]);
         ctr[0] = ctr[1] = ctr[2] = 0;
@@ -149,7 +169,7 @@ __global__ void EHSS_DRAIN_KERNEL k_drain(SceneView S, SolveArgs A, int32_t repl
         if (!pending) break;
         const int32_t j = __builtin_amdgcn_readlane(cur, __builtin_ctzll(pending));
         const bool mine = cur == j;
-        unsigned long long* c = stripes + ((int64_t)j * kDrainStripes + stripe) * 4;
+        unsigned long long* c = stripes + ((int64_t)j * kDrainStripes + stripe) * kStripeWords;
         for (int k = 0; k < 3; ++k) {
             const int64_t v = wave_sum_i64(mine ? (int64_t)ctr[k] : 0);
             if (lane == 0 && v) atomicAdd(&c[k], (unsigned long long)v);
@@ -238,9 +258,10 @@ __global__ void __launch_bounds__(256) k_finalize(const Partial* partials, int64
     const int32_t r = (int32_t)blockIdx.x;
     finalize_block(partials + r * nblocks, nblocks,
                    reinterpret_cast<Partial*>(counters + (int64_t)r * kCounterWords + 8));
-    if (stripes && threadIdx.x < 3) {
+    if (stripes && threadIdx.x < 6) {
         unsigned long long sum = 0;
-        for (int s = 0; s < kDrainStripes; ++s) sum += stripes[((int64_t)r * kDrainStripes + s) * 4 + threadIdx.x];
+        for (int s = 0; s < kDrainStripes; ++s)
+            sum += stripes[((int64_t)r * kDrainStripes + s) * kStripeWords + threadIdx.x];
         counters[(int64_t)r * kCounterWords + threadIdx.x] += sum;
     }
 }
@@ -504,10 +525,12 @@ void make_cross_tables(const ehss_sobol* sobs, const CrossMap* maps, int reps, i
 }
 
 // Cross-replicate maps for `reps` scrambles of 2^m points: the finest shared
-// spatial partition (m, m - 1, ... spatial bits) whose rows are independent.
-// False when none of at least m - 6 bits exists or the replicates disagree.
-bool make_cross_maps(const ehss_sobol* sobs, int reps, int m, CrossMap* out) {
-    for (int spatial = m; spatial >= std::max(1, m - 6); --spatial) {
+// spatial partition (m, m - 1, ... spatial bits, or exactly `only` bits when
+// only >= 0) whose rows are independent.  False when none of at least m - 6
+// bits exists or the replicates disagree.
+bool make_cross_maps(const ehss_sobol* sobs, int reps, int m, CrossMap* out, int only = -1) {
+    const int lo = only >= 0 ? only : std::max(1, m - 6);
+    for (int spatial = only >= 0 ? only : m; spatial >= lo; --spatial) {
         // Replicate 0 fixes the partition; the others must share it.
         auto one = [&](int r, std::array<int, 4>* part) -> bool {
             CrossMap& X = out[r];
@@ -572,7 +595,13 @@ namespace {
 // [8..] final Partial.
 constexpr int kDefaultVariant = 4;
 constexpr int kMaxReplicates = 16384;
-constexpr int kCrossMinReps = 16;  // replicates per launch from which they are scheduled across lanes
+constexpr int kCrossMinReps = 16;  // replicates per launch from which small nets are scheduled across lanes
+// Power-of-two replicate counts up to 64 scheduled across lanes with 2^b points
+// of each per wave (k_solve_x); 0: each replicate on its own wave-coherent map.
+#ifndef EHSS_MIXED_CELLS
+#define EHSS_MIXED_CELLS 1
+#endif
+constexpr bool kMixedCells = EHSS_MIXED_CELLS;
 #ifndef EHSS_DRAIN_LDS_KB
 #define EHSS_DRAIN_LDS_KB 144
 #endif
@@ -666,13 +695,33 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     CrossMap* h_cross = reinterpret_cast<CrossMap*>(h_sob + reps);
     RayMap* h_map = reinterpret_cast<RayMap*>(reinterpret_cast<unsigned char*>(h_sob) + base_bytes);
     parallel_for(reps, [&](int r) { h_sob[r] = sobs[r]; });
-    // Many replicates: schedule them across lanes when a shared partition exists.
-    // Measured on the 650-atom globule: the cross schedule wins for small nets
-    // (2^11 rays from 16 replicates, 2^14 from 64) and loses for large ones
-    // (2^14 rays x 16: 158 vs 192 M rays/s; 2^20 x 16: 0.40 vs 1.25 G rays/s).
+    // Many replicates of small nets: one point of each of 64 replicates per cell
+    // when a shared partition exists (measured on the 650-atom globule: box
+    // tests per ray 598 -> 327 at 2^11 rays).  Two to 64 replicates (a power of
+    // two): 2^(6 - log2 reps) points of each per wave, from cells 64 / 2^b times
+    // smaller than one replicate's wave-coherent map gives (16 x 2^20 rays:
+    // 164 -> 126 box tests per ray, 8.2 -> 6.9 ms per launch).  The work
+    // counters of these launches go through the stripes: one address per
+    // counter, taking an atomic from every wave, had made the same launch take
+    // 12.1 ms.
     const bool cross_shape = (power <= 12 && reps >= kCrossMinReps) || (power <= 16 && reps >= 64);
-    const bool cross = cross_shape && power >= 1 && !(flags & EHSS_FLAG_PER_REPLICATE) &&
-                       make_cross_maps(sobs, reps, power, h_cross);
+    // A power-of-two count of 2..64 replicates: 2^b points of each per wave from
+    // one cell of power - b spatial bits (b = 6 - log2 reps), when the
+    // replicates share such a partition.
+    int sub_bits = 0;
+    bool cross = false;
+    if (!(flags & EHSS_FLAG_PER_REPLICATE) && power >= 1) {
+        const int lr = __builtin_ctz((unsigned)reps);
+        if (kMixedCells && reps >= 2 && reps <= 64 && (reps & (reps - 1)) == 0 && power >= 6 - lr + 1) {
+            sub_bits = 6 - lr;
+            cross = make_cross_maps(sobs, reps, power, h_cross, power - sub_bits);
+        }
+        if (!cross) {
+            sub_bits = 0;
+            cross = cross_shape && make_cross_maps(sobs, reps, power, h_cross);
+        }
+    }
+    const int32_t xgroups = cross ? (reps + (64 >> sub_bits) - 1) / (64 >> sub_bits) : 0;
     uint32_t* h_words = reinterpret_cast<uint32_t*>(h_cross + reps);
     if (cross) make_cross_tables(sobs, h_cross, reps, rp, power, h_words);
     else parallel_for(reps, [&](int r) { ray_map_words(sobs[r], power, h_map[r]); });
@@ -744,9 +793,17 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
             spill = sc->queue2.as<RayState>();
         }
     }
+    // Counter stripes of several replicates (cross kernel and drains).
+    unsigned long long* stripes = nullptr;
+    if (reps > 1) {
+        const int64_t words = (int64_t)reps * kDrainStripes * kStripeWords;
+        EHSS_TRY(sc->drain_stripes.reserve(sizeof(unsigned long long) * words));
+        stripes = sc->drain_stripes.as<unsigned long long>();
+        EHSS_TRY(launch_zero(stripes, words, s));
+    }
     EHSS_HIP(hipEventRecord(sc->ev0, s));
-    const int64_t cell_waves = (n + kCrossCells - 1) / kCrossCells;
-    const dim3 xgrid((unsigned)((cell_waves * groups * 64 + 255) / 256));
+    const int64_t cell_waves = n >> sub_bits;
+    const dim3 xgrid((unsigned)((cell_waves * xgroups * 64 + 255) / 256));
     // Packet variants stage the scene in LDS on request (EHSS_FLAG_LDS_PACKETS).
     const size_t stage_bytes = staged_scene_bytes(sc->view, true);
     const bool lds_packets = (variant >= 2) && stage_bytes <= kSolveLdsLimit && (flags & EHSS_FLAG_LDS_PACKETS);
@@ -754,8 +811,8 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
 #define EHSS_LAUNCH_L(M, P, K)                                                                               \
     do {                                                                                                     \
         if (cross)                                                                                           \
-            hipLaunchKernelGGL((k_solve_x<M, P, K>), xgrid, block, solve_lds, s, sc->view, A, XT, reps, groups,   \
-                               qbuf, qcount);                                                                \
+            hipLaunchKernelGGL((k_solve_x<M, P, K>), xgrid, block, solve_lds, s, sc->view, A, XT, reps, xgroups,  \
+                               sub_bits, qbuf, qcount, stripes);                                             \
         else                                                                                                 \
             hipLaunchKernelGGL((k_solve<M, P, K>), grid, block, solve_lds, s, sc->view, A, d_sob, d_map, qbuf,    \
                                qcount);                                                                      \
@@ -779,16 +836,9 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     }
 #undef EHSS_LAUNCH
 #undef EHSS_LAUNCH_L
-    unsigned long long* stripes = nullptr;
     if (variant == 4) {
         const size_t lds = drain_lds_bytes(sc->view);
         const dim3 dgrid((unsigned)drain_blocks(total));
-        if (reps > 1) {
-            const int64_t words = (int64_t)reps * kDrainStripes * 4;
-            EHSS_TRY(sc->drain_stripes.reserve(sizeof(unsigned long long) * words));
-            stripes = sc->drain_stripes.as<unsigned long long>();
-            EHSS_TRY(launch_zero(stripes, words, s));
-        }
         if (lds <= kDrainLdsLimit) EHSS_TRY(set_drain_lds_limit());
         // First drain: A.group lanes per ray; with a spill buffer, rays still
         // bouncing after stage1 queries move to a second drain with whole
@@ -889,7 +939,7 @@ int extend_enqueue(ehss_scene* sc, int new_cap) {
     A.times = nullptr;
     unsigned long long* stripes = nullptr;
     if (reps > 1) {
-        const int64_t sw = (int64_t)reps * kDrainStripes * 4;
+        const int64_t sw = (int64_t)reps * kDrainStripes * kStripeWords;
         EHSS_TRY(sc->drain_stripes.reserve(sizeof(unsigned long long) * sw));
         stripes = sc->drain_stripes.as<unsigned long long>();
         EHSS_TRY(launch_zero(stripes, sw, s));
@@ -1046,12 +1096,13 @@ int drain_queued(ehss_scene* sc, const SolveArgs& A, const RayState* queue, uint
 
 extern "C" {
 
-int ehss_cross_map(const ehss_sobol* sobols, int reps, int power, uint32_t* out) {
+int ehss_cross_map(const ehss_sobol* sobols, int reps, int power, int spatial, uint32_t* out) {
     EHSS_CHECK(sobols && out, "null sobols/out");
     EHSS_CHECK(reps >= 1 && reps <= kMaxReplicates, "replicate count %d out of range 1..%d", reps, kMaxReplicates);
     EHSS_CHECK(power >= 1 && power <= 30, "power %d out of range 1..30", power);
+    EHSS_CHECK(spatial >= 0 && spatial <= power, "spatial bits %d out of range 0..%d", spatial, power);
     std::vector<CrossMap> maps((size_t)reps);
-    if (!make_cross_maps(sobols, reps, power, maps.data()))
+    if (!make_cross_maps(sobols, reps, power, maps.data(), spatial > 0 ? spatial : -1))
         return set_error(EHSS_ERR_INVALID, "no spatial partition shared by the replicates");
     memcpy(out, maps.data(), sizeof(CrossMap) * reps);
     return EHSS_OK;

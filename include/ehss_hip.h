@@ -244,15 +244,19 @@ int ehss_scramble_many(int64_t n, const uint64_t* seeds, ehss_sobol* out);
  * so the 64 rays of a wave share one elementary cell of the scrambled net. */
 int ehss_ray_map(const ehss_sobol* sobol, int power, uint32_t* cols);
 
-/* Host-only: the cross-replicate schedule ehss_solve_many uses from 16
- * replicates up (no reference counterpart).  out: per replicate 68 words --
+/* Host-only: the cross-replicate schedule ehss_solve_many uses (no reference
+ * counterpart): a spatial partition of `spatial` bits shared by the replicates
+ * (0: the finest one with at least power - 6 bits).  out: per replicate 68 words --
  * inv2[32] (columns of T2^-1), rows1[32] (rows of the replicate's own T1),
- * cell_shift, 3 pad.  Lane r of the wave for partition cell c traces Gray index
- * g = T2_r^-1 (c ^ cell_shift_r), whose point lies in cell c, and records it at
- * the slot of g under the replicate's own map.  EHSS_ERR_INVALID if the
- * replicates share no spatial partition (the launch then uses one replicate per
+ * cell_shift, 3 pad.  Index u = c | j << spatial (cell c, sub-index j < 2^(power
+ * - spatial)) is Gray index g = T2_r^-1 (u ^ cell_shift_r), whose point lies in
+ * cell c; it is recorded at the slot of g under the replicate's own map.  A
+ * launch of many replicates of small nets takes one point (index c) of each of
+ * 64 replicates per wave; a launch of 2^k <= 64 replicates takes 2^(6 - k)
+ * points of each, from a partition of power - 6 + k bits.  EHSS_ERR_INVALID if
+ * the replicates share no such partition (the launch then uses one replicate per
  * block row). */
-int ehss_cross_map(const ehss_sobol* sobols, int reps, int power, uint32_t* out);
+int ehss_cross_map(const ehss_sobol* sobols, int reps, int power, int spatial, uint32_t* out);
 
 /* Synchronous solve of 2^power rays.  histogram (cap+1 int64) and ledger
  * (cap+1 double) may be NULL. */

@@ -72,3 +72,24 @@ def test_fp32_filter_kernels_equal_the_plain_kernel(kind, seed):
             np.testing.assert_array_equal(a[1]["contributions"], b[1]["contributions"], err_msg=f"{kind}/{seed}/{key}")
             np.testing.assert_array_equal(a[1]["flags"] & PHYS, b[1]["flags"] & PHYS, err_msg=f"{kind}/{seed}/{key}")
             np.testing.assert_array_equal(a[0]["histogram"], b[0]["histogram"], err_msg=f"{kind}/{seed}/{key}")
+
+
+@pytest.mark.parametrize("atoms", [550, 650, 750, 850])
+def test_lds_staged_packets_near_the_staging_limit(atoms):
+    """The packet kernels stage the scene in LDS beside their candidate lists
+    (static LDS) when the scene is small enough; scenes around that limit
+    (about 700 atoms) run staged or fall back, and either way equal the default
+    kernel bit for bit."""
+    from ehss_amd import molecules
+    xyz, el = molecules.globule(atoms, 40 + atoms)
+    r = molecules.radii_for(el)
+    say("lds staging", atoms, "atoms")
+    with GPUScene(xyz, r) as sc:
+        a = sc.solve(scramble(3), 14, 64, pa=True, keep_rays=True, histogram=True)
+        ra = sc.last_rays(1 << 14)
+        b = sc.solve(scramble(3), 14, 64, pa=True, keep_rays=True, histogram=True, lds=True)
+        rb = sc.last_rays(1 << 14)
+    assert a["omega"] == b["omega"] and a["pa"] == b["pa"]
+    np.testing.assert_array_equal(ra["contributions"], rb["contributions"])
+    np.testing.assert_array_equal(ra["flags"] & PHYS, rb["flags"] & PHYS)
+    np.testing.assert_array_equal(a["histogram"], b["histogram"])

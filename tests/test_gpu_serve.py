@@ -83,12 +83,14 @@ def test_results_are_the_same_alone_and_in_the_mix():
 
 @pytest.mark.parametrize("precision,variant", [("fp64", 0), ("fp32-fixup", 2), ("fp32-fixup", 3),
                                                ("fp32-fixup", 4)])
-@pytest.mark.parametrize("reps", [5, 20, 70])  # 20 and 70: replicates scheduled across lanes
+# 20 and 70: small nets scheduled one point of each replicate per lane; 2, 16
+# and 64: 2^(6 - log2 reps) points of each replicate per wave from one cell.
+@pytest.mark.parametrize("reps", [5, 20, 70, 2, 16, 64])
 def test_replicates_in_one_launch_equal_separate_solves(precision, variant, reps):
     for name, (c, r) in (("globule650", scenes.globule(650, 1)), ("c60", scenes.SCENES["c60"]()),
                          ("single", (np.zeros((1, 3)), np.array([3.1])))):
         with GPUScene(c, r) as s:
-            for power, cap in ((12, 64), (5, 16)):
+            for power, cap in ((12, 64), (5, 16)) + (((16, 64),) if reps == 16 and name != "single" else ()):
                 sobols = [scramble(300 + k) for k in range(reps)]
                 say("replicates", name, precision, variant, power)
                 many = s.solve_many(sobols, power, cap, precision=precision, histogram=True, variant=variant,

@@ -53,19 +53,25 @@ def _points(oracle, seed, g):
     return q
 
 
-@pytest.mark.parametrize("power", [3, 7, 11, 14])
-def test_cross_replicate_schedule(oracle, power):
-    """Every replicate's (cell -> Gray index) is a bijection, its slot is the one
-    its own map gives, and the 64 replicates of a wave share one spatial cell."""
+@pytest.mark.parametrize("power,sub", [(3, 0), (7, 0), (11, 0), (14, 0), (11, 2), (14, 4), (9, 5), (16, 2)])
+def test_cross_replicate_schedule(oracle, power, sub):
+    """Every replicate's (index -> Gray index) is a bijection, its slot is the one
+    its own map gives, and all points with one cell index -- across replicates
+    and sub-indices -- lie in one cell of the shared partition.  sub = 0: the
+    finest partition (one point of each replicate per cell, the schedule of many
+    replicates of small nets); sub = b > 0: a partition of power - b bits, 2^b
+    points of each replicate per cell (the schedule of 2^(6 - b) replicates)."""
     seeds = [3 + 7 * k for k in range(20)]
     sobs = (L.Sobol * len(seeds))(*[scramble(s) for s in seeds])
     out = np.zeros((len(seeds), 68), np.uint32)
-    L.call("ehss_cross_map", sobs, len(seeds), power, L.ptr(out))
+    L.call("ehss_cross_map", sobs, len(seeds), power, power - sub if sub else 0, L.ptr(out))
     n = 1 << power
     c = np.arange(n, dtype=np.uint64)
     cells = []
     for r, seed in enumerate(seeds):
         inv2, rows1, shift = out[r, :32], out[r, 32:64], np.uint64(out[r, 64])
+        if sub:
+            assert int(shift) < (1 << (power - sub))  # the digital shift touches cell bits only
         u = c ^ shift
         g = np.zeros(n, np.uint64)
         for k in range(power):
@@ -86,10 +92,13 @@ def test_cross_replicate_schedule(oracle, power):
         own = rays_of(scramble(seed), power)
         assert np.array_equal(own[t.astype(np.int64)], g)
         cells.append(_points(oracle, seed, g))
-    # replicates at the same cell index: their points agree on a common set of top bits
-    q = np.stack(cells)  # (reps, n, 4)
-    spread = [np.ptp(q[:, :, d].astype(np.float64) / 2 ** 30, axis=0).max() for d in range(4)]
-    assert np.prod(np.maximum(spread, 1e-12)) <= 2.0 ** -(power - 6) + 1e-12
+    # index = cell | sub-index << (power - sub): the points of one cell (every
+    # replicate, every sub-index) share the partition's top bits, so their spread
+    # in the four coordinates has volume at most 2^-(power - sub)
+    q = np.stack(cells).astype(np.float64) / 2 ** 30  # (reps, n, 4)
+    q = q.reshape(len(seeds), 1 << sub, n >> sub, 4).transpose(0, 1, 3, 2).reshape(-1, 4, n >> sub)
+    spread = np.ptp(q, axis=0)  # (4, cells)
+    assert np.prod(np.maximum(spread, 1e-12), axis=0).max() <= 2.0 ** -(power - sub) * (1 + 1e-9)
 
 
 @pytest.mark.parametrize("power", [5, 11, 14])
@@ -99,9 +108,9 @@ def test_cross_maps_of_many_replicates_equal_one_at_a_time(power):
     sob_list = [scramble(11 + 13 * k) for k in range(40)] * 13  # 520 replicates
     sobs = (L.Sobol * len(sob_list))(*sob_list)
     many = np.zeros((len(sob_list), 68), np.uint32)
-    L.call("ehss_cross_map", sobs, len(sob_list), power, L.ptr(many))
+    L.call("ehss_cross_map", sobs, len(sob_list), power, 0, L.ptr(many))
     for r in range(40):
         one = np.zeros((1, 68), np.uint32)
-        L.call("ehss_cross_map", ct.byref(sobs[r]), 1, power, L.ptr(one))
+        L.call("ehss_cross_map", ct.byref(sobs[r]), 1, power, 0, L.ptr(one))
         for k in range(13):
             np.testing.assert_array_equal(many[r + 40 * k], one[0])
