@@ -41,7 +41,7 @@ METRIC = "EHSS traced rays/s + CCS/s at 1% RMSE on 1 MI355X; p50/p99 serve laten
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 POWER, CAP, SEED = 20, 64, 1
-STEPS_PER_LAUNCH = 16     # 2^20-ray steps per launch: 1.85 G rays/s (2 in flight) vs 0.55 one per launch
+STEPS_PER_LAUNCH = 32     # 2^20-ray steps per launch (2 in flight): 2.59 G rays/s at 16, 2.71 at 32, 2.74 at 64
 CCS_PER_LAUNCH = 8192     # 2,048-ray CCS estimates per launch (0.68 M CCS/s with two launches in flight)
 COPY_BYTES = 256 << 20    # calibration copy of the traffic probe (bytes read = bytes written)
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_solve_globule650.txt"
@@ -380,7 +380,7 @@ def traffic_probe(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=512)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="fp32-fixup", choices=["fp32-fixup", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -484,7 +484,7 @@ def main():
     full = [ms for ms, r in zip(kernel_ms, launch_rays) if r == rays_per_launch]
     avg_kernel_s = float(np.mean(full)) / 1e3
     achieved = fpr * rays_per_launch / avg_kernel_s / 1e12
-    kernels = ("k_solve<4,1,0> + 2 x k_drain<1>" if args.precision == "fp32-fixup" else "k_solve<0,1,0>")
+    kernels = ("k_solve_x<4,1,0> + 2 x k_drain<1>" if args.precision == "fp32-fixup" else "k_solve_x<0,1,0>")
     roofline = dict(bound="valu", achieved=round(achieved, 4), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=round(achieved / FP32_PEAK_TFLOPS, 6), traffic=read_traffic(),
                     kernel=kernels, kernel_ms=round(float(np.mean(full)), 4), rays_per_launch=rays_per_launch,
