@@ -28,8 +28,8 @@ from typing import Callable
 
 DEFAULT_SPACE = {
     "packets": (3, 2, 4, 5),    # wave-uniform (packet) queries before the drain
-    "group": (64, 8, 4, 1),     # lanes per ray in the first drain (64 / group rays per wavefront)
-    "stage1": (6, 2, 4, 12, 0),  # queries before a ray spills to whole-wave lanes (0: one drain)
+    "group": (None, 64, 8, 4, 2, 1),  # lanes per ray in the first drain (None: chosen by the drain from its queue)
+    "stage1": (6, 2, 4, 12, 0),  # queries before a ray spills to the second drain (0: one drain)
     "coop": (8, 4, 16),         # live rays per wave from which the tail is finished cooperatively
     "lds": (False, True),       # packet kernels read the scene staged in LDS
     "leaf_size": (4, 2, 8),     # spheres per BVH leaf group
@@ -71,11 +71,11 @@ def _store(path: Path, data: dict) -> None:
 
 
 def _norm(cfg: dict) -> dict:
-    """Drop settings that have no effect: stage1 only matters with a grouped drain
-    of fewer than 64 lanes."""
+    """Drop settings that have no effect: stage1 does not matter with a first
+    drain of 64 lanes per ray."""
     out = dict(cfg)
-    if out.get("group") in (None, 64) and "stage1" in out:
-        out["stage1"] = None if out.get("group") is None else 0
+    if out.get("group") == 64 and "stage1" in out:
+        out["stage1"] = 0
     return out
 
 
@@ -205,7 +205,7 @@ def tune_plan(structures, power: int, cap: int, sobols, precision: str = "fp32-f
     # Start from the library's own choice for this many rays, spelled out, so
     # that the stage1 axis is explored around it.
     at_size = len(structures) << power >= 1 << 18
-    start = {"packets": 3, "group": 4 if at_size else 64, "stage1": 6 if at_size else 0, "coop": 8, "lds": False,
+    start = {"packets": 3, "group": None if at_size else 64, "stage1": 6 if at_size else 0, "coop": 8, "lds": False,
              "leaf_size": 4}
     try:
         res = tune(measure, space=space, start=start, repeats=repeats, passes=passes, fingerprint=fingerprint, key=key,

@@ -727,6 +727,146 @@ __device__ __forceinline__ void nearest_group(const SceneView& S, const V3& o, c
     group_merge_query(q, G);
 }
 
+// Two-pass form of nearest_group (drain kernels, at most 256 threads per
+// block).  The group's lanes split the leaf boxes as nearest_group does, but a
+// sphere of an entered leaf gets only the FP32 test of nearest_packet_2p: the
+// lane keeps (slot, tlo) when tlo <= U + tol, U being the smallest upper bound
+// thi of a robust hit so far, shared over the group after every round of
+// 4 G leaf boxes.  The FP64 re-tests then run in lock step, each lane on its
+// own j-th candidate, skipping those above the group's final U + tol; the
+// group merges as before.  nearest_group instead runs each lane's FP64 re-tests
+// inside its own scan, so the wave issues the FP64 block at nearly every sphere
+// position some lane reaches.  A lane whose kGroupCand rows are full after
+// dropping what its bound excludes re-tests them at once and starts over (a
+// superset of the needed re-tests, so the answer is unchanged).  Every sphere
+// whose FP64 entry lies within best + tol is re-tested on some lane (the
+// argument of nearest_packet_2p), so the answer is nearest_group's.
+#ifndef EHSS_GROUP_CANDIDATES
+#define EHSS_GROUP_CANDIDATES 4
+#endif
+constexpr int kGroupCand = EHSS_GROUP_CANDIDATES;
+struct GroupCandLists {
+    int2 e[kGroupCand][256];
+};
+static __shared__ GroupCandLists g_gcand;
+
+// Minimum over the G lanes of an aligned group, on every lane of the group.
+__device__ __forceinline__ float group_min(float v, int G) {
+    for (int off = G >> 1; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+template <class C = uint64_t>
+__device__ __forceinline__ void nearest_group_2p(const SceneView& S, const V3& o, const V3& d, int32_t last,
+                                                 double tol, int G, int sub, Query& q, C& boxes, C& spheres,
+                                                 C& retests) {
+    query_init(q);
+    const Ray32 r = ray32(S, o, d);
+    const int tid = (int)threadIdx.x;
+    const float ftol = (float)tol * 1.01f;
+    const float plim = S.m_p + ftol;
+    const float k2 = 2.01f * ftol;
+    float U = INFINITY;          // FP32 upper bound of the FP64 winning entry
+    float ut = 3.40282347e+38f;  // >= U + tol (FLT_MAX while U is +inf)
+    float bestf = INFINITY;      // box pruning bound
+    int32_t nc = 0;
+    // The lane's candidates with tlo above ut: dropped; if the rows are still
+    // full, re-tested now.
+    auto make_room = [&]() {
+        int32_t w = 0;
+        for (int32_t j = 0; j < nc; ++j) {
+            const int2 c = g_gcand.e[j][tid];
+            if (!(__int_as_float(c.y) > ut)) g_gcand.e[w++][tid] = c;
+        }
+        nc = w;
+        if (nc < kGroupCand) return;
+        for (int32_t j = 0; j < nc; ++j) {
+            const int2 c = g_gcand.e[j][tid];
+            ++retests;
+            eval64(o, d, S.sphs[c.x], S.ids[c.x], tol, q, nullptr);
+        }
+        nc = 0;
+    };
+    auto filter_leaf = [&](int32_t first, int32_t count) {
+        for (int32_t k = first; k < first + count; ++k) {
+            const int32_t wanted = S.ids[k];
+            if (wanted == last) continue;
+            ++spheres;
+            const float4 s = S.sph32[k];
+            const float x0 = r.ox - s.x, x1 = r.oy - s.y, x2 = r.oz - s.z;
+            const float p = __builtin_fmaf(x2, r.dz, __builtin_fmaf(x1, r.dy, x0 * r.dx));
+            const float q0 = __builtin_fmaf(-p, r.dx, x0), q1 = __builtin_fmaf(-p, r.dy, x1),
+                        q2 = __builtin_fmaf(-p, r.dz, x2);
+            const float disc = __builtin_fmaf(s.w, s.w, -__builtin_fmaf(q2, q2, __builtin_fmaf(q1, q1, q0 * q0)));
+            const float mdisc = __builtin_fmaf(s.w, k2, S.m_disc);
+            if (!(p < plim && disc >= -mdisc)) continue;  // behind or a clear miss
+            const float xh = fmaxf(disc, 0.f) + S.m_disc;
+            const float ri = __builtin_amdgcn_rsqf(xh);
+            const float tlo = (-p - S.m_p) - xh * ri * (1.0f + 0x1p-20f);
+            const float a = S.m_p - p;
+            const float thi0 = (a - (disc - S.m_disc) * ri * (1.0f - 0x1p-20f)) + a * 0x1p-21f;
+            float thi = disc > S.m_disc ? thi0 : INFINITY;
+            thi = tlo > 0.f ? thi : INFINITY;
+            if (tlo <= ut) {
+                if (nc == kGroupCand) make_room();
+                g_gcand.e[nc++][tid] = make_int2(k, __float_as_int(tlo));
+            }
+            U = fminf(U, thi);
+            ut = upper_sum(U, ftol);
+            bestf = ut * (1.0f + 4.0f * kU32) + S.m_box;
+        }
+    };
+    const int32_t ng = S.n_groups;
+    if (ng <= kBruteGroups) {
+        // Rounds of 4 G leaf boxes (the same count on every lane), four loads in
+        // flight per lane; the bound is shared over the group after each round.
+        for (int32_t gb = 0; gb < ng; gb += 4 * G) {
+            const int32_t g0 = gb + sub;
+            Node32 leaf[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (g0 + u * G < ng) leaf[u] = S.leaves32[g0 + u * G];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (g0 + u * G >= ng) break;
+                ++boxes;
+                if (box32(r, leaf[u], bestf)) filter_leaf(leaf[u].end, leaf[u].leaf);
+            }
+            U = group_min(U, G);
+            ut = upper_sum(U, ftol);
+            bestf = ut * (1.0f + 4.0f * kU32) + S.m_box;
+        }
+    } else {
+        for (int32_t e = sub; e < S.n_frontier; e += G) {
+            int32_t node = S.frontier[e];
+            const int32_t stop = S.nodes32[node].end;
+            while (node < stop) {
+                const Node32 n = S.nodes32[node];
+                ++boxes;
+                if (!box32(r, n, bestf)) {
+                    node = n.end;
+                    continue;
+                }
+                const int32_t g = n.leaf;
+                if (g >= 0) filter_leaf(S.offsets[g], S.offsets[g + 1] - S.offsets[g]);
+                ++node;
+            }
+        }
+        U = group_min(U, G);
+        ut = upper_sum(U, ftol);
+    }
+    for (int32_t j = 0;; ++j) {
+        const bool act = j < nc;
+        if (__ballot(act) == 0) break;
+        const int2 c = g_gcand.e[act ? j : 0][tid];
+        if (act && !(__int_as_float(c.y) > ut)) {
+            ++retests;
+            eval64(o, d, S.sphs[c.x], S.ids[c.x], tol, q, nullptr);
+        }
+    }
+    group_merge_query(q, G);
+}
+
 // The whole wave on one ray.
 template <class C = uint64_t>
 __device__ __forceinline__ void nearest_coop(const SceneView& S, const V3& o, const V3& d, int32_t last, double tol,

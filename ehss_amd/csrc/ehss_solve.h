@@ -395,14 +395,42 @@ inline int coop_from_flags(unsigned flags) {
 }
 
 // Drain shape from flags: lanes per ray of the first drain (bits 19..21,
-// log2 + 1) and the queries a ray gets there before it moves to the whole-wave
-// second drain (bits 22..25, q + 1; 0 queries = one drain).  Defaults (field
-// 0): from 2^18 rays, where the queue fills the chip, 4 lanes per ray for 6
-// queries, then the second drain; below, one drain with a wave per ray.
+// log2 + 1) and the queries a ray gets there before it moves to the second
+// drain (bits 22..25, q + 1; 0 queries = one drain) with kStage2Group lanes per
+// ray.  Defaults (field 0): from 2^18 rays, where the queue fills the chip,
+// kAutoGroup (the drain picks its lanes per ray from its queue length,
+// drain_group_for) for 6 queries, then the second drain; below, one drain with
+// a wave per ray.  Lanes per ray of a grouped drain are otherwise a power of
+// two in 2..64; 1 is the per-lane drain.
+constexpr int kAutoGroup = 0;
+#ifndef EHSS_DRAIN_GROUP
+#define EHSS_DRAIN_GROUP 0
+#endif
 inline int group_from_flags(unsigned flags, int64_t rays) {
     const int f = (int)((flags >> EHSS_GROUP_SHIFT) & 0x7u);
-    if (f == 0) return rays >= (1 << 18) ? 4 : 64;
+    if (f == 0) return rays >= (1 << 18) ? EHSS_DRAIN_GROUP : 64;
     return 1 << (f - 1);
+}
+
+// Lanes per ray of the second drain stage (the long paths) and of the drain of
+// continued rays (kAutoGroup: from the queue length).
+#ifndef EHSS_STAGE2_GROUP
+#define EHSS_STAGE2_GROUP 0
+#endif
+constexpr int kStage2Group = EHSS_STAGE2_GROUP;
+static_assert(kStage2Group == kAutoGroup ||
+                  (kStage2Group >= 2 && kStage2Group <= kWave && (kStage2Group & (kStage2Group - 1)) == 0),
+              "second-stage lanes per ray: kAutoGroup or a power of two in 2..64");
+
+// Lanes per ray of a drain left to choose (kAutoGroup): the fewest, at least 2,
+// with which its `len` queued rays cover all `lanes` lanes of its grid.  A long
+// queue is then drained for throughput (a two-pass grouped query costs less per
+// ray with fewer lanes), a short one for latency (more lanes per ray shorten
+// each query of the long paths that set its length).  Wave-uniform.
+__device__ __forceinline__ int drain_group_for(uint32_t len, uint32_t lanes) {
+    int G = 2;
+    while (G < kWave && (uint64_t)len * (uint64_t)G < lanes) G <<= 1;
+    return G;
 }
 
 // Packet queries of variant 4 (bits 26..29; 0 = default).  Measured with the
@@ -753,6 +781,13 @@ __device__ __forceinline__ uint64_t group_sum(uint64_t v, int G) {
     return v;
 }
 
+// Grouped drain queries: the two-pass form (nearest_group_2p) unless built
+// with EHSS_GROUP_TWO_PASS=0.
+#ifndef EHSS_GROUP_TWO_PASS
+#define EHSS_GROUP_TWO_PASS 1
+#endif
+constexpr bool kGroupTwoPass = EHSS_GROUP_TWO_PASS != 0;
+
 // Queue drain with G lanes per ray (G >= 2).  Every lane of a group holds the
 // group's ray state and applies the same query result to it, so the copies stay
 // identical; each query is a nearest_group over the group's lanes.  An idle
@@ -763,7 +798,7 @@ __device__ __forceinline__ uint64_t group_sum(uint64_t v, int G) {
 // With max_queries > 0 a ray that is still bouncing after that many queries in
 // this drain is appended to `spill` (count in *spill_count) instead, for a
 // second drain with more lanes per ray: the first drain keeps its lanes on the
-// many short paths, the second gives the few long ones whole waves.
+// many short paths, the second gives the few long ones more lanes each.
 template <class SceneOf, class ArgsOf, class Account>
 __device__ __forceinline__ void drain_groups(const RayState* queue, uint32_t queue_len, uint32_t* queue_head,
                                              int G, SceneOf scene_of, ArgsOf args_of, Account account,
@@ -799,7 +834,8 @@ __device__ __forceinline__ void drain_groups(const RayState* queue, uint32_t que
         if (have) {
             const SceneView& S = scene_of(job);
             Query q;
-            nearest_group(S, R.o, R.d, R.last, R.tol, G, sub, q, boxes, spheres, retests);
+            if constexpr (kGroupTwoPass) nearest_group_2p(S, R.o, R.d, R.last, R.tol, G, sub, q, boxes, spheres, retests);
+            else nearest_group(S, R.o, R.d, R.last, R.tol, G, sub, q, boxes, spheres, retests);
             const SolveArgs& A = args_of(job);
             ray_process(S, A.cap, R, q);
             if (!R.done && ++queries == max_queries) {
@@ -984,7 +1020,7 @@ __device__ __forceinline__ void finalize_block(const Partial* partials, int64_t 
 // written on the device, words[1..3] = head, second-stage count and head, all
 // zero on entry) to their ends on stream s with the fused solve's drain
 // schedule for `capacity` rays and `flags` (A.group lanes per ray for the
-// first-stage queries, then a whole wave per ray), recording each ray at its
+// first-stage queries, then kStage2Group lanes per ray), recording each ray at its
 // slot through A.  Defined in solve.hip.
 int drain_queued(ehss_scene* sc, const SolveArgs& A, const RayState* queue, uint32_t* words, int64_t capacity,
                  unsigned flags, hipStream_t s);

@@ -72,6 +72,7 @@ __global__ void EHSS_PLAN_DRAIN_KERNEL k_plan_drain(const PlanJob* __restrict__ 
     if (head >= len) return;
     const SceneView S = J.view;
     const SolveArgs A = J.args;
+    if (group == kAutoGroup) group = drain_group_for(len, gridDim.x * blockDim.x);
     uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
     auto scene_of = [&](int32_t) -> const SceneView& { return S; };
     auto args_of = [&](int32_t) -> const SolveArgs& { return A; };
@@ -179,9 +180,9 @@ int enqueue_plan(ehss_plan* p) {
         auto drain = [&](int stage, int group, int max_queries) {
             hipLaunchKernelGGL(k_plan_drain, dgrid, block, 0, s, jd, stage, group, max_queries);
         };
-        const bool two = p->stage1 > 0 && p->group > 1;
+        const bool two = p->stage1 > 0 && p->group != 1;
         drain(0, p->group, two ? p->stage1 : 0);
-        if (two) drain(1, kWave, 0);
+        if (two) drain(1, kStage2Group, 0);
     }
     hipLaunchKernelGGL(k_plan_reduce, dim3((unsigned)p->max_red, (unsigned)p->n_jobs), block, 0, s, jd);
     hipLaunchKernelGGL(k_plan_finalize, dim3((unsigned)p->n_jobs), block, 0, s, jd);
@@ -245,7 +246,7 @@ int fill_plan(ehss_plan* p, int n_jobs, ehss_scene* const* scenes, const ehss_so
     EHSS_TRY(p->bins.reserve(sizeof(unsigned long long) * 2 * bins));
     if (p->variant == 4) {
         EHSS_TRY(p->queue.reserve(sizeof(RayState) * rays));
-        if (p->stage1 > 0 && p->group > 1) EHSS_TRY(p->queue2.reserve(sizeof(RayState) * rays));
+        if (p->stage1 > 0 && p->group != 1) EHSS_TRY(p->queue2.reserve(sizeof(RayState) * rays));
     }
     EHSS_TRY(p->jobs_dev.reserve(sizeof(PlanJob) * n_jobs));
     unsigned long long* W = p->words.as<unsigned long long>();
@@ -286,7 +287,7 @@ int fill_plan(ehss_plan* p, int n_jobs, ehss_scene* const* scenes, const ehss_so
         J.qwords = reinterpret_cast<uint32_t*>(W + (int64_t)kPlanWords * j + kCounterWords);
         if (p->variant == 4) {
             J.queue = p->queue.as<RayState>() + ray_off[j];
-            if (p->stage1 > 0 && p->group > 1) J.spill = p->queue2.as<RayState>() + ray_off[j];
+            if (p->stage1 > 0 && p->group != 1) J.spill = p->queue2.as<RayState>() + ray_off[j];
         }
     }
     int64_t most_rays = 0;

@@ -129,6 +129,7 @@ __global__ void EHSS_DRAIN_KERNEL k_drain(SceneView S, SolveArgs A, int32_t repl
     const uint32_t len = __builtin_amdgcn_readfirstlane(*queue_count);
     if (len == 0) return;
     const SceneView L = kLds ? stage_scene(S, lds) : S;
+    if (group == kAutoGroup) group = drain_group_for(len, gridDim.x * blockDim.x);
     auto scene_of = [&](int32_t) -> const SceneView& { return L; };
     auto args_of = [&](int32_t j) -> SolveArgs { return replicate_args(A, j); };
     // Per-lane sums.  One replicate: flushed once per wave at the end.  Several:
@@ -606,6 +607,8 @@ constexpr bool kMixedCells = EHSS_MIXED_CELLS;
 #define EHSS_DRAIN_LDS_KB 144
 #endif
 constexpr size_t kDrainLdsLimit = (size_t)EHSS_DRAIN_LDS_KB * 1024;  // of the 160 KB per CU
+static_assert(kDrainLdsLimit + sizeof(GroupCandLists) <= 160 * 1024,
+              "staged scene and the grouped queries' candidate rows exceed a workgroup's LDS");
 // Packet kernels: a staged scene of at most this size keeps several blocks per
 // CU and fits beside the kernels' static LDS in a workgroup's 64 KB.
 constexpr size_t kSolveLdsLimit = 40 * 1024;
@@ -788,7 +791,7 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     if (variant == 4) {
         EHSS_TRY(sc->queue.reserve(sizeof(RayState) * total));
         qbuf = sc->queue.as<RayState>();
-        if (stage1 > 0 && A.group > 1) {
+        if (stage1 > 0 && A.group != 1) {
             EHSS_TRY(sc->queue2.reserve(sizeof(RayState) * total));
             spill = sc->queue2.as<RayState>();
         }
@@ -841,8 +844,8 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
         const dim3 dgrid((unsigned)drain_blocks(total));
         if (lds <= kDrainLdsLimit) EHSS_TRY(set_drain_lds_limit());
         // First drain: A.group lanes per ray; with a spill buffer, rays still
-        // bouncing after stage1 queries move to a second drain with whole
-        // waves per ray.
+        // bouncing after stage1 queries move to a second drain with
+        // kStage2Group lanes per ray.
         auto drain = [&](const RayState* in, const uint32_t* count, uint32_t* head, int group, int max_q,
                          RayState* out, uint32_t* out_count) {
             if (lds <= kDrainLdsLimit)
@@ -853,7 +856,7 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
                                    max_q, out, out_count, stripes);
         };
         drain(qbuf, qcount, qcount + 1, A.group, spill ? stage1 : 0, spill, scount);
-        if (spill) drain(spill, scount, scount + 1, kWave, 0, nullptr, nullptr);
+        if (spill) drain(spill, scount, scount + 1, kStage2Group, 0, nullptr, nullptr);
     }
     EHSS_HIP(hipGetLastError());
     EHSS_HIP(hipEventRecord(sc->ev1, s));
@@ -884,7 +887,7 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
 }
 
 // Continues the scene's last continuable solve at new_cap (ehss_solve_extend):
-// the saved unresolved rays are drained with a whole wave per ray (they are
+// the saved unresolved rays are drained with kStage2Group lanes per ray (they are
 // the long, trapped paths), rays unresolved again are saved into the other
 // queue, and every replicate's records are reduced again.  Enqueued like a
 // solve; solve_collect waits for it.
@@ -928,7 +931,7 @@ int extend_enqueue(ehss_scene* sc, int new_cap) {
     A.contrib = sc->ray_contrib.as<double>();
     A.counters = ctr;
     A.coop = coop_from_flags(sc->cont_flags);
-    A.group = kWave;
+    A.group = kStage2Group;
     A.packets = packets_from_flags(sc->cont_flags);
     A.pad_ = 0;
     A.trig = trig_compat();
@@ -951,10 +954,10 @@ int extend_enqueue(ehss_scene* sc, int new_cap) {
     if (lds <= kDrainLdsLimit) {
         EHSS_TRY(set_drain_lds_limit());
         hipLaunchKernelGGL(k_drain<1>, dgrid, dim3(256), lds, s, sc->view, A, reps, q, cw + 2 * in, cw + 2 * in + 1,
-                           kWave, 0, nullptr, nullptr, stripes);
+                           kStage2Group, 0, nullptr, nullptr, stripes);
     } else {
         hipLaunchKernelGGL(k_drain<0>, dgrid, dim3(256), 0, s, sc->view, A, reps, q, cw + 2 * in, cw + 2 * in + 1,
-                           kWave, 0, nullptr, nullptr, stripes);
+                           kStage2Group, 0, nullptr, nullptr, stripes);
     }
     EHSS_HIP(hipGetLastError());
     EHSS_HIP(hipEventRecord(sc->ev1, s));
@@ -1070,7 +1073,7 @@ int drain_queued(ehss_scene* sc, const SolveArgs& A, const RayState* queue, uint
                  unsigned flags, hipStream_t s) {
     const int stage1 = stage1_queries(flags, capacity);
     RayState* spill = nullptr;
-    if (stage1 > 0 && A.group > 1) {
+    if (stage1 > 0 && A.group != 1) {
         EHSS_TRY(sc->queue2.reserve(sizeof(RayState) * capacity));
         spill = sc->queue2.as<RayState>();
     }
@@ -1087,7 +1090,7 @@ int drain_queued(ehss_scene* sc, const SolveArgs& A, const RayState* queue, uint
                                out, out_count, nullptr);
     };
     drain(queue, words, words + 1, A.group, spill ? stage1 : 0, spill, words + 2);
-    if (spill) drain(spill, words + 2, words + 3, kWave, 0, nullptr, nullptr);
+    if (spill) drain(spill, words + 2, words + 3, kStage2Group, 0, nullptr, nullptr);
     EHSS_HIP(hipGetLastError());
     return EHSS_OK;
 }
@@ -1154,7 +1157,7 @@ int ehss_solve_noalloc(ehss_scene* sc, const ehss_sobol* sobol, int power, int c
     EHSS_CHECK(variant >= 0 && variant <= 4, "unknown kernel variant %d", variant);
     const int group = group_from_flags(flags, n);
     const int stage1 = stage1_queries(flags, n);
-    EHSS_CHECK(!(variant == 4 && stage1 > 0 && group > 1),
+    EHSS_CHECK(!(variant == 4 && stage1 > 0 && group != 1),
                "the allocation-free solve has no second drain queue (EHSS_STAGE1 with EHSS_GROUP_LOG2 > 0)");
     unsigned char* base = static_cast<unsigned char*>(noalloc_scratch());
     if (!base) return set_error(EHSS_ERR_HIP, "the allocation-free work array has no address on this device");

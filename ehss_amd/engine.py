@@ -311,7 +311,8 @@ class GPUScene:
         `coop` is the tail threshold (a wave with at most that many live rays
         finishes them with wave-cooperative queries; 0 off, None default);
         `group` is the lanes per ray of variant 4's drain, `stage1` the queries a
-        ray gets there before a whole wave takes it, `packets` the packet queries
+        ray gets there before the second drain takes it (None: the library's choice; from 2^18 rays the
+        drains pick their lanes per ray from their queue lengths), `packets` the packet queries
         before the drain.  All settings return identical per-ray results."""
         totals = L.Totals()
         hist = np.zeros(cap + 1, np.int64) if histogram else None

@@ -9,7 +9,7 @@ from ehss_amd import autotune
 def separable_cost(cfg):
     """Unique minimum at packets 4, group 4, stage1 2, coop 16, lds False, leaf 8."""
     c = {None: 0.3, 3: 0.3, 2: 0.6, 4: 0.1, 5: 0.2}[cfg["packets"]]
-    c += {64: 3.0, 8: 2.0, 4: 1.0, 1: 4.0, None: 3.0}[cfg["group"]]
+    c += {64: 3.0, 8: 2.0, 4: 1.0, 2: 1.5, 1: 4.0, None: 3.0}[cfg["group"]]
     c += {None: 0.5, 0: 0.7, 1: 0.4, 2: 0.1, 4: 0.3, 6: 0.2, 12: 0.35}[cfg["stage1"]]
     c += {None: 0.2, 8: 0.2, 4: 0.3, 16: 0.05}[cfg["coop"]]
     c += 0.0 if not cfg["lds"] else 0.25
@@ -66,9 +66,9 @@ def test_a_configuration_that_changes_results_is_refused():
         autotune.tune(separable_cost, repeats=1, fingerprint=fingerprint)
 
 
-def test_stage1_is_ignored_without_a_grouped_drain():
+def test_stage1_is_ignored_with_a_whole_wave_first_drain():
     assert autotune._norm({"group": 64, "stage1": 2}) == {"group": 64, "stage1": 0}
-    assert autotune._norm({"group": None, "stage1": 2}) == {"group": None, "stage1": None}
+    assert autotune._norm({"group": None, "stage1": 2}) == {"group": None, "stage1": 2}
     assert autotune._norm({"group": 4, "stage1": 2}) == {"group": 4, "stage1": 2}
 
 
