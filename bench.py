@@ -41,7 +41,7 @@ METRIC = "EHSS traced rays/s + CCS/s at 1% RMSE on 1 MI355X; p50/p99 serve laten
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 POWER, CAP, SEED = 20, 64, 1
-STEPS_PER_LAUNCH = 32     # 2^20-ray steps per launch (2 in flight): 2.59 G rays/s at 16, 2.71 at 32, 2.74 at 64
+STEPS_PER_LAUNCH = 64     # 2^20-ray steps per launch (2 in flight): 3.19 G rays/s at 32, 3.37 at 64
 CCS_PER_LAUNCH = 8192     # 2,048-ray CCS estimates per launch (0.68 M CCS/s with two launches in flight)
 COPY_BYTES = 256 << 20    # calibration copy of the traffic probe (bytes read = bytes written)
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_solve_globule650.txt"
@@ -380,7 +380,7 @@ def traffic_probe(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=1024)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="fp32-fixup", choices=["fp32-fixup", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
