@@ -114,7 +114,10 @@ __device__ __forceinline__ bool box32_near(const Ray32& r, const Node32& n, floa
     const float tz0 = __builtin_fmaf(n.lo[2], r.iz, -r.oiz), tz1 = __builtin_fmaf(n.hi[2], r.iz, -r.oiz);
     near = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.f));
     const float far = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
-    return near <= far && near <= bestf;
+    // near <= far && near <= bestf as one comparison (the same for all
+    // non-NaN values; ray32 keeps every reciprocal finite, so no slab distance
+    // is NaN, and neither is bestf).
+    return near <= fminf(far, bestf);
 }
 
 __device__ __forceinline__ bool box32(const Ray32& r, const Node32& n, float bestf) {
@@ -589,8 +592,11 @@ __device__ __forceinline__ void nearest_packet_2p(const SceneView& S, const V3& 
         float bestf = INFINITY;
         packet_walk<kLds>(S, r, live, bestf, boxes, [&](int32_t g, uint64_t mask) {
             if (!lane_in(mask)) return;
-            const int32_t k1 = M::offset(S, g + 1);
-            for (int32_t k = M::offset(S, g); k < k1; ++k) {
+            const int32_t k0 = M::offset(S, g), k1 = M::offset(S, g + 1);
+            // Every sphere of the leaf is tested on the lane (the last atom's test
+            // runs in lock step with the others and is then discarded).
+            spheres += (uint32_t)(k1 - k0);
+            for (int32_t k = k0; k < k1; ++k) {
                 const int32_t wanted = M::id(S, k);
                 const float4 s = M::sphere(S, k);
                 const float x0 = r.ox - s.x, x1 = r.oy - s.y, x2 = r.oz - s.z;
@@ -600,7 +606,6 @@ __device__ __forceinline__ void nearest_packet_2p(const SceneView& S, const V3& 
                 const float disc = __builtin_fmaf(s.w, s.w, -__builtin_fmaf(q2, q2, __builtin_fmaf(q1, q1, q0 * q0)));
                 const float mdisc = __builtin_fmaf(s.w, k2, S.m_disc);
                 const bool other = wanted != last;
-                count_lanes(spheres, 1, other);
                 // Excluded on every lane (the common case): one uniform branch.
                 const bool keep = other && p < plim && disc >= -mdisc;
                 if (__ballot(keep) == 0) continue;
