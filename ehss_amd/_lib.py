@@ -30,6 +30,7 @@ FLAG_TIMESTAMPS = 8
 FLAG_LDS_PACKETS = 16
 FLAG_PER_REPLICATE = 32
 FLAG_CONTINUABLE = 64
+FLAG_UNSORTED = 128
 VARIANT_SHIFT = 8
 COOP_SHIFT = 12
 GROUP_SHIFT = 19

@@ -259,7 +259,7 @@ struct ehss_scene {
     ehss::HostBuf host_table;  // scrambles and ray maps of the pending launch (staging)
     // solve workspace (grow-only)
     ehss::DevBuf ray_info, ray_contrib, partials, hist, ledger, counters, queue, queue2, ray_times, tables,
-        drain_stripes, custom_groups;
+        drain_stripes, custom_groups, sort_bins, sort_order;
     ehss::DevBuf src_weights, src_cdf;  // alternative-source solves: per-ray weights, searchsorted table
     ehss::BuildWork build_work;  // tree-build work arrays, reused by ehss_scene_rebuild
     // continuation of the last solve (EHSS_FLAG_CONTINUABLE): two queues of

@@ -51,6 +51,117 @@ __global__ void EHSS_SOLVE_KERNEL k_solve(SceneView S, SolveArgs A, const ehss_s
     flush_counters(ctr, B.counters);
 }
 
+// Sorted schedule of launches of many replicates of small nets.  The shared
+// partition gives every replicate exactly one point per cell, so its cells
+// cannot be finer than one replicate's net (2^-11 of the unit cube at 2^11
+// rays), however many replicates a launch holds.  Here the launch's rays are
+// instead ordered by the 4-D cell of their Sobol points: the top kSortBits bits
+// of each 30-bit coordinate word, interleaved from the top (Morton order), by a
+// counting sort over the kSortBins cells.  The scatter writes each ray's Sobol
+// words, replicate and slot bits at its sorted position, so wave w of the
+// packet kernel reads rays 64 w .. 64 w + 63 with two coalesced loads; they
+// come from about (64 / rays per cell) neighbouring cells of 2^-20 of the cube.
+// Which rays share a wave does not change any ray's result (each is traced
+// exactly), and records stay at their replicate's slots, so every replicate's
+// totals are those of a separate solve.
+constexpr int kSortBits = 5;
+constexpr int64_t kSortBins = 1LL << (4 * kSortBits);
+constexpr int kSortChunk = 1024;  // bins per block of the local scan (256 threads x 4)
+constexpr int kSortBlocks = (int)(kSortBins / kSortChunk);
+static_assert(kSortBlocks <= 1024, "top-level scan runs in one block of 1024 threads");
+
+__device__ __forceinline__ uint32_t cell_key(const PointQ& p) {
+    uint32_t key = 0;
+    for (int j = 0; j < kSortBits; ++j)
+        for (int d = 0; d < 4; ++d) key = (key << 1) | ((p.q[d] >> (29 - j)) & 1u);
+    return key;
+}
+
+// Sobol words p and slot bits u of point c of replicate r under the cross
+// tables.  The sort passes number the launch's rays replicate-fastest (ray g is
+// point g / reps of replicate g mod reps): a wave's table loads are coalesced,
+// and its points share a cell of the shared partition, so its bin atomics land
+// close together.
+__device__ __forceinline__ void sort_point(const CrossTables& X, int32_t reps, int32_t bits, uint32_t g, int32_t& r,
+                                           PointQ& p, uint32_t& u) {
+    r = (int32_t)(g % (uint32_t)reps);
+    const uint32_t c = g / (uint32_t)reps;
+    u = X.uoff[r];
+    for (int d = 0; d < 4; ++d) p.q[d] = X.off[d * X.rp + r];
+    for (int k = 0; k < bits; ++k) {
+        if (!((c >> k) & 1u)) continue;
+        u ^= X.ucol[(int64_t)k * X.rp + r];
+        for (int d = 0; d < 4; ++d) p.q[d] ^= X.col[((int64_t)k * 4 + d) * X.rp + r];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sort_count(CrossTables X, int32_t reps, int32_t bits, int64_t total,
+                                                    uint32_t* bins) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    int32_t r;
+    PointQ p;
+    uint32_t u;
+    sort_point(X, reps, bits, (uint32_t)g, r, p, u);
+    atomicAdd(&bins[cell_key(p)], 1u);
+}
+
+// Exclusive scan of each chunk of kSortChunk bins in place; chunk totals to tops.
+__global__ void __launch_bounds__(256) k_sort_scan_local(uint32_t* bins, uint32_t* tops) {
+    __shared__ uint32_t s[256];
+    const int t = threadIdx.x;
+    uint4* q = reinterpret_cast<uint4*>(bins + (int64_t)blockIdx.x * kSortChunk) + t;
+    const uint4 v = *q;
+    const uint32_t sum = v.x + v.y + v.z + v.w;
+    s[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t a = t >= off ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += a;
+        __syncthreads();
+    }
+    uint4 o;
+    o.x = s[t] - sum;
+    o.y = o.x + v.x;
+    o.z = o.y + v.y;
+    o.w = o.z + v.z;
+    *q = o;
+    if (t == 255) tops[blockIdx.x] = s[255];
+}
+
+// Exclusive scan of the kSortBlocks chunk totals (one block).
+__global__ void __launch_bounds__(1024) k_sort_scan_top(uint32_t* tops) {
+    __shared__ uint32_t s[1024];
+    const int t = threadIdx.x;
+    const uint32_t v = t < kSortBlocks ? tops[t] : 0u;
+    s[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t a = t >= off ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += a;
+        __syncthreads();
+    }
+    if (t < kSortBlocks) tops[t] = s[t] - v;
+}
+
+// Ray g's words, replicate and slot bits at its sorted position (the order
+// within a cell is whatever the atomics give).
+__global__ void __launch_bounds__(256) k_sort_scatter(CrossTables X, int32_t reps, int32_t bits, int64_t total,
+                                                      uint32_t* bins, const uint32_t* tops, uint4* sq, uint2* sru) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    int32_t r;
+    PointQ p;
+    uint32_t u;
+    sort_point(X, reps, bits, (uint32_t)g, r, p, u);
+    const uint32_t key = cell_key(p);
+    const uint32_t at = tops[key / kSortChunk] + atomicAdd(&bins[key], 1u);
+    sq[at] = make_uint4(p.q[0], p.q[1], p.q[2], p.q[3]);
+    sru[at] = make_uint2((uint32_t)r, u);
+}
+
 // Cross-replicate schedule (see CrossMap / CrossTables): the replicates share
 // one spatial partition of m - b bits, so each has exactly 2^b points in every
 // cell.  Wave w takes cell c = w / groups for 64 / 2^b replicates (group
@@ -66,34 +177,55 @@ __global__ void EHSS_SOLVE_KERNEL k_solve(SceneView S, SolveArgs A, const ehss_s
 template <int kMode, int kPA, int kLds>
 __global__ void EHSS_SOLVE_KERNEL k_solve_x(SceneView S0, SolveArgs A, CrossTables X, int32_t reps,
                                                  int32_t groups, int32_t sub_bits, RayState* queue,
-                                                 uint32_t* queue_count, unsigned long long* stripes) {
+                                                 uint32_t* queue_count, unsigned long long* stripes,
+                                                 const uint4* sq, const uint2* sru) {
     extern __shared__ __align__(16) unsigned char lds[];
     const SceneView S = kLds ? stage_scene<true>(S0, lds) : S0;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    const uint32_t c = __builtin_amdgcn_readfirstlane((uint32_t)(wave / groups));
-    const int32_t r = (int32_t)(wave % groups) * (64 >> sub_bits) + (lane >> sub_bits);
-    const uint32_t jsub = (uint32_t)lane & ((1u << sub_bits) - 1u);
     const int m = A.power;
-    const int cell_bits = m - sub_bits;
+    int32_t r;
+    uint32_t jsub = 0;
+    bool valid;
+    PointQ p;
+    uint32_t u;
+    if (sq) {
+        // Sorted schedule (sub_bits 0): the words of the launch's i-th ray in
+        // cell order (k_sort_scatter).
+        const int64_t i = wave * 64 + lane;
+        valid = i < ((int64_t)reps << m);
+        const uint4 q = valid ? sq[i] : make_uint4(0u, 0u, 0u, 0u);
+        const uint2 ru = valid ? sru[i] : make_uint2(0u, 0u);
+        p.q[0] = q.x;
+        p.q[1] = q.y;
+        p.q[2] = q.z;
+        p.q[3] = q.w;
+        u = ru.y;
+        r = valid ? (int32_t)ru.x : reps;
+    } else {
+        const uint32_t c = __builtin_amdgcn_readfirstlane((uint32_t)(wave / groups));
+        r = (int32_t)(wave % groups) * (64 >> sub_bits) + (lane >> sub_bits);
+        jsub = (uint32_t)lane & ((1u << sub_bits) - 1u);
+        const int cell_bits = m - sub_bits;
+        const int32_t r0 = r < reps ? r : 0;
+        valid = r < reps && (int64_t)c < (A.n >> sub_bits);
+        u = X.uoff[r0];
+        for (int d = 0; d < 4; ++d) p.q[d] = X.off[d * X.rp + r0];
+        for (int k = 0; k < cell_bits; ++k) {
+            if (!((c >> k) & 1u)) continue;  // wave-uniform
+            u ^= X.ucol[(int64_t)k * X.rp + r0];
+            for (int d = 0; d < 4; ++d) p.q[d] ^= X.col[((int64_t)k * 4 + d) * X.rp + r0];
+        }
+        for (int k = 0; k < sub_bits; ++k) {
+            if (!((jsub >> k) & 1u)) continue;
+            const int64_t kk = cell_bits + k;
+            u ^= X.ucol[kk * X.rp + r0];
+            for (int d = 0; d < 4; ++d) p.q[d] ^= X.col[(kk * 4 + d) * X.rp + r0];
+        }
+    }
     const int32_t rr = r < reps ? r : 0;
-    const bool valid = r < reps && (int64_t)c < (A.n >> sub_bits);
     const SolveArgs B = replicate_args(A, rr);
     uint64_t ctr[6] = {0, 0, 0, 0, 0, 0};
-    PointQ p;
-    uint32_t u = X.uoff[rr];
-    for (int d = 0; d < 4; ++d) p.q[d] = X.off[d * X.rp + rr];
-    for (int k = 0; k < cell_bits; ++k) {
-        if (!((c >> k) & 1u)) continue;  // wave-uniform
-        u ^= X.ucol[(int64_t)k * X.rp + rr];
-        for (int d = 0; d < 4; ++d) p.q[d] ^= X.col[((int64_t)k * 4 + d) * X.rp + rr];
-    }
-    for (int k = 0; k < sub_bits; ++k) {
-        if (!((jsub >> k) & 1u)) continue;
-        const int64_t kk = cell_bits + k;
-        u ^= X.ucol[kk * X.rp + rr];
-        for (int d = 0; d < 4; ++d) p.q[d] ^= X.col[(kk * 4 + d) * X.rp + rr];
-    }
     const uint32_t t = m >= 6 ? (((u & ((1u << (m - 6)) - 1u)) << 6) | (u >> (m - 6))) : u;
     solve_ray_at<kMode, kPA, kLds>(S, B, p, valid, 0u, t, ctr, queue, queue_count, rr);
     // Counters: summed over the lanes of a replicate, then added to one of its
@@ -597,6 +729,8 @@ namespace {
 constexpr int kDefaultVariant = 4;
 constexpr int kMaxReplicates = 16384;
 constexpr int kCrossMinReps = 16;  // replicates per launch from which small nets are scheduled across lanes
+constexpr int kSortMinReps = 256;  // replicates per launch from which their rays are sorted by cell
+constexpr int64_t kSortMaxRays = 1LL << 26;  // sorted launches hold 24 B per ray of sorted words
 // Power-of-two replicate counts up to 64 scheduled across lanes with 2^b points
 // of each per wave (k_solve_x); 0: each replicate on its own wave-coherent map.
 #ifndef EHSS_MIXED_CELLS
@@ -807,6 +941,29 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     EHSS_HIP(hipEventRecord(sc->ev0, s));
     const int64_t cell_waves = n >> sub_bits;
     const dim3 xgrid((unsigned)((cell_waves * xgroups * 64 + 255) / 256));
+    // Many replicates of small nets: order the launch's rays by cell (see
+    // k_sort_count) when a wave would otherwise cover a cell coarser than
+    // 2^-20 of the cube by a factor of four or more.
+    const uint4* sq = nullptr;
+    const uint2* sru = nullptr;
+    if (cross && sub_bits == 0 && reps >= kSortMinReps && total <= kSortMaxRays && !(flags & EHSS_FLAG_UNSORTED)) {
+        const size_t bin_bytes = sizeof(uint32_t) * (kSortBins + kSortBlocks);
+        EHSS_TRY(sc->sort_bins.reserve(bin_bytes));
+        EHSS_TRY(sc->sort_order.reserve((sizeof(uint4) + sizeof(uint2)) * total));
+        uint32_t* bins = sc->sort_bins.as<uint32_t>();
+        uint32_t* tops = bins + kSortBins;
+        uint4* q = sc->sort_order.as<uint4>();
+        uint2* ru = reinterpret_cast<uint2*>(q + total);
+        const unsigned tb = (unsigned)((total + 255) / 256);
+        EHSS_TRY(launch_zero(sc->sort_bins.as<unsigned long long>(), (int64_t)(bin_bytes / 8), s));
+        hipLaunchKernelGGL(k_sort_count, dim3(tb), dim3(256), 0, s, XT, reps, power, total, bins);
+        hipLaunchKernelGGL(k_sort_scan_local, dim3(kSortBlocks), dim3(256), 0, s, bins, tops);
+        hipLaunchKernelGGL(k_sort_scan_top, dim3(1), dim3(1024), 0, s, tops);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(tb), dim3(256), 0, s, XT, reps, power, total, bins, tops, q, ru);
+        EHSS_HIP(hipGetLastError());
+        sq = q;
+        sru = ru;
+    }
     // Packet variants stage the scene in LDS on request (EHSS_FLAG_LDS_PACKETS).
     const size_t stage_bytes = staged_scene_bytes(sc->view, true);
     const bool lds_packets = (variant >= 2) && stage_bytes <= kSolveLdsLimit && (flags & EHSS_FLAG_LDS_PACKETS);
@@ -815,7 +972,7 @@ int solve_enqueue(ehss_scene* sc, int reps, const ehss_sobol* sobs, int power, i
     do {                                                                                                     \
         if (cross)                                                                                           \
             hipLaunchKernelGGL((k_solve_x<M, P, K>), xgrid, block, solve_lds, s, sc->view, A, XT, reps, xgroups,  \
-                               sub_bits, qbuf, qcount, stripes);                                             \
+                               sub_bits, qbuf, qcount, stripes, sq, sru);                                    \
         else                                                                                                 \
             hipLaunchKernelGGL((k_solve<M, P, K>), grid, block, solve_lds, s, sc->view, A, d_sob, d_map, qbuf,    \
                                qcount);                                                                      \

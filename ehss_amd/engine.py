@@ -332,10 +332,12 @@ class GPUScene:
                    histogram: bool = False, variant: int = 0, coop: int | None = None, group: int | None = None,
                    keep_rays: bool = False, stage1: int | None = None, lds: bool = False,
                    per_replicate: bool = False, packets: int | None = None, as_array: bool = False,
-                   continuable: bool = False):
+                   continuable: bool = False, unsorted: bool = False):
         """len(sobols) independent solves in one launch; one result dict per
         scramble, each equal to solve() with that scramble.  per_replicate:
-        never schedule the replicates across the lanes of a wave.  as_array: the
+        never schedule the replicates across the lanes of a wave.  unsorted: with
+        256 or more replicates of small nets, keep the shared-partition schedule
+        instead of ordering the launch's rays by 4-D cell.  as_array: the
         totals as one NumPy structured array (fields of ehss_totals) instead of
         dicts, for launches of thousands of replicates.  `sobols` may be a
         prebuilt sobol_table()."""
@@ -350,6 +352,7 @@ class GPUScene:
             (int(variant) << L.VARIANT_SHIFT) | coop_flags(coop, group, stage1, packets) | \
             (L.FLAG_LDS_PACKETS if lds else 0)
         flags |= (L.FLAG_PER_REPLICATE if per_replicate else 0) | (L.FLAG_CONTINUABLE if continuable else 0)
+        flags |= L.FLAG_UNSORTED if unsorted else 0
         L.call("ehss_solve_many", self.handle, reps, table, int(power), int(cap), precision_code(precision), flags,
                totals, L.ptr(hist), L.ptr(ledger))
         if as_array:

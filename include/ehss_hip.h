@@ -57,6 +57,8 @@ extern "C" {
 #define EHSS_FLAG_PER_REPLICATE 32u /* solve_many: never schedule replicates across the lanes of a wave */
 #define EHSS_FLAG_CONTINUABLE 64u /* keep the records and the state of every ray that ends unresolved, so that
                                      ehss_solve_extend can continue exactly those rays at a larger cap */
+#define EHSS_FLAG_UNSORTED 128u /* solve_many of >= 256 replicates of small nets: keep the shared-partition
+                                   schedule instead of ordering the launch's rays by 4-D cell (same results) */
 /* Kernel variant of the FP32 filter path in flag bits 8..11 (0 = default):
  * 1 one ray per lane, 2 wave-uniform first query then per-lane bounces,
  * 3 wave-uniform traversal for every bounce, 4 wave-uniform first queries, then

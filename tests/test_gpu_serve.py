@@ -109,6 +109,33 @@ def test_replicates_in_one_launch_equal_separate_solves(precision, variant, reps
                         np.testing.assert_array_equal(first["bounces"], rays["bounces"])
 
 
+# 256 and more replicates of small nets: the launch's rays are ordered by the
+# 4-D cell of their Sobol points before the packet kernel; 300 is not a whole
+# number of waves of replicates, and 2^3 x 4096 leaves most cells empty.
+@pytest.mark.parametrize("reps,power", [(256, 11), (300, 8), (4096, 3), (8192, 11)])
+def test_sorted_schedule_equals_unsorted_and_separate_solves(reps, power):
+    c, r = scenes.globule(650, 1)
+    with GPUScene(c, r) as s:
+        sobols = [scramble(7000 + k) for k in range(reps)]
+        say("sorted schedule", reps, power)
+        srt = s.solve_many(sobols, power, 64, histogram=True, keep_rays=True)
+        rays_srt = s.last_rays(1 << power)  # the first replicate's records
+        uns = s.solve_many(sobols, power, 64, histogram=True, keep_rays=True, unsorted=True)
+        rays_uns = s.last_rays(1 << power)
+        for k in range(reps):
+            for key in KEYS:
+                assert srt[k][key] == uns[k][key], (reps, power, k, key)
+            np.testing.assert_array_equal(srt[k]["histogram"], uns[k]["histogram"])
+            np.testing.assert_array_equal(srt[k]["ledger"], uns[k]["ledger"])
+        for f in ("flags", "contributions", "bounces"):
+            np.testing.assert_array_equal(rays_srt[f], rays_uns[f], err_msg=f)
+        for k in (0, reps // 2, reps - 1):
+            one = s.solve(sobols[k], power, 64, histogram=True)
+            for key in KEYS:
+                assert srt[k][key] == one[key], (reps, power, k, key)
+            np.testing.assert_array_equal(srt[k]["histogram"], one["histogram"])
+
+
 @pytest.mark.parametrize("power,reps", [(12, 20), (15, 16)])
 def test_per_replicate_schedule_equals_separate_solves(power, reps):
     """EHSS_FLAG_PER_REPLICATE (and the automatic choice at 2^15 x 16, which is
