@@ -42,7 +42,8 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 POWER, CAP, SEED = 20, 64, 1
 STEPS_PER_LAUNCH = 64     # 2^20-ray steps per launch (2 in flight): 3.19 G rays/s at 32, 3.37 at 64
-CCS_PER_LAUNCH = 8192     # 2,048-ray CCS estimates per launch (0.68 M CCS/s with two launches in flight)
+CCS_PER_LAUNCH = 8192     # 2,048-ray CCS estimates per launch
+CCS_STREAMS = 3           # CCS launches in flight, each on its own scene copy and stream
 COPY_BYTES = 256 << 20    # calibration copy of the traffic probe (bytes read = bytes written)
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_solve_globule650.txt"
 VALU_FILE = ROOT / "profiles" / "valu_issue_solve_globule650.md"
@@ -512,16 +513,18 @@ def main():
         say("CCS at 1% RMSE")
         power_ccs, rel = rays_per_ccs(scene, scramble, replicate_seed)
         per = CCS_PER_LAUNCH
-        ccs_seeds = np.array([replicate_seed(7, k) for k in range(4 * per)], dtype=np.uint64)
+        ccs_seeds = np.array([replicate_seed(7, k) for k in range(12 * per)], dtype=np.uint64)
         seq_fast = ccs_rate_sequential(scene, ccs_seeds[:400], power_ccs, "fp32-fixup")
         seq_plain = ccs_rate_sequential(scene, ccs_seeds[:100], power_ccs, "fp64")
         ccs_rate_replicated(scene, ccs_seeds[:per], power_ccs, "fp32-fixup", per)
-        rep_fast = ccs_rate_replicated(scene, ccs_seeds, power_ccs, "fp32-fixup", per)
+        rep_fast = ccs_rate_replicated(scene, ccs_seeds[:4 * per], power_ccs, "fp32-fixup", per)
         rep_plain = ccs_rate_replicated(scene, ccs_seeds[:per], power_ccs, "fp64", per)
-        copies = [GPUScene(centers, radii) for _ in range(2)]
-        ccs_rate_overlapped(copies, ccs_seeds[:2 * per], power_ccs, "fp32-fixup", per)
+        # Launches in flight on their own streams, over 12 launches (6 for the
+        # plain kernel) so that the first and last launches' ramps weigh little.
+        copies = [GPUScene(centers, radii) for _ in range(CCS_STREAMS)]
+        ccs_rate_overlapped(copies, ccs_seeds[:CCS_STREAMS * per], power_ccs, "fp32-fixup", per)
         ovl_fast = ccs_rate_overlapped(copies, ccs_seeds, power_ccs, "fp32-fixup", per)
-        ovl_plain = ccs_rate_overlapped(copies, ccs_seeds[:2 * per], power_ccs, "fp64", per)
+        ovl_plain = ccs_rate_overlapped(copies, ccs_seeds[:6 * per], power_ccs, "fp64", per)
         for c in copies:
             c.close()
         extras["globule650"] = dict(
@@ -537,7 +540,7 @@ def main():
             ccs_per_s_plain_fp64_single_solves=seq_plain, ccs_per_s_fp32_fixup_single_solves=seq_fast,
             replicates_per_launch=per,
             ccs_per_s_plain_fp64_one_launch=rep_plain, ccs_per_s_fp32_fixup_one_launch=rep_fast,
-            ccs_per_s_plain_fp64_2_streams=ovl_plain, ccs_per_s_fp32_fixup_2_streams=ovl_fast,
+            streams=CCS_STREAMS, ccs_per_s_plain_fp64_streams=ovl_plain, ccs_per_s_fp32_fixup_streams=ovl_fast,
             batched_fp32_fixup_vs_plain_fp64_kernel=best_fast / seq_plain,
             batched_fp32_fixup_vs_batched_plain_fp64=best_fast / max(rep_plain, ovl_plain),
             rays_per_s_at_ccs_size=best_fast * (1 << power_ccs))
@@ -621,7 +624,7 @@ def main():
         if "ccs_1pct" in extras:
             cc = extras["ccs_1pct"]
             cpu_ccs = cpu["value"] / cc["rays_per_ccs"]
-            best = max(cc["ccs_per_s_fp32_fixup_one_launch"], cc["ccs_per_s_fp32_fixup_2_streams"])
+            best = max(cc["ccs_per_s_fp32_fixup_one_launch"], cc["ccs_per_s_fp32_fixup_streams"])
             ratios.update(cpu_ccs_per_s=cpu_ccs, ccs_per_s_vs_cpu=best / cpu_ccs)
         extras["vs_cpu_baseline"] = ratios
     line = dict(metric=METRIC, value=value, unit="rays/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
