@@ -378,6 +378,26 @@ def traffic_probe(args):
     return 0
 
 
+def ccs_probe(args):
+    """The command profiled for the CCS-size launch: `--steps` rounds of one
+    launch of CCS_PER_LAUNCH replicates of 2^11 rays (cap 64, no PA) in each of
+    four forms: FP32+fixup sorted, FP32+fixup unsorted (EHSS_FLAG_UNSORTED),
+    plain FP64 sorted, plain FP64 unsorted."""
+    from ehss_amd import molecules
+    from ehss_amd.engine import GPUScene, scramble_many, replicate_seed
+    centers, radii, _ = molecules.workload("globule650")
+    seeds = np.array([replicate_seed(7, k) for k in range(CCS_PER_LAUNCH)], dtype=np.uint64)
+    with GPUScene(centers, radii) as scene:
+        for _ in range(args.steps):
+            for precision in ("fp32-fixup", "fp64"):
+                for unsorted in (False, True):
+                    t = time.perf_counter()
+                    scene.solve_many(scramble_many(seeds), 11, CAP, precision=precision, pa=False, as_array=True,
+                                     unsorted=unsorted)
+                    say(f"ccs probe {precision} unsorted={unsorted}: {1e3 * (time.perf_counter() - t):.2f} ms")
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -387,6 +407,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the side measurements")
     ap.add_argument("--traffic-probe", action="store_true")
+    ap.add_argument("--ccs-probe", action="store_true")
     ap.add_argument("--hw-queues", type=int, default=12,
                     help="GPU_MAX_HW_QUEUES for this process (1..16)")
     args = ap.parse_args()
@@ -404,6 +425,8 @@ def main():
         return 2
     if args.traffic_probe:
         return traffic_probe(args)
+    if args.ccs_probe:
+        return ccs_probe(args)
 
     from ehss_amd import _lib as L
     from ehss_amd import molecules, serve
